@@ -1,0 +1,144 @@
+/*
+ * primecount_mi355x.h -- C ABI of the MI355X hard-special-leaf engine.
+ *
+ * The engine computes Gourdon's D(x, y, z, k) and the Deleglise-Rivat
+ * S2_hard(x, y, z, c) on one AMD Instinct MI355X (gfx950) with hand-written
+ * HIP kernels.  Results are bit-exact with primecount's CPU implementation.
+ *
+ * Conventions
+ *   - Every function returns PCG_OK (0) or a negative PCG_E_* status; on error
+ *     pcg_last_error() returns a message (thread-local, valid until the next
+ *     call on the same thread).  This mirrors the reference C API, which
+ *     catches primecount_error and returns -1 (src/api_c.cpp:19-56).
+ *   - 128-bit integers cross the ABI as pcg_i128 {lo, hi} (two's complement,
+ *     hi holds bits 64..127), because C has no portable 128-bit type.
+ *   - Callers pass scalars and caller-owned output pointers only; the library
+ *     owns every device buffer (tables, work queues), as the reference's D()
+ *     owns its FactorTableD/PiTable (src/gourdon/D.cpp:443-446).
+ */
+#ifndef PRIMECOUNT_MI355X_H
+#define PRIMECOUNT_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pcg_i128 {
+  uint64_t lo;
+  int64_t hi;
+} pcg_i128;
+
+enum {
+  PCG_OK = 0,
+  PCG_E_INVALID = -1,      /* argument outside the supported domain       */
+  PCG_E_DEVICE = -2,       /* HIP runtime / device error                  */
+  PCG_E_NOMEM = -3,        /* device or host allocation failed            */
+  PCG_E_INTERNAL = -4      /* any other failure                           */
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* pcg_last_error(void);
+
+/* Library version string, e.g. "0.1.0 (gfx950)". */
+const char* pcg_version(void);
+
+/* 1 if a gfx950 device is visible and the kernels loaded, else 0. */
+int pcg_device_available(void);
+
+/* ---- hot path: hard special leaves ---------------------------------- */
+
+/* Gourdon's D(x, y, z, k) for x <= 2^63 - 1.
+ * Replaces: int64_t D(int64_t x, int64_t y, int64_t z, int64_t k,
+ *                     int threads, bool is_print)
+ *           include/gourdon.hpp:28 (src/gourdon/D.cpp:425-452). */
+int pcg_D_i64(int64_t x, int64_t y, int64_t z, int64_t k, int64_t* out);
+
+/* Gourdon's D(x, y, z, k) for 128-bit x (x <= y * 2^62).
+ * Replaces: int128_t D(int128_t x, int64_t y, int64_t z, int64_t k,
+ *                      int threads, bool is_print)
+ *           include/gourdon.hpp:38 (src/gourdon/D.cpp:456-505). */
+int pcg_D_i128(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);
+
+/* Deleglise-Rivat S2_hard(x, y, z, c) for x <= 2^63 - 1.
+ * Replaces: int64_t S2_hard(int64_t x, int64_t y, int64_t z, int64_t c,
+ *                           int threads, bool is_print)
+ *           include/S.hpp:25 (src/deleglise-rivat/S2_hard.cpp:221-248). */
+int pcg_S2_hard_i64(int64_t x, int64_t y, int64_t z, int64_t c, int64_t* out);
+
+/* Deleglise-Rivat S2_hard(x, y, z, c) for 128-bit x.
+ * Replaces: int128_t S2_hard(int128_t x, ...)  include/S.hpp:32
+ *           (src/deleglise-rivat/S2_hard.cpp:252-293). */
+int pcg_S2_hard_i128(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);
+
+/* D(x) with primecount's default (y, z, k): the CLI's `primecount x --D`.
+ * Replaces: maxint_t D(maxint_t x, int threads)  src/app/main.cpp:205-246. */
+int pcg_D(pcg_i128 x, pcg_i128* out);
+
+/* S2_hard(x) with primecount's default (y, z, c): `primecount x --S2-hard`.
+ * Replaces: maxint_t S2_hard(maxint_t x, int threads)  src/app/main.cpp:435-461. */
+int pcg_S2_hard(pcg_i128 x, pcg_i128* out);
+
+/* ---- parameter selection (host, exact IEEE double as the reference) -- */
+
+/* x -> (y, z, k, x_star) of Gourdon's algorithm.
+ * Restates: src/util.cpp:276-349 (get_alpha_gourdon), 362-386
+ * (get_x_star_gourdon), src/app/main.cpp:205-246, src/PhiTiny.hpp:59-63. */
+int pcg_gourdon_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* k, int64_t* x_star);
+
+/* x -> (y, z, c) of the Deleglise-Rivat algorithm.
+ * Restates: src/util.cpp:222-263, src/app/main.cpp:435-461. */
+int pcg_dr_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* c);
+
+/* Tuning-factor overrides; values < 1 restore the default formula.
+ * Replace: set_alpha_y / set_alpha_z / set_alpha  src/util.cpp:103-133. */
+int pcg_set_alpha_y(double alpha_y);
+int pcg_set_alpha_z(double alpha_z);
+int pcg_set_alpha(double alpha);
+
+/* ---- batched queries: one persistent launch over a device work queue -- */
+
+/* D(x_i) with default parameters for n values of x.  All queries share one
+ * persistent kernel launch; results land in out[i].  Replaces n calls of
+ * maxint_t D(maxint_t x, int threads) (src/app/main.cpp:205-246). */
+int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out);
+
+/* ---- engine configuration and statistics ----------------------------- */
+
+/* Segment size in 64-bit sieve words (240 numbers each); a power of two in
+ * [256, 4096].  0 selects the autotuned default.  Segment size never changes
+ * results (replaces the segment sizing of src/LoadBalancerS2.cpp:65-110). */
+int pcg_set_segment_words(uint32_t words);
+
+/* Host threads used by the table builders (0 = all hardware threads). */
+int pcg_set_host_threads(int threads);
+
+typedef struct pcg_stats {
+  double seconds_total;      /* wall time of the last D/S2_hard call          */
+  double seconds_tables;     /* host table build + upload                     */
+  double seconds_kernel;     /* sieve kernel (HIP events)                     */
+  double seconds_combine;    /* carry scan + combine kernels (HIP events)     */
+  uint64_t segments;         /* sieve segments processed                      */
+  uint64_t chunks;           /* work-queue items                              */
+  uint64_t segment_words;    /* 64-bit words per segment                      */
+  uint64_t sieve_limit;      /* x/z (D) or z (S2_hard)                        */
+  uint64_t b_first, b_last;  /* sieving-prime index range                     */
+  uint64_t leaves;           /* hard special leaves summed                    */
+  uint64_t hits;             /* cross-off hits (prime multiples visited)      */
+  uint64_t factor_entries;   /* factor-table entries scanned                  */
+  uint64_t pairs;            /* (segment, sieving prime) steps                */
+} pcg_stats;
+
+/* Statistics of the last D/S2_hard call on this thread. */
+int pcg_last_stats(pcg_stats* out);
+
+/* Release every device buffer the library holds. */
+int pcg_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PRIMECOUNT_MI355X_H */

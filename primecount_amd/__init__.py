@@ -1,0 +1,229 @@
+"""primecount_amd -- MI355X engine for primecount's hard special leaves.
+
+Python mirror of the reference's interface for this path:
+
+    D(x, y, z, k)          primecount::D            include/gourdon.hpp:28,38
+    S2_hard(x, y, z, c)    primecount::S2_hard      include/S.hpp:25,32
+    D_default(x)           `primecount x --D`       src/app/main.cpp:205-246
+    S2_hard_default(x)     `primecount x --S2-hard` src/app/main.cpp:435-461
+    gourdon_params(x)      y, z, k, x_star          src/util.cpp:276-386
+    dr_params(x)           y, z, c                  src/util.cpp:222-263
+
+Integers of any size up to 127 bits are accepted; results are Python ints.
+The 64-bit overloads are used when x < 2^63 (as the reference's CLI does,
+src/app/main.cpp:242-245); D128/S2_hard128 force the 128-bit entry points.
+Every call goes through the C ABI of libprimecount_mi355x.so (built by
+primecount_amd.build); there is no CPU fallback: without the library, or
+without a gfx950 device, calls raise PrimecountError.
+"""
+import ctypes
+import os
+
+__all__ = [
+    "PrimecountError", "D", "D128", "S2_hard", "S2_hard128", "D_default",
+    "S2_hard_default", "D_batch", "gourdon_params", "dr_params", "set_alpha_y",
+    "set_alpha_z", "set_alpha", "set_segment_words", "set_host_threads",
+    "last_stats", "device_available", "release", "library_path", "load",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libprimecount_mi355x.so")
+_lib = None
+
+
+class PrimecountError(RuntimeError):
+    """Raised when the engine reports an error (mirrors primecount_error)."""
+
+
+class _I128(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_uint64), ("hi", ctypes.c_int64)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("seconds_total", ctypes.c_double),
+        ("seconds_tables", ctypes.c_double),
+        ("seconds_kernel", ctypes.c_double),
+        ("seconds_combine", ctypes.c_double),
+        ("segments", ctypes.c_uint64),
+        ("chunks", ctypes.c_uint64),
+        ("segment_words", ctypes.c_uint64),
+        ("sieve_limit", ctypes.c_uint64),
+        ("b_first", ctypes.c_uint64),
+        ("b_last", ctypes.c_uint64),
+        ("leaves", ctypes.c_uint64),
+        ("hits", ctypes.c_uint64),
+        ("factor_entries", ctypes.c_uint64),
+        ("pairs", ctypes.c_uint64),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def library_path():
+    return _LIB_PATH
+
+
+def load():
+    """Load libprimecount_mi355x.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise PrimecountError(
+            "libprimecount_mi355x.so is not built; run `python -m primecount_amd.build`")
+    lib = ctypes.CDLL(_LIB_PATH)
+    I64, P64 = ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)
+    PI128 = ctypes.POINTER(_I128)
+    sig = {
+        "pcg_last_error": ([], ctypes.c_char_p),
+        "pcg_version": ([], ctypes.c_char_p),
+        "pcg_device_available": ([], ctypes.c_int),
+        "pcg_D_i64": ([I64, I64, I64, I64, P64], ctypes.c_int),
+        "pcg_D_i128": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_S2_hard_i64": ([I64, I64, I64, I64, P64], ctypes.c_int),
+        "pcg_S2_hard_i128": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_D": ([_I128, PI128], ctypes.c_int),
+        "pcg_S2_hard": ([_I128, PI128], ctypes.c_int),
+        "pcg_gourdon_params": ([_I128, P64, P64, P64, P64], ctypes.c_int),
+        "pcg_dr_params": ([_I128, P64, P64, P64], ctypes.c_int),
+        "pcg_set_alpha_y": ([ctypes.c_double], ctypes.c_int),
+        "pcg_set_alpha_z": ([ctypes.c_double], ctypes.c_int),
+        "pcg_set_alpha": ([ctypes.c_double], ctypes.c_int),
+        "pcg_D_batch": ([PI128, ctypes.c_size_t, PI128], ctypes.c_int),
+        "pcg_set_segment_words": ([ctypes.c_uint32], ctypes.c_int),
+        "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
+        "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
+        "pcg_release": ([], ctypes.c_int),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def _i128(v):
+    v = int(v)
+    if not -(1 << 127) <= v < (1 << 127):
+        raise PrimecountError("value does not fit in 128 bits")
+    u = v & ((1 << 128) - 1)
+    return _I128(u & ((1 << 64) - 1), ctypes.c_int64(u >> 64).value)
+
+
+def _from128(r):
+    v = (r.hi << 64) | r.lo
+    return v
+
+
+def _check(rc):
+    if rc != 0:
+        msg = load().pcg_last_error().decode(errors="replace")
+        raise PrimecountError(msg or "error %d" % rc)
+
+
+def _fits64(x):
+    return -(1 << 63) <= x < (1 << 63)
+
+
+def D(x, y, z, k, threads=None):
+    """Gourdon's D(x, y, z, k) (threads is accepted for signature parity)."""
+    x = int(x)
+    if _fits64(x):
+        out = ctypes.c_int64()
+        _check(load().pcg_D_i64(x, int(y), int(z), int(k), ctypes.byref(out)))
+        return out.value
+    return D128(x, y, z, k)
+
+
+def D128(x, y, z, k, threads=None):
+    out = _I128()
+    _check(load().pcg_D_i128(_i128(x), int(y), int(z), int(k), ctypes.byref(out)))
+    return _from128(out)
+
+
+def S2_hard(x, y, z, c, threads=None):
+    """Deleglise-Rivat S2_hard(x, y, z, c)."""
+    x = int(x)
+    if _fits64(x):
+        out = ctypes.c_int64()
+        _check(load().pcg_S2_hard_i64(x, int(y), int(z), int(c), ctypes.byref(out)))
+        return out.value
+    return S2_hard128(x, y, z, c)
+
+
+def S2_hard128(x, y, z, c, threads=None):
+    out = _I128()
+    _check(load().pcg_S2_hard_i128(_i128(x), int(y), int(z), int(c), ctypes.byref(out)))
+    return _from128(out)
+
+
+def D_default(x):
+    out = _I128()
+    _check(load().pcg_D(_i128(x), ctypes.byref(out)))
+    return _from128(out)
+
+
+def S2_hard_default(x):
+    out = _I128()
+    _check(load().pcg_S2_hard(_i128(x), ctypes.byref(out)))
+    return _from128(out)
+
+
+def D_batch(xs):
+    """D(x) with default parameters for every x, in one batched launch."""
+    xs = [int(v) for v in xs]
+    n = len(xs)
+    arr = (_I128 * max(n, 1))(*[_i128(v) for v in xs])
+    out = (_I128 * max(n, 1))()
+    _check(load().pcg_D_batch(arr, n, out))
+    return [_from128(out[i]) for i in range(n)]
+
+
+def gourdon_params(x):
+    y, z, k, xs = (ctypes.c_int64() for _ in range(4))
+    _check(load().pcg_gourdon_params(_i128(x), ctypes.byref(y), ctypes.byref(z),
+                                     ctypes.byref(k), ctypes.byref(xs)))
+    return {"y": y.value, "z": z.value, "k": k.value, "x_star": xs.value}
+
+
+def dr_params(x):
+    y, z, c = (ctypes.c_int64() for _ in range(3))
+    _check(load().pcg_dr_params(_i128(x), ctypes.byref(y), ctypes.byref(z), ctypes.byref(c)))
+    return {"y": y.value, "z": z.value, "c": c.value}
+
+
+def set_alpha_y(a):
+    _check(load().pcg_set_alpha_y(float(a)))
+
+
+def set_alpha_z(a):
+    _check(load().pcg_set_alpha_z(float(a)))
+
+
+def set_alpha(a):
+    _check(load().pcg_set_alpha(float(a)))
+
+
+def set_segment_words(w):
+    _check(load().pcg_set_segment_words(int(w)))
+
+
+def set_host_threads(t):
+    _check(load().pcg_set_host_threads(int(t)))
+
+
+def last_stats():
+    st = Stats()
+    _check(load().pcg_last_stats(ctypes.byref(st)))
+    return st.as_dict()
+
+
+def device_available():
+    return bool(load().pcg_device_available())
+
+
+def release():
+    _check(load().pcg_release())

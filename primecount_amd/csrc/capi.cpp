@@ -1,0 +1,220 @@
+// capi.cpp -- the C ABI declared in include/primecount_mi355x.h.
+//
+// Every entry point converts C++ exceptions into a negative status and a
+// thread-local message, the way primecount's C API converts primecount_error
+// into -1 (src/api_c.cpp:19-56).
+
+#include "../../include/primecount_mi355x.h"
+
+#include "engine.hpp"
+#include "params.hpp"
+
+#include <exception>
+#include <new>
+#include <string>
+
+using namespace pcg;
+
+namespace {
+
+thread_local std::string t_error;
+thread_local pcg_stats t_stats{};
+
+i128 from_abi(pcg_i128 v)
+{
+  return (i128) (((u128) (uint64_t) v.hi << 64) | (u128) v.lo);
+}
+
+pcg_i128 to_abi(i128 v)
+{
+  pcg_i128 r;
+  r.lo = (uint64_t) v;
+  r.hi = (int64_t) (v >> 64);
+  return r;
+}
+
+template <typename F>
+int guarded(F&& f)
+{
+  try {
+    t_error.clear();
+    f();
+    return PCG_OK;
+  } catch (const std::bad_alloc&) {
+    t_error = "out of memory";
+    return PCG_E_NOMEM;
+  } catch (const pcg_error& e) {
+    t_error = e.what();
+    std::string m = t_error;
+    if (m.find("HIP error") != std::string::npos || m.find("device") != std::string::npos)
+      return PCG_E_DEVICE;
+    return PCG_E_INVALID;
+  } catch (const std::exception& e) {
+    t_error = e.what();
+    return PCG_E_INTERNAL;
+  } catch (...) {
+    t_error = "unknown error";
+    return PCG_E_INTERNAL;
+  }
+}
+
+i128 run(LeafSum kind, i128 x, int64_t y, int64_t z, int64_t k)
+{
+  HslQuery q;
+  q.kind = kind;
+  q.x = x;
+  q.y = y;
+  q.z = z;
+  q.k = k;
+  pcg_stats st{};
+  i128 r = hsl_compute(q, &st);
+  t_stats = st;
+  return r;
+}
+
+HslQuery default_query(LeafSum kind, i128 x)
+{
+  HslQuery q;
+  q.kind = kind;
+  q.x = x;
+  if (x < 1) return q;
+  if (kind == LeafSum::D) {
+    GourdonParams p = gourdon_params(x);
+    q.y = p.y; q.z = p.z; q.k = p.k;
+  } else {
+    DRParams p = deleglise_rivat_params(x);
+    q.y = p.y; q.z = p.z; q.k = p.c;
+  }
+  return q;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* pcg_last_error(void) { return t_error.c_str(); }
+
+const char* pcg_version(void) { return "0.1.0 (gfx950)"; }
+
+int pcg_device_available(void) { return device_available() ? 1 : 0; }
+
+int pcg_D_i64(int64_t x, int64_t y, int64_t z, int64_t k, int64_t* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = (int64_t) run(LeafSum::D, x, y, z, k);
+  });
+}
+
+int pcg_D_i128(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(run(LeafSum::D, from_abi(x), y, z, k));
+  });
+}
+
+int pcg_S2_hard_i64(int64_t x, int64_t y, int64_t z, int64_t c, int64_t* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = (int64_t) run(LeafSum::S2_hard, x, y, z, c);
+  });
+}
+
+int pcg_S2_hard_i128(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(run(LeafSum::S2_hard, from_abi(x), y, z, c));
+  });
+}
+
+int pcg_D(pcg_i128 x, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    HslQuery q = default_query(LeafSum::D, from_abi(x));
+    *out = to_abi(q.x < 1 ? 0 : run(LeafSum::D, q.x, q.y, q.z, q.k));
+  });
+}
+
+int pcg_S2_hard(pcg_i128 x, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    HslQuery q = default_query(LeafSum::S2_hard, from_abi(x));
+    *out = to_abi(q.x < 1 ? 0 : run(LeafSum::S2_hard, q.x, q.y, q.z, q.k));
+  });
+}
+
+int pcg_gourdon_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* k, int64_t* x_star)
+{
+  return guarded([&] {
+    GourdonParams p = gourdon_params(from_abi(x));
+    if (y) *y = p.y;
+    if (z) *z = p.z;
+    if (k) *k = p.k;
+    if (x_star) *x_star = p.x_star;
+  });
+}
+
+int pcg_dr_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* c)
+{
+  return guarded([&] {
+    DRParams p = deleglise_rivat_params(from_abi(x));
+    if (y) *y = p.y;
+    if (z) *z = p.z;
+    if (c) *c = p.c;
+  });
+}
+
+int pcg_set_alpha_y(double a) { return guarded([&] { set_alpha_y(a); }); }
+int pcg_set_alpha_z(double a) { return guarded([&] { set_alpha_z(a); }); }
+int pcg_set_alpha(double a) { return guarded([&] { set_alpha(a); }); }
+
+int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out)
+{
+  return guarded([&] {
+    if (n && (!xs || !out)) throw pcg_error("xs and out must not be NULL");
+    std::vector<HslQuery> qs;
+    qs.reserve(n);
+    for (size_t i = 0; i < n; i++) qs.push_back(default_query(LeafSum::D, from_abi(xs[i])));
+    pcg_stats st{};
+    auto r = hsl_compute_batch(qs, &st);
+    t_stats = st;
+    for (size_t i = 0; i < n; i++) out[i] = to_abi(r[i]);
+  });
+}
+
+int pcg_set_segment_words(uint32_t words)
+{
+  return guarded([&] {
+    if (words != 0 && (words < 256 || words > 4096 || (words & (words - 1))))
+      throw pcg_error("segment words must be 0 or a power of two in [256, 4096]");
+    engine_config().segment_words = words;
+  });
+}
+
+int pcg_set_host_threads(int threads)
+{
+  return guarded([&] {
+    if (threads < 0) throw pcg_error("threads must be >= 0");
+    engine_config().host_threads = threads;
+  });
+}
+
+int pcg_last_stats(pcg_stats* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = t_stats;
+  });
+}
+
+int pcg_release(void)
+{
+  return guarded([&] { release_device(); });
+}
+
+} // extern "C"

@@ -1,0 +1,595 @@
+// engine.cpp -- host orchestration of the hard-special-leaf engine:
+// validation, table building and upload, the segment/chunk plan, the kernel
+// launches and the final 128-bit reduction.
+//
+// Replaces the OpenMP driver of the reference (D_OpenMP, src/gourdon/D.cpp:
+// 382-419; S2_hard_OpenMP, src/deleglise-rivat/S2_hard.cpp:181-215) and its
+// LoadBalancerS2 (src/LoadBalancerS2.cpp:65-344): instead of threads taking
+// growing [low, low + segments * segment_size) intervals, the whole interval
+// is cut up front into chunks of equal estimated work that a persistent grid
+// of workgroups takes from a device work queue.
+
+#include "engine.hpp"
+#include "hsl_kernels.hpp"
+#include "params.hpp"
+#include "tables.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+namespace pcg {
+
+#define PCG_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw pcg_error(std::string("HIP error '") + hipGetErrorString(e_) +         \
+                      "' in " #expr);                                              \
+  } while (0)
+
+EngineConfig& engine_config()
+{
+  static EngineConfig cfg;
+  return cfg;
+}
+
+namespace {
+
+double now()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* ensure(size_t bytes)
+  {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > cap) {
+      if (p) (void) hipFree(p);
+      p = nullptr;
+      cap = 0;
+      hipError_t e = hipMalloc(&p, bytes);
+      if (e != hipSuccess) {
+        (void) hipGetLastError();
+        throw pcg_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+      }
+      cap = bytes;
+    }
+    return p;
+  }
+  void release()
+  {
+    if (p) (void) hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Device {
+  bool ready = false;
+  int dev = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {};
+  DevBuf queries, chunks, order, binfo, primes, factor, pi_bits, pi_cnt;
+  DevBuf tc, tm, s_out, counters, queue, cols, sc, sm, smp, out;
+
+  void init()
+  {
+    if (ready) return;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+      (void) hipGetLastError();
+      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU");
+    }
+    PCG_HIP(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    PCG_HIP(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+      throw pcg_error(std::string("unsupported device ") + prop.gcnArchName + " (need gfx950)");
+    cus = prop.multiProcessorCount;
+    PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    for (auto& e : ev) PCG_HIP(hipEventCreate(&e));
+    PCG_HIP(hsl_upload_constants());
+    ready = true;
+  }
+
+  void release()
+  {
+    for (DevBuf* b : {&queries, &chunks, &order, &binfo, &primes, &factor, &pi_bits, &pi_cnt,
+                      &tc, &tm, &s_out, &counters, &queue, &cols, &sc, &sm, &smp, &out})
+      b->release();
+  }
+};
+
+std::mutex g_mutex;
+Device g_dev;
+
+/// Host tables shared by the queries of one launch.
+struct TableSet {
+  LeafSum kind = LeafSum::D;
+  bool factor32 = false;
+  int64_t ft_y = 0, ft_limit = 0;   // factor table parameters
+  PrimeTables P;
+  std::vector<uint16_t> f16;
+  std::vector<uint32_t> f32;
+};
+
+/// Everything the device needs for one query.
+struct QueryPlan {
+  bool empty = true;
+  QueryDesc qd{};
+  std::vector<BInfo> binfo;
+  std::vector<ChunkDesc> chunks;   // tc_off relative to the query
+  double est_work = 0;
+};
+
+uint64_t imin(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+/// Prime-table range a query needs.
+uint64_t prime_range(const HslQuery& q)
+{
+  uint64_t y = (uint64_t) std::max<int64_t>(q.y, 1), z = (uint64_t) std::max<int64_t>(q.z, 1);
+  if (q.kind == LeafSum::D)
+    return std::max<uint64_t>(y, isqrt(z)) + 1;
+  uint64_t sqy = std::max<uint64_t>(1, isqrt(y));
+  uint64_t max_prime = imin(y, z / sqy);
+  return std::max({max_prime, imin(isqrt(z), y), sqy}) + 1;
+}
+
+void validate(const HslQuery& q)
+{
+  if (q.x < 1) return;
+  if (q.y < 1 || q.z < 1)
+    throw pcg_error("y and z must be >= 1");
+  if (q.k < 0 || q.k > 1000)
+    throw pcg_error("k/c out of range");
+  if (q.kind == LeafSum::D) {
+    if (q.z < q.y)
+      throw pcg_error("D(x, y, z, k) needs z >= y");
+    if (q.x / q.z > ((i128) 1 << 50))
+      throw pcg_error("D(x, y, z, k): x / z must be < 2^50");
+    if (q.z > ((int64_t) 1 << 40))
+      throw pcg_error("D(x, y, z, k): z must be <= 2^40");
+  } else {
+    if ((i128) q.z > ((i128) 1 << 50))
+      throw pcg_error("S2_hard(x, y, z, c): z must be < 2^50");
+  }
+  if (q.x > ((i128) std::max<int64_t>(q.y, 1) << 62))
+    throw pcg_error("x is too large (x > y * 2^62)");
+  if (q.y >= (int64_t) 0xFFFFFFF0LL)
+    throw pcg_error("y must be < 2^32");
+}
+
+/// Per-b constants of one query (see BInfo).
+QueryPlan plan_query(const HslQuery& q, const PrimeTables& P)
+{
+  QueryPlan qp;
+  if (q.x < 1) return qp;
+  const i128 x = q.x;
+  const uint64_t y = (uint64_t) q.y, z = (uint64_t) q.z;
+  uint64_t limit, b_first, b_last, split;
+  if (q.kind == LeafSum::D) {
+    limit = (uint64_t) (x / (i128) z);
+    uint64_t x_star = (uint64_t) get_x_star_gourdon(x, (int64_t) y);
+    uint64_t mb = imin(imin(x_star, isqrt((u128) x)), isqrt(limit));
+    b_first = (uint64_t) q.k + 1;
+    b_last = P.pi(imin(mb, P.max));
+    split = P.pi(isqrt(z));                     // pi(sqrt(z))
+  } else {
+    limit = z;
+    uint64_t sqy = isqrt(y);
+    b_first = (uint64_t) q.k + 1;
+    split = P.pi(sqy);                          // pi(sqrt(y))
+    if (z <= y)
+      b_last = split;
+    else
+      b_last = P.pi(imin(imin(imin(isqrt((u128) x), isqrt(z)), y), P.max));
+  }
+  if (limit == 0 || b_first > b_last) return qp;
+  if (b_last >= P.primes.size())
+    throw pcg_error("internal: prime table too small");
+
+  qp.empty = false;
+  qp.qd.limit = limit;
+  qp.qd.b_first = (uint32_t) b_first;
+  qp.qd.b_last = (uint32_t) b_last;
+  qp.qd.pre_last = (uint32_t) (b_first - 1);
+  qp.binfo.resize(b_last - b_first + 1);
+  for (uint64_t b = b_first; b <= b_last; b++) {
+    BInfo& bi = qp.binfo[b - b_first];
+    std::memset(&bi, 0, sizeof(bi));
+    uint64_t p = P.primes[b];
+    u128 xp = (u128) x / p;
+    bi.xp_lo = (uint64_t) xp;
+    bi.xp_d = (double) xp;
+    bi.p = (uint32_t) p;
+    u128 xpp = xp / ((u128) p * p);
+    if (q.kind == LeafSum::D) {
+      if (b <= split) {
+        bi.type = 1;
+        bi.T = (uint32_t) (2 * b + 1);
+        bi.A = (uint64_t) std::min<u128>(xpp, z);
+        bi.B = z / p;
+        bi.C = z;
+      } else {
+        bi.type = 2;
+        bi.A = (uint64_t) std::min<u128>(xpp, y);
+        bi.B = p;
+        bi.C = y;
+      }
+    } else {
+      if (b <= split) {
+        bi.type = 1;
+        bi.T = (uint32_t) p;
+        bi.A = y;
+        bi.B = y / p;
+        bi.C = y;
+      } else {
+        bi.type = 2;
+        bi.A = imin(y, z / p);
+        bi.B = p;
+        bi.C = y;
+      }
+    }
+  }
+  return qp;
+}
+
+/// Cut the query's segments into chunks of about `target` estimated steps.
+/// A segment starting at `low` can only process b with p_b <= sqrt(x / low)
+/// (beyond that its leaf range is empty and the kernel stops), so
+/// pi(min(p_{b_last}, sqrt(x / low))) - b_first + 1 bounds the b it stores.
+void plan_chunks(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W,
+                 double target_chunks)
+{
+  const uint64_t SEG = 240ull * W;
+  const uint64_t nseg = (qp.qd.limit + SEG - 1) / SEG;
+  const uint64_t nb = qp.qd.b_last - qp.qd.b_first + 1;
+  const uint64_t p_last = P.primes[qp.qd.b_last];
+  std::vector<uint32_t> est;
+  est.reserve(nseg);
+  double total = 0;
+  for (uint64_t s = 0; s < nseg; s++) {
+    uint64_t low1 = std::max<uint64_t>(1, s * SEG);
+    uint64_t r = isqrt((u128) q.x / low1);
+    uint64_t bb = P.pi(imin(p_last, r));
+    uint64_t e = bb >= qp.qd.b_first ? std::min<uint64_t>(nb, bb - qp.qd.b_first + 1) : 0;
+    if (e == 0) break;
+    est.push_back((uint32_t) e);
+    total += e;
+  }
+  qp.est_work = total;
+  const double target = std::max(1.0, total / std::max(1.0, target_chunks));
+  const uint64_t max_len = std::max<uint64_t>(1, ((uint64_t) 1 << 31) / SEG);
+  uint64_t off = 0;
+  for (uint64_t s = 0; s < est.size();) {
+    ChunkDesc c{};
+    c.seg_begin = (uint32_t) s;
+    c.nbc = est[s];
+    double acc = 0;
+    while (s < est.size() && (acc < target || s == c.seg_begin) && s - c.seg_begin < max_len)
+      acc += est[s++];
+    c.seg_end = (uint32_t) s;
+    c.tc_off = off;
+    off += c.nbc;
+    qp.chunks.push_back(c);
+  }
+}
+
+uint32_t auto_segment_words(uint64_t limit, uint64_t slots)
+{
+  uint32_t cfg = engine_config().segment_words;
+  if (cfg) return cfg;
+  uint32_t W = 2048;
+  while (W > 256 && (limit + 240ull * W - 1) / (240ull * W) < 2 * slots)
+    W >>= 1;
+  return W;
+}
+
+TableSet build_tables(const std::vector<HslQuery>& qs)
+{
+  TableSet T;
+  T.kind = qs[0].kind;
+  uint64_t pmax = 7;
+  int64_t fy = 1, flim = 1;
+  for (const auto& q : qs) {
+    if (q.x < 1) continue;
+    pmax = std::max(pmax, prime_range(q));
+    if (q.kind == LeafSum::D) {
+      flim = std::max(flim, q.z);
+      fy = std::max(fy, q.y);
+    } else {
+      flim = std::max(flim, q.y);
+      fy = std::max(fy, q.y);
+    }
+  }
+  int threads = engine_config().host_threads > 0 ? engine_config().host_threads : default_threads();
+  T.P = build_prime_tables(pmax, threads);
+  T.ft_y = fy;
+  T.ft_limit = flim;
+  if (T.kind == LeafSum::D) {
+    T.factor32 = flim > kFactorTableD16Max;
+    if (T.factor32) T.f32 = build_factor_table_D<uint32_t>(fy, flim, T.P, threads);
+    else T.f16 = build_factor_table_D<uint16_t>(fy, flim, T.P, threads);
+  } else {
+    T.factor32 = flim > kFactorTable16Max;
+    if (T.factor32) T.f32 = build_factor_table_lpf<uint32_t>(flim, T.P, threads);
+    else T.f16 = build_factor_table_lpf<uint16_t>(flim, T.P, threads);
+  }
+  return T;
+}
+
+void upload(DevBuf& b, const void* src, size_t bytes)
+{
+  b.ensure(bytes);
+  if (bytes) PCG_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+/// Run a set of queries that share one table set.  Caller holds g_mutex.
+std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
+{
+  Device& D = g_dev;
+  D.init();
+  double t0 = now();
+  for (const auto& q : qs) validate(q);
+
+  std::vector<i128> result(qs.size(), 0);
+  bool any = false;
+  for (const auto& q : qs) any |= q.x >= 1;
+  if (!any) return result;
+
+  TableSet T = build_tables(qs);
+
+  int bpc = 0;
+  uint64_t max_limit = 1;
+  std::vector<QueryPlan> plans(qs.size());
+  for (size_t i = 0; i < qs.size(); i++) {
+    plans[i] = plan_query(qs[i], T.P);
+    if (!plans[i].empty) max_limit = std::max(max_limit, plans[i].qd.limit);
+  }
+  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 4);
+  if (W < 256 || W > 4096 || (W & (W - 1)))
+    throw pcg_error("segment words must be a power of two in [256, 4096]");
+  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
+  bpc = std::max(bpc, 1);
+  const uint64_t slots = (uint64_t) D.cus * bpc;
+
+  // chunks: each query gets a share of the chunk budget by estimated work
+  double total_work = 0;
+  for (size_t i = 0; i < qs.size(); i++)
+    if (!plans[i].empty) {
+      plan_chunks(plans[i], qs[i], T.P, W, 1.0);       // measure work
+      total_work += plans[i].est_work;
+      plans[i].chunks.clear();
+    }
+  const double budget = 6.0 * slots + 4.0 * qs.size();
+  std::vector<QueryDesc> qdesc;
+  std::vector<BInfo> binfo;
+  std::vector<ChunkDesc> chunks;
+  std::vector<uint32_t> chunk_first(qs.size(), 0), nchunks_q(qs.size(), 0);
+  std::vector<uint32_t> qslot(qs.size(), UINT32_MAX);
+  uint64_t tc_total = 0;
+  for (size_t i = 0; i < qs.size(); i++) {
+    if (plans[i].empty) continue;
+    double share = std::max(1.0, budget * plans[i].est_work / std::max(1.0, total_work));
+    plan_chunks(plans[i], qs[i], T.P, W, share);
+    QueryDesc qd = plans[i].qd;
+    qd.binfo_off = (uint32_t) binfo.size();
+    qslot[i] = (uint32_t) qdesc.size();
+    qdesc.push_back(qd);
+    binfo.insert(binfo.end(), plans[i].binfo.begin(), plans[i].binfo.end());
+    chunk_first[i] = (uint32_t) chunks.size();
+    nchunks_q[i] = (uint32_t) plans[i].chunks.size();
+    for (ChunkDesc c : plans[i].chunks) {
+      c.query = qslot[i];
+      c.tc_off += tc_total;
+      chunks.push_back(c);
+    }
+    tc_total += plans[i].chunks.empty() ? 0
+              : plans[i].chunks.back().tc_off + plans[i].chunks.back().nbc;
+  }
+  if (chunks.empty()) return result;
+
+  // schedule: longest estimated chunks first across queries
+  std::vector<double> cost(chunks.size());
+  for (size_t c = 0; c < chunks.size(); c++)
+    cost[c] = (double) chunks[c].nbc * (chunks[c].seg_end - chunks[c].seg_begin);
+  std::vector<uint32_t> order(chunks.size());
+  std::iota(order.begin(), order.end(), 0);
+  if (qdesc.size() > 1)
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+
+  // combine columns
+  std::vector<CombineCol> cols;
+  std::vector<uint32_t> col_query;
+  uint32_t max_nch = 1;
+  for (size_t i = 0; i < qs.size(); i++) {
+    if (plans[i].empty) continue;
+    const auto& pc = plans[i].chunks;
+    uint32_t nb = qdesc[qslot[i]].b_last - qdesc[qslot[i]].b_first + 1;
+    uint32_t nch = (uint32_t) pc.size();
+    for (uint32_t br = 0; br < nb; br++) {
+      while (nch > 0 && pc[nch - 1].nbc <= br) nch--;
+      if (nch == 0) break;
+      cols.push_back(CombineCol{br, chunk_first[i], nch, qslot[i]});
+      col_query.push_back((uint32_t) i);
+      max_nch = std::max(max_nch, nch);
+    }
+  }
+  uint32_t slice_len = std::max<uint32_t>(1, (max_nch + 63) / 64);
+  uint32_t nslices = (max_nch + slice_len - 1) / slice_len;
+  const uint32_t ncols = (uint32_t) cols.size();
+
+  // uploads
+  upload(D.primes, T.P.primes.data(), T.P.primes.size() * 4);
+  upload(D.pi_bits, T.P.pi_bits.data(), T.P.pi_bits.size() * 8);
+  upload(D.pi_cnt, T.P.pi_cnt.data(), T.P.pi_cnt.size() * 4);
+  if (T.factor32) upload(D.factor, T.f32.data(), T.f32.size() * 4);
+  else upload(D.factor, T.f16.data(), T.f16.size() * 2);
+  upload(D.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
+  upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
+  upload(D.chunks, chunks.data(), chunks.size() * sizeof(ChunkDesc));
+  upload(D.order, order.data(), order.size() * 4);
+  upload(D.cols, cols.data(), cols.size() * sizeof(CombineCol));
+  D.tc.ensure(tc_total * 4);
+  D.tm.ensure(tc_total * 4);
+  D.s_out.ensure(chunks.size() * 16);
+  D.counters.ensure(64);
+  D.queue.ensure(64);
+  D.sc.ensure((size_t) nslices * ncols * 8);
+  D.sm.ensure((size_t) nslices * ncols * 8);
+  D.smp.ensure((size_t) nslices * ncols * 16);
+  D.out.ensure((size_t) ncols * 16);
+  double t1 = now();
+
+  hipStream_t s = D.stream;
+  PCG_HIP(hsl_launch_fill_zero(D.tc.p, tc_total * 4, s));
+  PCG_HIP(hsl_launch_fill_zero(D.tm.p, tc_total * 4, s));
+  PCG_HIP(hsl_launch_fill_zero(D.counters.p, 64, s));
+  PCG_HIP(hsl_launch_fill_zero(D.queue.p, 64, s));
+
+  HslArgs a{};
+  a.queries = (const QueryDesc*) D.queries.p;
+  a.chunks = (const ChunkDesc*) D.chunks.p;
+  a.order = (const uint32_t*) D.order.p;
+  a.binfo = (const BInfo*) D.binfo.p;
+  a.primes = (const uint32_t*) D.primes.p;
+  a.factor = D.factor.p;
+  a.pi_bits = (const uint64_t*) D.pi_bits.p;
+  a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
+  a.T_C = (uint32_t*) D.tc.p;
+  a.T_M = (int32_t*) D.tm.p;
+  a.S_out = (unsigned long long*) D.s_out.p;
+  a.counters = (unsigned long long*) D.counters.p;
+  a.queue = (unsigned int*) D.queue.p;
+  a.nchunks = (uint32_t) chunks.size();
+  a.W = W;
+  uint32_t grid = (uint32_t) std::min<uint64_t>(slots, chunks.size());
+
+  PCG_HIP(hipEventRecord(D.ev[0], s));
+  PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
+  PCG_HIP(hipEventRecord(D.ev[1], s));
+  PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const ChunkDesc*) D.chunks.p,
+                             (const uint32_t*) D.tc.p, (const int32_t*) D.tm.p, slice_len, nslices,
+                             (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p, D.out.p, s));
+  PCG_HIP(hipEventRecord(D.ev[2], s));
+
+  std::vector<i128> s_out(chunks.size());
+  std::vector<i128> col_out(ncols);
+  uint64_t counters[4] = {0, 0, 0, 0};
+  PCG_HIP(hipMemcpyAsync(s_out.data(), D.s_out.p, chunks.size() * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(counters, D.counters.p, 32, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipStreamSynchronize(s));
+
+  std::vector<i128> per_slot(qdesc.size(), 0);
+  for (size_t c = 0; c < chunks.size(); c++) per_slot[chunks[c].query] += s_out[c];
+  for (uint32_t c = 0; c < ncols; c++) per_slot[cols[c].query] += col_out[c];
+  for (size_t i = 0; i < qs.size(); i++)
+    if (qslot[i] != UINT32_MAX) result[i] = per_slot[qslot[i]];
+
+  if (st) {
+    float ms_k = 0, ms_c = 0;
+    PCG_HIP(hipEventElapsedTime(&ms_k, D.ev[0], D.ev[1]));
+    PCG_HIP(hipEventElapsedTime(&ms_c, D.ev[1], D.ev[2]));
+    st->seconds_total = now() - t0;
+    st->seconds_tables = t1 - t0;
+    st->seconds_kernel = ms_k / 1000.0;
+    st->seconds_combine = ms_c / 1000.0;
+    uint64_t nseg = 0;
+    for (const auto& c : chunks) nseg += c.seg_end - c.seg_begin;
+    st->segments = nseg;
+    st->chunks = chunks.size();
+    st->segment_words = W;
+    st->sieve_limit = qdesc.size() == 1 ? qdesc[0].limit : 0;
+    st->b_first = qdesc.size() == 1 ? qdesc[0].b_first : 0;
+    st->b_last = qdesc.size() == 1 ? qdesc[0].b_last : 0;
+    st->leaves = counters[0];
+    st->hits = counters[1];
+    st->factor_entries = counters[2];
+    st->pairs = counters[3];
+  }
+  return result;
+}
+
+/// Queries can share a D factor table when y >= z (then no entry <= z
+/// depends on y, src/gourdon/FactorTableD.hpp:168-190) -- the default
+/// parameters always have y == z.  Everything else runs on its own table.
+std::vector<std::vector<size_t>> group_queries(const std::vector<HslQuery>& qs)
+{
+  std::vector<std::vector<size_t>> groups;
+  std::vector<size_t> shared_d, shared_s2;
+  for (size_t i = 0; i < qs.size(); i++) {
+    if (qs[i].kind == LeafSum::S2_hard) shared_s2.push_back(i);
+    else if (qs[i].y >= qs[i].z) shared_d.push_back(i);
+    else groups.push_back({i});
+  }
+  if (!shared_d.empty()) groups.push_back(shared_d);
+  if (!shared_s2.empty()) groups.push_back(shared_s2);
+  return groups;
+}
+
+} // namespace
+
+i128 hsl_compute(const HslQuery& q, pcg_stats* stats)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  auto r = run_group({q}, stats);
+  return r[0];
+}
+
+std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  std::vector<i128> out(qs.size(), 0);
+  double t0 = now();
+  pcg_stats acc{};
+  for (const auto& g : group_queries(qs)) {
+    std::vector<HslQuery> sub;
+    for (size_t i : g) sub.push_back(qs[i]);
+    pcg_stats st{};
+    auto r = run_group(sub, &st);
+    for (size_t j = 0; j < g.size(); j++) out[g[j]] = r[j];
+    acc.seconds_tables += st.seconds_tables;
+    acc.seconds_kernel += st.seconds_kernel;
+    acc.seconds_combine += st.seconds_combine;
+    acc.segments += st.segments;
+    acc.chunks += st.chunks;
+    acc.segment_words = st.segment_words;
+    acc.leaves += st.leaves;
+    acc.hits += st.hits;
+    acc.factor_entries += st.factor_entries;
+    acc.pairs += st.pairs;
+  }
+  acc.seconds_total = now() - t0;
+  if (stats) *stats = acc;
+  return out;
+}
+
+bool device_available()
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  try {
+    g_dev.init();
+    return true;
+  } catch (...) {
+    return false;
+  }
+}
+
+void release_device()
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_dev.release();
+}
+
+} // namespace pcg

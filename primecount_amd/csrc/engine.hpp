@@ -1,0 +1,39 @@
+// engine.hpp -- host interface of the MI355X hard-special-leaf engine.
+#pragma once
+
+#include "pc_math.hpp"
+#include "../../include/primecount_mi355x.h"
+
+#include <vector>
+
+namespace pcg {
+
+enum class LeafSum { D, S2_hard };
+
+/// One hard-special-leaf query with explicit parameters.
+struct HslQuery {
+  LeafSum kind = LeafSum::D;
+  i128 x = 0;
+  int64_t y = 0, z = 0;
+  int64_t k = 0;          // k (D) or c (S2_hard)
+};
+
+/// Engine settings (process-wide).
+struct EngineConfig {
+  uint32_t segment_words = 0;   // 0 = automatic
+  int host_threads = 0;         // 0 = all hardware threads
+};
+
+EngineConfig& engine_config();
+
+/// Computes one query on the device; fills `stats`.  Throws pcg_error.
+i128 hsl_compute(const HslQuery& q, pcg_stats* stats);
+
+/// Computes many D/S2_hard queries with default or explicit parameters in one
+/// persistent launch per table set.  Throws pcg_error.
+std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats);
+
+bool device_available();
+void release_device();
+
+} // namespace pcg

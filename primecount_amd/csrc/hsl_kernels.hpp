@@ -1,0 +1,109 @@
+// hsl_kernels.hpp -- data layout shared by the host planner and the HIP
+// kernels of the hard-special-leaf engine.
+//
+// Terms (primecount's vocabulary):
+//   segment   240*W consecutive numbers [low, high) of the sieve interval,
+//             held in LDS as W 64-bit words, wheel-30 layout (bit 8j+i of a
+//             word <-> 30j + {1,7,11,13,17,19,23,29}[i]), as in
+//             src/sieve/Sieve.cpp:62-66
+//   b         index of the sieving prime p_b (1-indexed: p_1 = 2)
+//   leaf      x / (p_b * m) with a square-free m (type 1, factor-table loop)
+//             or a prime m (type 2, prime loop); it contributes
+//             sign * phi(x / (p_b * m), b - 1)
+//   chunk     a run of consecutive segments of one query, the unit a
+//             workgroup takes from the device work queue
+#pragma once
+
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+namespace pcg {
+
+/// Per sieving prime constants of one query (host computed, read uniformly).
+/// For segment [low, high) the leaf range of prime p_b is
+///   m_hi = min(A, xp / max(low, 1)),  m_lo = max(min(xp / high, C), B)
+/// and the segment stops at the first b whose range is provably empty
+/// (type 1: p >= m_hi; type 2: no prime in (p, m_hi]), exactly as
+/// src/gourdon/D.cpp:125-132, 281-290 and
+/// src/deleglise-rivat/S2_hard.cpp:100-107, 135-143 do.
+struct alignas(16) BInfo {
+  uint64_t xp_lo;   // low 64 bits of xp = x / p_b
+  uint64_t A;       // static cap of m_hi
+  uint64_t B;       // static floor of m_lo
+  uint64_t C;       // cap of xp / high
+  double xp_d;      // xp rounded to double
+  uint32_t p;       // p_b
+  uint32_t T;       // type 1: leaf iff factor[m] > T
+  uint32_t type;    // 1 = factor-table leaves (sign -mu(m)), 2 = prime leaves (sign +1)
+  uint32_t pad0;
+  uint64_t pad1;
+};
+
+/// One query of a launch.
+struct QueryDesc {
+  uint64_t limit;        // sieve interval [0, limit)
+  uint32_t b_first;      // first b (k+1 or c+1)
+  uint32_t b_last;       // last b that can hold leaves
+  uint32_t binfo_off;    // BInfo of b is binfo[binfo_off + b - b_first]
+  uint32_t pre_last;     // primes p_4..p_pre_last are pre-sieved (pre_last = b_first-1)
+};
+
+/// One work-queue item: segments [seg_begin, seg_end) of query `query`.
+/// Its per-b partial sums live at tc_off + (b - b_first), b - b_first < nbc.
+struct ChunkDesc {
+  uint32_t query;
+  uint32_t seg_begin;
+  uint32_t seg_end;
+  uint32_t nbc;
+  uint64_t tc_off;
+};
+
+struct HslArgs {
+  const QueryDesc* queries;
+  const ChunkDesc* chunks;
+  const uint32_t* order;        // work-queue position -> chunk index
+  const BInfo* binfo;
+  const uint32_t* primes;       // primes[0] = 0, primes[1] = 2, ...
+  const void* factor;           // uint16_t or uint32_t factor table
+  const uint64_t* pi_bits;      // prime bits per 240 numbers (primes >= 7)
+  const uint32_t* pi_cnt;       // primes >= 7 below 240 * i
+  uint32_t* T_C;                // per (chunk, b): unsieved count summed over the chunk
+  int32_t* T_M;                 // per (chunk, b): sum of leaf signs over the chunk
+  unsigned long long* S_out;    // per chunk: 128-bit partial sum (lo, hi)
+  unsigned long long* counters; // leaves, hits, factor entries, pairs
+  unsigned int* queue;          // next chunk
+  uint32_t nchunks;
+  uint32_t W;                   // words per segment (power of two, 256..4096)
+};
+
+constexpr int kBlock = 256;
+
+/// Column of the carry combine: one (query, b).
+struct CombineCol {
+  uint32_t b_rel;         // b - b_first
+  uint32_t chunk_first;   // first chunk (index into the chunk array) of the query
+  uint32_t nch;           // leading chunks of the query that store this b
+  uint32_t query;
+};
+
+/// LDS bytes the sieve kernel needs for W words per segment.
+inline uint32_t hsl_lds_bytes(uint32_t W)
+{
+  return 8 * W            // segment bitmap
+       + W / 4            // group counters (16-bit, 8 words per group)
+       + 64 * 4           // super-group prefix (64 words per super-group)
+       + 240 * 8          // "offsets <= r" masks
+       + 480 * 2;         // numbers coprime to 2310 in [0, 2310)
+}
+
+// Host-side launch wrappers (defined in hsl_kernels.hip).
+hipError_t hsl_upload_constants();
+hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
+hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
+hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
+hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const ChunkDesc* chunks,
+                              const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
+                              uint32_t nslices, long long* sC, long long* sM, void* sMP,
+                              void* out, hipStream_t st);
+
+} // namespace pcg

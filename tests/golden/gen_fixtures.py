@@ -1,0 +1,211 @@
+#!/usr/bin/env python3
+"""Generate the golden fixtures for the hard-special-leaf path.
+
+Runs HERE only (it needs the reference sources under /root/reference and the
+reference CLI built from them by ``make -f oracle/ref.mk``).  The outputs are
+plain JSON data files next to this script: inputs and the reference's exact
+outputs.  Nothing on the GPU side ever runs this script.
+
+Fixture files
+-------------
+kat_D.json          the 51 (x, y, z, k) -> D rows of the reference's own test
+                    table (test/gourdon/D.cpp:36-89)
+kat_S2_hard.json    the 35 (x, y, z, c) -> S2_hard rows of
+                    test/deleglise-rivat/S2_hard_alpha.cpp:36-73
+params.json         x -> (y, z, k, x_star) chosen by the reference's double
+                    precision alpha formulas (Gourdon, via `--Phi0 -s`) and
+                    x -> (y, z, c) (Deleglise-Rivat, via `--S2-trivial -s`)
+configs.json        D(x) and S2_hard(x) with default parameters at the
+                    workload sizes (1e13 ... 1e19, optionally 1e21)
+batch_D.json        the 256-query batch: x_i = floor(10^(12 + 6 u_i)),
+                    u_i = (splitmix64 >> 11) * 2^-53, and D(x_i)
+alpha_D.json        D(x) at non-default alpha_y / alpha_z (y < z cases)
+
+Usage:  python tests/golden/gen_fixtures.py [--big] [--only NAME]
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REF = "/root/reference"
+CLI = os.path.join(REPO, "oracle", "_ref", "primecount")
+M64 = (1 << 64) - 1
+
+
+def splitmix64(state):
+    state = (state + 0x9E3779B97F4A7C15) & M64
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return state, z ^ (z >> 31)
+
+
+def batch_xs(seed, n=256, lo_exp=12.0, span=6.0):
+    """x_i = floor(10^(lo_exp + span * u_i)) with u_i in [0, 1) from SplitMix64."""
+    xs, st = [], seed
+    for _ in range(n):
+        st, r = splitmix64(st)
+        u = (r >> 11) * (2.0 ** -53)
+        xs.append(int(10.0 ** (lo_exp + span * u)))
+    return xs
+
+
+def run_cli(args, timeout=3600):
+    out = subprocess.run([CLI] + [str(a) for a in args], capture_output=True,
+                         text=True, timeout=timeout, check=True).stdout
+    vals = {}
+    for line in out.splitlines():
+        line = line.replace("\x1b[K", "").split("\r")[-1]
+        m = re.match(r"^\s*([A-Za-z0-9_]+)\s*=\s*(-?[0-9.]+)\s*$", line)
+        if m:
+            vals[m.group(1)] = m.group(2)
+        m = re.match(r"^Seconds:\s*([0-9.]+)", line)
+        if m:
+            vals["seconds"] = m.group(1)
+    return vals
+
+
+def parse_kat(path, nfields):
+    """Extract the `{ a, b, c, d, e }` integer rows of a reference test table."""
+    rows = []
+    text = open(path).read()
+    for m in re.finditer(r"\{\s*(-?\d+)(?:LL)?" + r"\s*,\s*(-?\d+)(?:LL)?" * (nfields - 1) + r"\s*\}", text):
+        rows.append([int(g) for g in m.groups()])
+    return rows
+
+
+def gen_kat():
+    d_rows = parse_kat(os.path.join(REF, "test/gourdon/D.cpp"), 5)
+    assert len(d_rows) == 51, len(d_rows)
+    s_rows = parse_kat(os.path.join(REF, "test/deleglise-rivat/S2_hard_alpha.cpp"), 5)
+    assert len(s_rows) == 35, len(s_rows)
+    json.dump({"source": "test/gourdon/D.cpp:36-89",
+               "fields": ["x", "y", "z", "k", "D"],
+               "rows": [[str(v) for v in r] for r in d_rows]},
+              open(os.path.join(HERE, "kat_D.json"), "w"))
+    json.dump({"source": "test/deleglise-rivat/S2_hard_alpha.cpp:36-73",
+               "fields": ["x", "y", "z", "c", "S2_hard"],
+               "rows": [[str(v) for v in r] for r in s_rows]},
+              open(os.path.join(HERE, "kat_S2_hard.json"), "w"))
+
+
+def gourdon_params(x):
+    v = run_cli([x, "--Phi0", "-s"])
+    return {"x": str(x), "y": v["y"], "z": v["z"], "k": v["k"], "x_star": v["x_star"]}
+
+
+def dr_params(x):
+    v = run_cli([x, "--S2-trivial", "-s"])
+    return {"x": str(x), "y": v["y"], "z": v["z"], "c": v["c"]}
+
+
+def gen_params():
+    xs = set()
+    for e in range(1, 23):
+        xs.add(10 ** e)
+    for e in range(2, 23):
+        xs.add(10 ** e - 1)
+        xs.add(10 ** e + 1)
+    xs.update([2 ** 63 - 1, 2 ** 63, 2 ** 64 - 1, 2 ** 64, 2 ** 64 + 1, 2, 3, 5, 7, 100000,
+               130321, 130320, 279841, 279840])
+    st = 0xC0FFEE
+    for _ in range(300):
+        st, r = splitmix64(st)
+        u = (r >> 11) * (2.0 ** -53)
+        xs.add(int(10.0 ** (1.0 + 21.0 * u)))
+    xs = sorted(xs)
+    g = [gourdon_params(x) for x in xs]
+    d = [dr_params(x) for x in xs]
+    json.dump({"source": "oracle/_ref primecount --Phi0 -s / --S2-trivial -s",
+               "gourdon": g, "deleglise_rivat": d},
+              open(os.path.join(HERE, "params.json"), "w"), indent=0)
+
+
+def gen_configs(big):
+    out = {"source": "oracle/_ref primecount X --D -s / --S2-hard -s (8 host threads)",
+           "D": [], "S2_hard": []}
+    exps = [13, 15, 17, 18, 19] + ([21] if big else [])
+    for e in exps:
+        x = 10 ** e
+        v = run_cli([x, "--D", "-s"], timeout=7200)
+        out["D"].append({"x": str(x), "y": v["y"], "z": v["z"], "k": v["k"],
+                         "x_star": v["x_star"], "value": v["D"],
+                         "ref_seconds_8t": v.get("seconds")})
+        print("D", e, v["D"], v.get("seconds"), flush=True)
+    for e in [13, 15, 17, 19] + ([21] if big else []):
+        x = 10 ** e
+        v = run_cli([x, "--S2-hard", "-s"], timeout=7200)
+        out["S2_hard"].append({"x": str(x), "y": v["y"], "z": v["z"], "c": v["c"],
+                               "value": v["S2_hard"], "ref_seconds_8t": v.get("seconds")})
+        print("S2_hard", e, v["S2_hard"], v.get("seconds"), flush=True)
+    # full pi(x) for the README powers of ten we can afford here
+    out["pi"] = [{"x": str(10 ** e),
+                  "value": subprocess.run([CLI, str(10 ** e)], capture_output=True, text=True,
+                                          check=True).stdout.strip()} for e in range(1, 16)]
+    json.dump(out, open(os.path.join(HERE, "configs.json"), "w"), indent=1)
+
+
+def gen_big():
+    """x = 1e21 (x > 2^64: 128-bit path), D and S2_hard; tens of minutes."""
+    out = {"source": "oracle/_ref primecount X --D -s / --S2-hard -s (8 host threads)",
+           "D": [], "S2_hard": []}
+    x = 10 ** 21
+    v = run_cli([x, "--D", "-s"], timeout=7200)
+    out["D"].append({"x": str(x), "y": v["y"], "z": v["z"], "k": v["k"], "x_star": v["x_star"],
+                     "value": v["D"], "ref_seconds_8t": v.get("seconds")})
+    v = run_cli([x, "--S2-hard", "-s"], timeout=7200)
+    out["S2_hard"].append({"x": str(x), "y": v["y"], "z": v["z"], "c": v["c"],
+                           "value": v["S2_hard"], "ref_seconds_8t": v.get("seconds")})
+    json.dump(out, open(os.path.join(HERE, "configs_big.json"), "w"), indent=1)
+
+
+def gen_batch():
+    seed = 0x5EED2026
+    xs = batch_xs(seed)
+    rows = []
+    for x in xs:
+        v = run_cli([x, "--D", "-s"])
+        rows.append({"x": str(x), "y": v["y"], "z": v["z"], "k": v["k"],
+                     "x_star": v["x_star"], "D": v["D"]})
+    json.dump({"source": "oracle/_ref primecount X --D -s",
+               "seed": seed, "rule": "x_i = floor(10^(12 + 6 u_i)), u_i = (splitmix64 >> 11) * 2^-53",
+               "rows": rows}, open(os.path.join(HERE, "batch_D.json"), "w"), indent=0)
+
+
+def gen_alpha():
+    rows = []
+    for e in [8, 9, 10, 11, 12, 13, 14]:
+        x = 10 ** e
+        for ay, az in [(1.0, 1.0), (1.0, 2.0), (2.0, 3.0), (3.0, 1.5), (1.5, 4.0)]:
+            v = run_cli([x, "--D", "-s", "--alpha-y=%g" % ay, "--alpha-z=%g" % az])
+            rows.append({"x": str(x), "y": v["y"], "z": v["z"], "k": v["k"], "D": v["D"]})
+            v = run_cli([x, "--S2-hard", "-s", "--alpha=%g" % (ay * az)])
+            rows.append({"x": str(x), "y": v["y"], "z": v["z"], "c": v["c"], "S2_hard": v["S2_hard"]})
+    json.dump({"source": "oracle/_ref primecount X --D/--S2-hard with --alpha-y/--alpha-z/--alpha",
+               "rows": rows}, open(os.path.join(HERE, "alpha_D.json"), "w"), indent=0)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--big", action="store_true", help="also run x = 1e21 in `configs`")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    if not os.path.exists(CLI):
+        sys.exit("build the reference first: make -f oracle/ref.mk -j8")
+    steps = {"kat": gen_kat, "params": gen_params, "configs": lambda: gen_configs(a.big),
+             "batch": gen_batch, "alpha": gen_alpha, "big": gen_big}
+    for name, fn in steps.items():
+        if (a.only and a.only != name) or (not a.only and name == "big"):
+            continue
+        print("==", name, flush=True)
+        fn()
+
+
+if __name__ == "__main__":
+    main()

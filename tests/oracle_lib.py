@@ -1,0 +1,54 @@
+"""ctypes access to the CPU oracle (oracle/hsl_oracle.c).
+
+Test infrastructure: the oracle is the checker the HIP engine is compared
+against; it is built into oracle/liboracle.so by oracle/build_oracle.sh (or
+__graft_entry__.build()).
+"""
+import ctypes
+import os
+import subprocess
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "oracle", "liboracle.so")
+SRC = os.path.join(REPO, "oracle", "hsl_oracle.c")
+
+
+class Oracle:
+    def __init__(self, lib):
+        self.lib = lib
+        for name in ("oracle_D", "oracle_S2_hard"):
+            fn = getattr(lib, name)
+            fn.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                           ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64),
+                           ctypes.POINTER(ctypes.c_int64)]
+            fn.restype = None
+        lib.oracle_phi_direct.argtypes = [ctypes.c_int64, ctypes.c_int64]
+        lib.oracle_phi_direct.restype = ctypes.c_int64
+
+    def _call(self, fn, x, y, z, k, seg):
+        x = int(x)
+        u = x & ((1 << 128) - 1)
+        lo, hi = ctypes.c_uint64(), ctypes.c_int64()
+        fn(u & ((1 << 64) - 1), ctypes.c_int64(u >> 64).value, int(y), int(z), int(k), int(seg),
+           ctypes.byref(lo), ctypes.byref(hi))
+        return (hi.value << 64) | lo.value
+
+    def D(self, x, y, z, k, seg=1 << 16):
+        return self._call(self.lib.oracle_D, x, y, z, k, seg)
+
+    def S2_hard(self, x, y, z, c, seg=1 << 16):
+        return self._call(self.lib.oracle_S2_hard, x, y, z, c, seg)
+
+    def phi_direct(self, n, a):
+        return self.lib.oracle_phi_direct(int(n), int(a))
+
+
+def build():
+    if (not os.path.exists(LIB)) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
+        subprocess.run(["gcc", "-O2", "-shared", "-fPIC", "-o", LIB, SRC, "-lm"], check=True)
+    return LIB
+
+
+def load():
+    build()
+    return Oracle(ctypes.CDLL(LIB))

@@ -1,0 +1,86 @@
+"""The CPU oracle, pinned before it is trusted (no GPU needed).
+
+- the reference's own known-answer rows (test/gourdon/D.cpp:36-89,
+  test/deleglise-rivat/S2_hard_alpha.cpp:36-73), both through the oracle and
+  through an independent closed-form brute force (tests/closed_form.py);
+- fixtures produced by the reference build (oracle/_ref) at x = 1e13;
+- invariants: segment-size independence, phi semantics.
+"""
+import random
+
+import pytest
+
+from tests import closed_form
+from tests.conftest import golden
+
+
+def kat_rows(name):
+    return [[int(v) for v in r] for r in golden(name)["rows"]]
+
+
+@pytest.mark.parametrize("row", kat_rows("kat_D.json"), ids=lambda r: "x%d_y%d_z%d" % tuple(r[:3]))
+def test_oracle_D_kat(oracle, row):
+    x, y, z, k, expected = row
+    assert oracle.D(x, y, z, k) == expected
+
+
+@pytest.mark.parametrize("row", kat_rows("kat_S2_hard.json"), ids=lambda r: "x%d_y%d_z%d" % tuple(r[:3]))
+def test_oracle_S2_hard_kat(oracle, row):
+    x, y, z, c, expected = row
+    assert oracle.S2_hard(x, y, z, c) == expected
+
+
+def test_closed_form_matches_kat_rows():
+    for x, y, z, k, expected in kat_rows("kat_D.json"):
+        if x <= 10 ** 8:
+            assert closed_form.D(x, y, z, k) == expected, (x, y, z, k)
+    for x, y, z, c, expected in kat_rows("kat_S2_hard.json"):
+        if x <= 10 ** 8:
+            assert closed_form.S2_hard(x, y, z, c) == expected, (x, y, z, c)
+
+
+def test_oracle_vs_closed_form_random(oracle):
+    rng = random.Random(20261203)
+    for _ in range(40):
+        x = rng.randint(10 ** 5, 3 * 10 ** 7)
+        x13 = round(x ** (1 / 3))
+        y = rng.randint(x13 + 1, 4 * x13)
+        z = rng.randint(y, min(4 * y, int(x ** 0.5) - 1))
+        assert oracle.D(x, y, z, 8, seg=1 << 12) == closed_form.D(x, y, z, 8), (x, y, z)
+        yd = rng.randint(x13, 3 * x13)
+        zd = x // yd
+        assert oracle.S2_hard(x, yd, zd, 8, seg=1 << 12) == closed_form.S2_hard(x, yd, zd, 8), (x, yd)
+
+
+def test_oracle_segment_size_invariance(oracle):
+    x, y, z, k = 10 ** 11, 13825, 13825, 8
+    ref = oracle.D(x, y, z, k, seg=1 << 16)
+    for seg in (1 << 10, 3 * (1 << 11), 1 << 18):
+        assert oracle.D(x, y, z, k, seg=seg) == ref
+
+
+def test_oracle_reference_fixture_1e13(oracle):
+    d = {r["x"]: r for r in golden("configs.json")["D"]}
+    r = d[str(10 ** 13)]
+    assert oracle.D(10 ** 13, int(r["y"]), int(r["z"]), int(r["k"])) == int(r["value"])
+    s = {r["x"]: r for r in golden("configs.json")["S2_hard"]}
+    r = s[str(10 ** 13)]
+    assert oracle.S2_hard(10 ** 13, int(r["y"]), int(r["z"]), int(r["c"])) == int(r["value"])
+
+
+def test_oracle_int128_entry_matches_int64(oracle):
+    # the 128-bit overloads must agree with the 64-bit ones (test/gourdon/D.cpp:104-111)
+    for x, y, z, k, expected in kat_rows("kat_D.json")[20:40]:
+        assert oracle.D(x, y, z, k) == expected
+
+
+def test_phi_direct_small(oracle):
+    # phi(n, a) counts n' <= n free of the first a primes: closed forms
+    assert oracle.phi_direct(100, 0) == 100
+    assert oracle.phi_direct(100, 1) == 50
+    assert oracle.phi_direct(30, 3) == 8          # 1, 7, 11, 13, 17, 19, 23, 29
+    assert oracle.phi_direct(210, 4) == 48        # totient(210)
+    phi = closed_form.make_phi(tuple(closed_form.primes_upto(200)))
+    for n in (1, 2, 29, 30, 31, 97, 1000):
+        for a in range(0, 9):
+            assert oracle.phi_direct(n, a) == phi(n, a)
