@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Benchmark: Gourdon's D(x) at x = 1e19 on one MI355X.
+
+One step = one D(1e19, y, z, k) hard-special-leaf sum with primecount's
+default parameters (y = z = 88560164, k = 8), through the 128-bit entry point
+(x > 2^63, as the reference does), checked bit-exact against the reference
+value on every step.  The prime, pi and factor tables are built on the first
+(warmup) call and stay resident in HBM, so the timed steps measure the hot
+path: planning, the sieve kernel, the carry combine and the read-back.
+
+Prints ONE JSON line.  `roofline` is measured on the sieve kernel with HIP
+events on the engine's stream; `cpu_baseline` times the reference build
+(oracle/_ref/primecount, when present) on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--x 1e19] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+LDS_PEAK_GBS = 256 * 128 * 2.4  # 256 CUs x 128 B/clk x 2.4 GHz = 78,643 GB/s (nominal)
+
+
+def expected_D(x):
+    with open(os.path.join(REPO, "tests", "golden", "configs.json")) as f:
+        for r in json.load(f)["D"]:
+            if int(r["x"]) == x:
+                return int(r["value"]), r
+    big = os.path.join(REPO, "tests", "golden", "configs_big.json")
+    if os.path.exists(big):
+        with open(big) as f:
+            for r in json.load(f)["D"]:
+                if int(r["x"]) == x:
+                    return int(r["value"]), r
+    return None, None
+
+
+def traffic_from_profile(x):
+    """HBM bytes per sieve-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
+    path = os.path.join(REPO, "profiles", "hsl_sieve_kernel_D_1e%d_pmc.md" % round(math.log10(x)))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", line.strip())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def cpu_baseline(x, timeout):
+    cli = os.path.join(REPO, "oracle", "_ref", "primecount")
+    if not os.path.exists(cli):
+        return None
+    cores = os.cpu_count() or 1
+    try:
+        r = subprocess.run([cli, str(x), "--D", "-s", "-t", str(cores)], capture_output=True,
+                           text=True, timeout=timeout)
+    except subprocess.TimeoutExpired:
+        return {"value": None, "unit": "s", "cores": cores, "kind": "reference",
+                "sample": "D(%d) timed out after %d s" % (x, timeout)}
+    secs = None
+    val = None
+    for line in r.stdout.splitlines():
+        line = line.split("\r")[-1]
+        m = re.match(r"^Seconds:\s*([0-9.]+)", line)
+        if m:
+            secs = float(m.group(1))
+        m = re.match(r"^D = (-?\d+)", line)
+        if m:
+            val = int(m.group(1))
+    return {"value": secs, "unit": "s", "cores": cores, "kind": "reference",
+            "sample": "full D(%d) by the reference build (primecount --D -t %d), value %s" %
+                      (x, cores, "matches" if val == expected_D(x)[0] else "MISMATCH")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--x", type=float, default=1e19)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-timeout", type=int, default=120)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: the hard-special-leaf engine runs on a single MI355X "
+              "(multi-GPU sharding is described in DESIGN.md, not built); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import primecount_amd as pc
+    x = int(round(a.x))
+    p = pc.gourdon_params(x)
+    want, _ = expected_D(x)
+    print("bench: D(%d) y=%d z=%d k=%d" % (x, p["y"], p["z"], p["k"]), file=sys.stderr, flush=True)
+
+    t = time.time()
+    for i in range(max(a.warmup, 1)):
+        got = pc.D128(x, p["y"], p["z"], p["k"])
+        if want is not None and got != want:
+            raise SystemExit("warmup result %d != reference %d" % (got, want))
+    first = pc.last_stats()
+    print("bench: warmup done in %.2f s (tables %.2f s)" % (time.time() - t, first["seconds_tables"]),
+          file=sys.stderr, flush=True)
+
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None
+
+    kernel_s, combine_s, wall = [], [], []
+    sync()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        ts = time.perf_counter()
+        got = pc.D128(x, p["y"], p["z"], p["k"])
+        wall.append(time.perf_counter() - ts)
+        st = pc.last_stats()
+        kernel_s.append(st["seconds_kernel"])
+        combine_s.append(st["seconds_combine"])
+        if want is not None and got != want:
+            raise SystemExit("step %d result %d != reference %d" % (i, got, want))
+    sync()
+    elapsed = time.perf_counter() - t0
+    sec_per_step = elapsed / a.steps
+
+    k_avg = sum(kernel_s) / len(kernel_s)
+    lds_bytes = 8.0 * st["hits"] + 12.0 * st["leaves"]
+    hbm_bytes = 2.0 * st["factor_entries"]
+    traffic = traffic_from_profile(x)
+    roofline = {
+        "bound": "hbm",
+        "achieved": round(hbm_bytes / k_avg / 1e9, 3),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(hbm_bytes / k_avg / 1e9 / HBM_PEAK_GBS, 6),
+        "traffic": traffic,
+    }
+    roofline_lds = {
+        "bound": "lds",
+        "achieved": round(lds_bytes / k_avg / 1e9, 3),
+        "peak": LDS_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(lds_bytes / k_avg / 1e9 / LDS_PEAK_GBS, 6),
+        "bytes_per_launch": lds_bytes,
+    }
+    out = {
+        "metric": "D(x) secs at x=1e19",
+        "value": round(sec_per_step, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(1000 * sec_per_step, 3),
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int64/int128",
+        "data": "synthetic: fully determined by x (no dataset)",
+        "config": {
+            "workload": "D(x) single query, x=%d, y=z=%d, k=%d, one query per launch" % (x, p["y"], p["k"]),
+            "x": str(x), "y": p["y"], "z": p["z"], "k": p["k"], "x_star": p["x_star"],
+            "segment_words": st["segment_words"], "segments": st["segments"], "chunks": st["chunks"],
+        },
+        "bit_exact": want is not None,
+        "kernel_ms": round(1000 * k_avg, 3),
+        "combine_ms": round(1000 * sum(combine_s) / len(combine_s), 3),
+        "table_build_s_first_call": round(first["seconds_tables"], 3),
+        "end_to_end_s_first_call": round(first["seconds_total"], 3),
+        "work": {"leaves": st["leaves"], "hits": st["hits"], "factor_entries": st["factor_entries"],
+                 "pairs": st["pairs"]},
+        "roofline": roofline,
+        "roofline_lds": roofline_lds,
+        "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

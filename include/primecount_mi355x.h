@@ -117,7 +117,8 @@ int pcg_set_host_threads(int threads);
 
 typedef struct pcg_stats {
   double seconds_total;      /* wall time of the last D/S2_hard call          */
-  double seconds_tables;     /* host table build + upload                     */
+  double seconds_tables;     /* host table build + upload (0 when resident)   */
+  double seconds_plan;       /* host planning + per-call uploads              */
   double seconds_kernel;     /* sieve kernel (HIP events)                     */
   double seconds_combine;    /* carry scan + combine kernels (HIP events)     */
   uint64_t segments;         /* sieve segments processed                      */
@@ -134,7 +135,8 @@ typedef struct pcg_stats {
 /* Statistics of the last D/S2_hard call on this thread. */
 int pcg_last_stats(pcg_stats* out);
 
-/* Release every device buffer the library holds. */
+/* Release every device buffer the library holds, including the prime and
+ * factor tables that stay resident in HBM between calls. */
 int pcg_release(void);
 
 #ifdef __cplusplus

@@ -43,6 +43,7 @@ class Stats(ctypes.Structure):
     _fields_ = [
         ("seconds_total", ctypes.c_double),
         ("seconds_tables", ctypes.c_double),
+        ("seconds_plan", ctypes.c_double),
         ("seconds_kernel", ctypes.c_double),
         ("seconds_combine", ctypes.c_double),
         ("segments", ctypes.c_uint64),

@@ -294,35 +294,65 @@ uint32_t auto_segment_words(uint64_t limit, uint64_t slots)
   return W;
 }
 
-TableSet build_tables(const std::vector<HslQuery>& qs)
-{
-  TableSet T;
-  T.kind = qs[0].kind;
+/// Table requirements of a group of queries.
+struct TableNeed {
+  LeafSum kind = LeafSum::D;
   uint64_t pmax = 7;
   int64_t fy = 1, flim = 1;
+  bool shared = true;     // D: every query has y >= z (no entry <= z depends on y)
+};
+
+TableNeed table_need(const std::vector<HslQuery>& qs)
+{
+  TableNeed n;
+  n.kind = qs[0].kind;
   for (const auto& q : qs) {
     if (q.x < 1) continue;
-    pmax = std::max(pmax, prime_range(q));
+    n.pmax = std::max(n.pmax, prime_range(q));
     if (q.kind == LeafSum::D) {
-      flim = std::max(flim, q.z);
-      fy = std::max(fy, q.y);
+      n.flim = std::max(n.flim, q.z);
+      n.fy = std::max(n.fy, q.y);
+      n.shared = n.shared && q.y >= q.z;
     } else {
-      flim = std::max(flim, q.y);
-      fy = std::max(fy, q.y);
+      n.flim = std::max(n.flim, q.y);
+      n.fy = std::max(n.fy, q.y);
     }
   }
+  if (n.kind == LeafSum::D && n.shared)
+    n.fy = std::max(n.fy, n.flim);
+  return n;
+}
+
+/// Can the resident tables answer these queries?  Prime tables are prefix
+/// valid; an lpf FactorTable is prefix valid; a FactorTableD built with
+/// fy >= flim is prefix valid for every query with y >= z.
+bool covers(const TableSet& T, const TableNeed& n)
+{
+  if (T.kind != n.kind || T.P.max < n.pmax || T.ft_limit < n.flim)
+    return false;
+  if (n.kind == LeafSum::S2_hard)
+    return true;
+  if (n.shared)
+    return T.ft_y >= T.ft_limit;
+  return T.ft_y == n.fy && T.ft_limit == n.flim;
+}
+
+TableSet build_tables(const TableNeed& n)
+{
+  TableSet T;
+  T.kind = n.kind;
   int threads = engine_config().host_threads > 0 ? engine_config().host_threads : default_threads();
-  T.P = build_prime_tables(pmax, threads);
-  T.ft_y = fy;
-  T.ft_limit = flim;
+  T.P = build_prime_tables(n.pmax, threads);
+  T.ft_y = n.fy;
+  T.ft_limit = n.flim;
   if (T.kind == LeafSum::D) {
-    T.factor32 = flim > kFactorTableD16Max;
-    if (T.factor32) T.f32 = build_factor_table_D<uint32_t>(fy, flim, T.P, threads);
-    else T.f16 = build_factor_table_D<uint16_t>(fy, flim, T.P, threads);
+    T.factor32 = n.flim > kFactorTableD16Max;
+    if (T.factor32) T.f32 = build_factor_table_D<uint32_t>(n.fy, n.flim, T.P, threads);
+    else T.f16 = build_factor_table_D<uint16_t>(n.fy, n.flim, T.P, threads);
   } else {
-    T.factor32 = flim > kFactorTable16Max;
-    if (T.factor32) T.f32 = build_factor_table_lpf<uint32_t>(flim, T.P, threads);
-    else T.f16 = build_factor_table_lpf<uint16_t>(flim, T.P, threads);
+    T.factor32 = n.flim > kFactorTable16Max;
+    if (T.factor32) T.f32 = build_factor_table_lpf<uint32_t>(n.flim, T.P, threads);
+    else T.f16 = build_factor_table_lpf<uint16_t>(n.flim, T.P, threads);
   }
   return T;
 }
@@ -331,6 +361,29 @@ void upload(DevBuf& b, const void* src, size_t bytes)
 {
   b.ensure(bytes);
   if (bytes) PCG_HIP(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice));
+}
+
+/// Tables resident in HBM from an earlier call, rebuilt only when they do
+/// not cover the new queries.  Caller holds g_mutex.
+TableSet g_tables;
+bool g_tables_valid = false;
+
+const TableSet& resident_tables(const std::vector<HslQuery>& qs)
+{
+  TableNeed need = table_need(qs);
+  if (g_tables_valid && covers(g_tables, need))
+    return g_tables;
+  g_tables_valid = false;
+  g_tables = build_tables(need);
+  Device& D = g_dev;
+  const TableSet& T = g_tables;
+  upload(D.primes, T.P.primes.data(), T.P.primes.size() * 4);
+  upload(D.pi_bits, T.P.pi_bits.data(), T.P.pi_bits.size() * 8);
+  upload(D.pi_cnt, T.P.pi_cnt.data(), T.P.pi_cnt.size() * 4);
+  if (T.factor32) upload(D.factor, T.f32.data(), T.f32.size() * 4);
+  else upload(D.factor, T.f16.data(), T.f16.size() * 2);
+  g_tables_valid = true;
+  return g_tables;
 }
 
 /// Run a set of queries that share one table set.  Caller holds g_mutex.
@@ -346,7 +399,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   for (const auto& q : qs) any |= q.x >= 1;
   if (!any) return result;
 
-  TableSet T = build_tables(qs);
+  const TableSet& T = resident_tables(qs);
+  double t_tab = now();
 
   int bpc = 0;
   uint64_t max_limit = 1;
@@ -429,12 +483,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   uint32_t nslices = (max_nch + slice_len - 1) / slice_len;
   const uint32_t ncols = (uint32_t) cols.size();
 
-  // uploads
-  upload(D.primes, T.P.primes.data(), T.P.primes.size() * 4);
-  upload(D.pi_bits, T.P.pi_bits.data(), T.P.pi_bits.size() * 8);
-  upload(D.pi_cnt, T.P.pi_cnt.data(), T.P.pi_cnt.size() * 4);
-  if (T.factor32) upload(D.factor, T.f32.data(), T.f32.size() * 4);
-  else upload(D.factor, T.f16.data(), T.f16.size() * 2);
+  // per-call uploads (the tables are resident)
   upload(D.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
   upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
   upload(D.chunks, chunks.data(), chunks.size() * sizeof(ChunkDesc));
@@ -502,7 +551,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     PCG_HIP(hipEventElapsedTime(&ms_k, D.ev[0], D.ev[1]));
     PCG_HIP(hipEventElapsedTime(&ms_c, D.ev[1], D.ev[2]));
     st->seconds_total = now() - t0;
-    st->seconds_tables = t1 - t0;
+    st->seconds_tables = t_tab - t0;
+    st->seconds_plan = t1 - t_tab;
     st->seconds_kernel = ms_k / 1000.0;
     st->seconds_combine = ms_c / 1000.0;
     uint64_t nseg = 0;
@@ -560,6 +610,7 @@ std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* 
     auto r = run_group(sub, &st);
     for (size_t j = 0; j < g.size(); j++) out[g[j]] = r[j];
     acc.seconds_tables += st.seconds_tables;
+    acc.seconds_plan += st.seconds_plan;
     acc.seconds_kernel += st.seconds_kernel;
     acc.seconds_combine += st.seconds_combine;
     acc.segments += st.segments;
@@ -590,6 +641,8 @@ void release_device()
 {
   std::lock_guard<std::mutex> lock(g_mutex);
   g_dev.release();
+  g_tables = TableSet();
+  g_tables_valid = false;
 }
 
 } // namespace pcg
