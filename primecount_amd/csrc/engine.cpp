@@ -78,8 +78,8 @@ struct Device {
   int cus = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {};
-  DevBuf queries, chunks, order, binfo, primes, factor, pi_bits, pi_cnt;
-  DevBuf tc, tm, s_out, counters, queue, cols, sc, sm, smp, out;
+  DevBuf queries, items, order, binfo, primes, factor, pi_bits, pi_cnt;
+  DevBuf tc, tm, s_out, counters, queue, cols, sc, sm, smp, out, carry;
 
   void init()
   {
@@ -103,8 +103,8 @@ struct Device {
 
   void release()
   {
-    for (DevBuf* b : {&queries, &chunks, &order, &binfo, &primes, &factor, &pi_bits, &pi_cnt,
-                      &tc, &tm, &s_out, &counters, &queue, &cols, &sc, &sm, &smp, &out})
+    for (DevBuf* b : {&queries, &items, &order, &binfo, &primes, &factor, &pi_bits, &pi_cnt,
+                      &tc, &tm, &s_out, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry})
       b->release();
   }
 };
@@ -127,11 +127,13 @@ struct QueryPlan {
   bool empty = true;
   QueryDesc qd{};
   std::vector<BInfo> binfo;
-  std::vector<ChunkDesc> chunks;   // tc_off relative to the query
-  double est_work = 0;
+  std::vector<uint32_t> nbs;       // per segment: slots (b values) it can reach
 };
 
 uint64_t imin(uint64_t a, uint64_t b) { return a < b ? a : b; }
+
+/// Slot budget of one pass (C and M take 8 bytes per slot).
+constexpr uint64_t kMaxSlotsPerPass = (uint64_t) 1 << 30;
 
 /// Prime-table range a query needs.
 uint64_t prime_range(const HslQuery& q)
@@ -243,44 +245,28 @@ QueryPlan plan_query(const HslQuery& q, const PrimeTables& P)
   return qp;
 }
 
-/// Cut the query's segments into chunks of about `target` estimated steps.
-/// A segment starting at `low` can only process b with p_b <= sqrt(x / low)
-/// (beyond that its leaf range is empty and the kernel stops), so
-/// pi(min(p_{b_last}, sqrt(x / low))) - b_first + 1 bounds the b it stores.
-void plan_chunks(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W,
-                 double target_chunks)
+/// Slots per segment.  A segment starting at `low` can only process b with
+/// p_b <= sqrt(x / low) (beyond that its leaf range is empty and the kernel
+/// stops), so pi(min(p_{b_last}, sqrt(x / low))) - b_first + 1 bounds the b
+/// it reaches.  The bound decreases with low; trailing segments that reach
+/// no b hold no leaves and are dropped.
+void plan_segments(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W)
 {
   const uint64_t SEG = 240ull * W;
   const uint64_t nseg = (qp.qd.limit + SEG - 1) / SEG;
   const uint64_t nb = qp.qd.b_last - qp.qd.b_first + 1;
   const uint64_t p_last = P.primes[qp.qd.b_last];
-  std::vector<uint32_t> est;
-  est.reserve(nseg);
-  double total = 0;
+  qp.nbs.clear();
+  qp.nbs.reserve(nseg);
+  uint64_t bb = qp.qd.b_last;                 // walks down as low grows
   for (uint64_t s = 0; s < nseg; s++) {
     uint64_t low1 = std::max<uint64_t>(1, s * SEG);
     uint64_t r = isqrt((u128) q.x / low1);
-    uint64_t bb = P.pi(imin(p_last, r));
+    if (r < p_last)
+      while (bb > 0 && P.primes[bb] > r) bb--;
     uint64_t e = bb >= qp.qd.b_first ? std::min<uint64_t>(nb, bb - qp.qd.b_first + 1) : 0;
     if (e == 0) break;
-    est.push_back((uint32_t) e);
-    total += e;
-  }
-  qp.est_work = total;
-  const double target = std::max(1.0, total / std::max(1.0, target_chunks));
-  const uint64_t max_len = std::max<uint64_t>(1, ((uint64_t) 1 << 31) / SEG);
-  uint64_t off = 0;
-  for (uint64_t s = 0; s < est.size();) {
-    ChunkDesc c{};
-    c.seg_begin = (uint32_t) s;
-    c.nbc = est[s];
-    double acc = 0;
-    while (s < est.size() && (acc < target || s == c.seg_begin) && s - c.seg_begin < max_len)
-      acc += est[s++];
-    c.seg_end = (uint32_t) s;
-    c.tc_off = off;
-    off += c.nbc;
-    qp.chunks.push_back(c);
+    qp.nbs.push_back((uint32_t) e);
   }
 }
 
@@ -387,6 +373,11 @@ const TableSet& resident_tables(const std::vector<HslQuery>& qs)
 }
 
 /// Run a set of queries that share one table set.  Caller holds g_mutex.
+///
+/// Work items are (query, segment) pairs; their (segment, b) slots are
+/// written by the sieve kernel and folded by the combine kernels.  Items are
+/// processed in passes of at most kMaxSlotsPerPass slots so that the slot
+/// arrays stay bounded; the per-column carry survives between passes.
 std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
 {
   Device& D = g_dev;
@@ -402,163 +393,178 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   const TableSet& T = resident_tables(qs);
   double t_tab = now();
 
-  int bpc = 0;
-  uint64_t max_limit = 1;
   std::vector<QueryPlan> plans(qs.size());
+  uint64_t max_limit = 1;
   for (size_t i = 0; i < qs.size(); i++) {
     plans[i] = plan_query(qs[i], T.P);
     if (!plans[i].empty) max_limit = std::max(max_limit, plans[i].qd.limit);
   }
-  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 4);
+  int bpc = 0;
+  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 6);
   if (W < 256 || W > 4096 || (W & (W - 1)))
     throw pcg_error("segment words must be a power of two in [256, 4096]");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
   bpc = std::max(bpc, 1);
-  const uint64_t slots = (uint64_t) D.cus * bpc;
+  const uint64_t slots_wg = (uint64_t) D.cus * bpc;
 
-  // chunks: each query gets a share of the chunk budget by estimated work
-  double total_work = 0;
-  for (size_t i = 0; i < qs.size(); i++)
-    if (!plans[i].empty) {
-      plan_chunks(plans[i], qs[i], T.P, W, 1.0);       // measure work
-      total_work += plans[i].est_work;
-      plans[i].chunks.clear();
-    }
-  const double budget = 6.0 * slots + 4.0 * qs.size();
+  // queries, per-b constants, items and columns
   std::vector<QueryDesc> qdesc;
   std::vector<BInfo> binfo;
-  std::vector<ChunkDesc> chunks;
-  std::vector<uint32_t> chunk_first(qs.size(), 0), nchunks_q(qs.size(), 0);
   std::vector<uint32_t> qslot(qs.size(), UINT32_MAX);
-  uint64_t tc_total = 0;
+  std::vector<uint32_t> colbase;                 // per slot: first global column
+  struct ItemRef { uint32_t slot, seg, nbs; };
+  std::vector<ItemRef> all;                      // (query, segment) in order
+  uint32_t ncols_total = 0;
   for (size_t i = 0; i < qs.size(); i++) {
     if (plans[i].empty) continue;
-    double share = std::max(1.0, budget * plans[i].est_work / std::max(1.0, total_work));
-    plan_chunks(plans[i], qs[i], T.P, W, share);
+    plan_segments(plans[i], qs[i], T.P, W);
+    if (plans[i].nbs.empty()) continue;
     QueryDesc qd = plans[i].qd;
     qd.binfo_off = (uint32_t) binfo.size();
     qslot[i] = (uint32_t) qdesc.size();
     qdesc.push_back(qd);
     binfo.insert(binfo.end(), plans[i].binfo.begin(), plans[i].binfo.end());
-    chunk_first[i] = (uint32_t) chunks.size();
-    nchunks_q[i] = (uint32_t) plans[i].chunks.size();
-    for (ChunkDesc c : plans[i].chunks) {
-      c.query = qslot[i];
-      c.tc_off += tc_total;
-      chunks.push_back(c);
-    }
-    tc_total += plans[i].chunks.empty() ? 0
-              : plans[i].chunks.back().tc_off + plans[i].chunks.back().nbc;
+    colbase.push_back(ncols_total);
+    ncols_total += plans[i].nbs[0];
+    for (uint32_t sgi = 0; sgi < plans[i].nbs.size(); sgi++)
+      all.push_back(ItemRef{qslot[i], sgi, plans[i].nbs[sgi]});
   }
-  if (chunks.empty()) return result;
+  if (all.empty()) return result;
 
-  // schedule: longest estimated chunks first across queries
-  std::vector<double> cost(chunks.size());
-  for (size_t c = 0; c < chunks.size(); c++)
-    cost[c] = (double) chunks[c].nbc * (chunks[c].seg_end - chunks[c].seg_begin);
-  std::vector<uint32_t> order(chunks.size());
-  std::iota(order.begin(), order.end(), 0);
-  if (qdesc.size() > 1)
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-
-  // combine columns
-  std::vector<CombineCol> cols;
-  std::vector<uint32_t> col_query;
-  uint32_t max_nch = 1;
-  for (size_t i = 0; i < qs.size(); i++) {
-    if (plans[i].empty) continue;
-    const auto& pc = plans[i].chunks;
-    uint32_t nb = qdesc[qslot[i]].b_last - qdesc[qslot[i]].b_first + 1;
-    uint32_t nch = (uint32_t) pc.size();
-    for (uint32_t br = 0; br < nb; br++) {
-      while (nch > 0 && pc[nch - 1].nbc <= br) nch--;
-      if (nch == 0) break;
-      cols.push_back(CombineCol{br, chunk_first[i], nch, qslot[i]});
-      col_query.push_back((uint32_t) i);
-      max_nch = std::max(max_nch, nch);
-    }
-  }
-  uint32_t slice_len = std::max<uint32_t>(1, (max_nch + 63) / 64);
-  uint32_t nslices = (max_nch + slice_len - 1) / slice_len;
-  const uint32_t ncols = (uint32_t) cols.size();
-
-  // per-call uploads (the tables are resident)
   upload(D.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
   upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
-  upload(D.chunks, chunks.data(), chunks.size() * sizeof(ChunkDesc));
-  upload(D.order, order.data(), order.size() * 4);
-  upload(D.cols, cols.data(), cols.size() * sizeof(CombineCol));
-  D.tc.ensure(tc_total * 4);
-  D.tm.ensure(tc_total * 4);
-  D.s_out.ensure(chunks.size() * 16);
+  D.carry.ensure((size_t) ncols_total * 8);
+  D.out.ensure((size_t) ncols_total * 16);
   D.counters.ensure(64);
-  D.queue.ensure(64);
-  D.sc.ensure((size_t) nslices * ncols * 8);
-  D.sm.ensure((size_t) nslices * ncols * 8);
-  D.smp.ensure((size_t) nslices * ncols * 16);
-  D.out.ensure((size_t) ncols * 16);
-  double t1 = now();
-
   hipStream_t s = D.stream;
-  PCG_HIP(hsl_launch_fill_zero(D.tc.p, tc_total * 4, s));
-  PCG_HIP(hsl_launch_fill_zero(D.tm.p, tc_total * 4, s));
+  PCG_HIP(hsl_launch_fill_zero(D.carry.p, (size_t) ncols_total * 8, s));
+  PCG_HIP(hsl_launch_fill_zero(D.out.p, (size_t) ncols_total * 16, s));
   PCG_HIP(hsl_launch_fill_zero(D.counters.p, 64, s));
-  PCG_HIP(hsl_launch_fill_zero(D.queue.p, 64, s));
 
-  HslArgs a{};
-  a.queries = (const QueryDesc*) D.queries.p;
-  a.chunks = (const ChunkDesc*) D.chunks.p;
-  a.order = (const uint32_t*) D.order.p;
-  a.binfo = (const BInfo*) D.binfo.p;
-  a.primes = (const uint32_t*) D.primes.p;
-  a.factor = D.factor.p;
-  a.pi_bits = (const uint64_t*) D.pi_bits.p;
-  a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
-  a.T_C = (uint32_t*) D.tc.p;
-  a.T_M = (int32_t*) D.tm.p;
-  a.S_out = (unsigned long long*) D.s_out.p;
-  a.counters = (unsigned long long*) D.counters.p;
-  a.queue = (unsigned int*) D.queue.p;
-  a.nchunks = (uint32_t) chunks.size();
-  a.W = W;
-  uint32_t grid = (uint32_t) std::min<uint64_t>(slots, chunks.size());
+  std::vector<long long> s_item(all.size(), 0);
+  double t_plan = 0, ms_kernel = 0, ms_combine = 0;
+  uint64_t max_slots = engine_config().max_slots_per_pass;
+  if (max_slots == 0) max_slots = kMaxSlotsPerPass;
 
-  PCG_HIP(hipEventRecord(D.ev[0], s));
-  PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
-  PCG_HIP(hipEventRecord(D.ev[1], s));
-  PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const ChunkDesc*) D.chunks.p,
-                             (const uint32_t*) D.tc.p, (const int32_t*) D.tm.p, slice_len, nslices,
-                             (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p, D.out.p, s));
-  PCG_HIP(hipEventRecord(D.ev[2], s));
+  for (size_t pass_begin = 0; pass_begin < all.size();) {
+    double tp = now();
+    // this pass: items [pass_begin, pass_end)
+    size_t pass_end = pass_begin;
+    uint64_t slots = 0;
+    while (pass_end < all.size() && (pass_end == pass_begin || slots + all[pass_end].nbs <= max_slots))
+      slots += all[pass_end++].nbs;
+    const uint32_t nitems = (uint32_t) (pass_end - pass_begin);
+    std::vector<SegItem> items(nitems);
+    uint64_t off = 0;
+    for (uint32_t i = 0; i < nitems; i++) {
+      const ItemRef& r = all[pass_begin + i];
+      SegItem& it = items[i];
+      std::memset(&it, 0, sizeof(it));
+      it.query = r.slot;
+      it.seg = r.seg;
+      it.nbs = r.nbs;
+      it.t_off = off;
+      off += r.nbs;
+    }
+    // schedule: segments with the most slots first (one query: natural order)
+    std::vector<uint32_t> order(nitems);
+    std::iota(order.begin(), order.end(), 0);
+    if (qdesc.size() > 1)
+      std::stable_sort(order.begin(), order.end(),
+                       [&](uint32_t a, uint32_t b) { return items[a].nbs > items[b].nbs; });
+    // combine columns of this pass
+    std::vector<CombineCol> cols;
+    uint32_t max_n = 1;
+    for (uint32_t i = 0; i < nitems;) {
+      uint32_t slot = items[i].query, j = i;
+      while (j < nitems && items[j].query == slot) j++;
+      // items [i, j) are consecutive segments of one query; nbs non-increasing
+      uint32_t n = j - i;
+      for (uint32_t br = 0; br < items[i].nbs; br++) {
+        while (n > 0 && items[i + n - 1].nbs <= br) n--;
+        if (n == 0) break;
+        cols.push_back(CombineCol{br, i, n, colbase[slot] + br});
+        max_n = std::max(max_n, n);
+      }
+      i = j;
+    }
+    uint32_t slice_len = std::max<uint32_t>(1, (max_n + 127) / 128);
+    uint32_t nslices = (max_n + slice_len - 1) / slice_len;
+    const uint32_t ncols = (uint32_t) cols.size();
 
-  std::vector<i128> s_out(chunks.size());
-  std::vector<i128> col_out(ncols);
+    upload(D.items, items.data(), items.size() * sizeof(SegItem));
+    upload(D.order, order.data(), order.size() * 4);
+    upload(D.cols, cols.data(), cols.size() * sizeof(CombineCol));
+    D.tc.ensure(slots * 4);
+    D.tm.ensure(slots * 4);
+    D.s_out.ensure((size_t) nitems * 8);
+    D.queue.ensure(64);
+    D.sc.ensure((size_t) nslices * ncols * 8);
+    D.sm.ensure((size_t) nslices * ncols * 8);
+    D.smp.ensure((size_t) nslices * ncols * 16);
+    PCG_HIP(hsl_launch_fill_zero(D.queue.p, 64, s));
+    t_plan += now() - tp;
+
+    HslArgs a{};
+    a.queries = (const QueryDesc*) D.queries.p;
+    a.items = (const SegItem*) D.items.p;
+    a.order = (const uint32_t*) D.order.p;
+    a.binfo = (const BInfo*) D.binfo.p;
+    a.primes = (const uint32_t*) D.primes.p;
+    a.factor = D.factor.p;
+    a.pi_bits = (const uint64_t*) D.pi_bits.p;
+    a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
+    a.T_C = (uint32_t*) D.tc.p;
+    a.T_M = (int32_t*) D.tm.p;
+    a.S_out = (long long*) D.s_out.p;
+    a.counters = (unsigned long long*) D.counters.p;
+    a.queue = (unsigned int*) D.queue.p;
+    a.nitems = nitems;
+    a.W = W;
+    uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, nitems);
+
+    PCG_HIP(hipEventRecord(D.ev[0], s));
+    PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
+    PCG_HIP(hipEventRecord(D.ev[1], s));
+    PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const SegItem*) D.items.p,
+                               (const uint32_t*) D.tc.p, (const int32_t*) D.tm.p, slice_len,
+                               nslices, (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p,
+                               (long long*) D.carry.p, D.out.p, s));
+    PCG_HIP(hipEventRecord(D.ev[2], s));
+    PCG_HIP(hipMemcpyAsync(s_item.data() + pass_begin, D.s_out.p, (size_t) nitems * 8,
+                           hipMemcpyDeviceToHost, s));
+    PCG_HIP(hipStreamSynchronize(s));
+    float mk = 0, mc = 0;
+    PCG_HIP(hipEventElapsedTime(&mk, D.ev[0], D.ev[1]));
+    PCG_HIP(hipEventElapsedTime(&mc, D.ev[1], D.ev[2]));
+    ms_kernel += mk;
+    ms_combine += mc;
+    pass_begin = pass_end;
+  }
+
+  std::vector<i128> col_out(ncols_total);
   uint64_t counters[4] = {0, 0, 0, 0};
-  PCG_HIP(hipMemcpyAsync(s_out.data(), D.s_out.p, chunks.size() * 16, hipMemcpyDeviceToHost, s));
-  PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, D.counters.p, 32, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
 
   std::vector<i128> per_slot(qdesc.size(), 0);
-  for (size_t c = 0; c < chunks.size(); c++) per_slot[chunks[c].query] += s_out[c];
-  for (uint32_t c = 0; c < ncols; c++) per_slot[cols[c].query] += col_out[c];
+  for (size_t i = 0; i < all.size(); i++) per_slot[all[i].slot] += s_item[i];
+  for (size_t q = 0; q < qdesc.size(); q++) {
+    uint32_t ncol_q = (q + 1 < qdesc.size() ? colbase[q + 1] : ncols_total) - colbase[q];
+    for (uint32_t c = 0; c < ncol_q; c++) per_slot[q] += col_out[colbase[q] + c];
+  }
   for (size_t i = 0; i < qs.size(); i++)
     if (qslot[i] != UINT32_MAX) result[i] = per_slot[qslot[i]];
 
   if (st) {
-    float ms_k = 0, ms_c = 0;
-    PCG_HIP(hipEventElapsedTime(&ms_k, D.ev[0], D.ev[1]));
-    PCG_HIP(hipEventElapsedTime(&ms_c, D.ev[1], D.ev[2]));
     st->seconds_total = now() - t0;
     st->seconds_tables = t_tab - t0;
-    st->seconds_plan = t1 - t_tab;
-    st->seconds_kernel = ms_k / 1000.0;
-    st->seconds_combine = ms_c / 1000.0;
-    uint64_t nseg = 0;
-    for (const auto& c : chunks) nseg += c.seg_end - c.seg_begin;
-    st->segments = nseg;
-    st->chunks = chunks.size();
+    st->seconds_plan = t_plan;
+    st->seconds_kernel = ms_kernel / 1000.0;
+    st->seconds_combine = ms_combine / 1000.0;
+    st->segments = all.size();
+    st->chunks = all.size();
     st->segment_words = W;
     st->sieve_limit = qdesc.size() == 1 ? qdesc[0].limit : 0;
     st->b_first = qdesc.size() == 1 ? qdesc[0].b_first : 0;

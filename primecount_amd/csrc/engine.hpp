@@ -22,6 +22,7 @@ struct HslQuery {
 struct EngineConfig {
   uint32_t segment_words = 0;   // 0 = automatic
   int host_threads = 0;         // 0 = all hardware threads
+  uint64_t max_slots_per_pass = 0;   // 0 = default (2^30 slots, 8 GiB)
 };
 
 EngineConfig& engine_config();

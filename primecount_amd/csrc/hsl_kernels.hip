@@ -1,24 +1,30 @@
 // hsl_kernels.hip -- CDNA4 (gfx950) kernels of the hard-special-leaf engine.
 //
-// One workgroup of 256 threads owns one segment of the sieve interval at a
-// time, held in LDS as a wheel-30 bitmap (240 numbers per 64-bit word, the
-// layout of src/sieve/Sieve.cpp:62-66).  For b = b_first, b_first+1, ... the
-// workgroup
-//   1. answers the leaves of p_b in the segment: each thread takes factor-table
-//      entries (or primes) strided by 256, filters them, computes the exact
-//      quotient x / (p_b * m) and counts the unsieved numbers <= it with a
-//      three-level count (super-group prefix, group counters, popcount);
-//   2. records C(s, b), the unsieved count of the segment before p_b is
-//      crossed off, and M(s, b), the sum of the leaf signs;
-//   3. crosses off p_b and its multiples with LDS atomics, decrementing the
-//      group counters for every number removed for the first time
+// A persistent grid of 256-thread workgroups takes segments from a device
+// work queue.  A workgroup holds its segment in LDS as a wheel-30 bitmap (240
+// numbers per 64-bit word, the layout of src/sieve/Sieve.cpp:62-66) plus
+// 16-bit counters of the unsieved numbers per group of 8 words, and walks the
+// sieving primes b = b_first, b_first + 1, ... in epochs:
+//   1. every wave scans the group counters into super-group (64 word)
+//      prefixes; the total is C(s, b), the count of unsieved numbers before
+//      p_b is crossed off;
+//   2. the leaves of p_b in the segment are answered: factor-table entries
+//      (or primes) of the leaf range are filtered, the exact quotient
+//      n = x / (p_b * m) is formed, and the unsieved numbers <= n are counted
+//      from the prefixes, the counters and popcounts of at most 8 words;
+//      M(s, b) is the sum of the leaf signs;
+//   3. p_b and its multiples are crossed off with LDS atomics, decrementing
+//      the group counter of every number removed for the first time
 //      (src/sieve/Sieve.cpp:497-845 counts the same first-time removals).
+// Epochs are separated by LDS-only barriers, so the global loads of the next
+// prime's factor-table entries stay in flight across them.  The leaf ranges of
+// up to 256 consecutive primes are computed in one parallel step per window.
+//
 // The reference carries phi(low, b-1) sequentially through its segments
-// (src/gourdon/D.cpp:96, 271).  Here the carry is split: a workgroup keeps
-// the carry within its chunk, and phi at the chunk start is an exclusive
-// scan over chunks of the per-chunk counts, done by hsl_combine_* afterwards:
-//   sum = sum_leaves sign * (phi_chunk(b) + phi_local(b) + count)
-//       = sum_chunks [ S(chunk) + sum_b M(chunk, b) * phi_chunk(b) ].
+// (src/gourdon/D.cpp:96, 271).  Here each segment writes its C(s, b) and
+// M(s, b) to HBM, and hsl_combine_* add the carry afterwards:
+//   sum = sum_leaves sign * (phi(low_s - 1, b - 1) + count)
+//       = sum_s S(s) + sum_b sum_s M(s, b) * sum_{s' < s} C(s', b).
 
 #include <hip/hip_runtime.h>
 
@@ -36,6 +42,43 @@ __constant__ uint8_t c_wheel_bit[30];     // residue -> bit in byte (0xff: none)
 namespace {
 
 constexpr int kWaves = kBlock / 64;
+constexpr int kPf = 2;              // factor entries per thread prefetched one prime ahead
+constexpr uint32_t kWin = kBlock;   // primes per leaf-range window
+
+/// Nibble table: number of wheel residues {1,7,11,...,29} that are <= r.
+constexpr uint64_t le_nibbles(int first)
+{
+  uint64_t v = 0;
+  for (int r = first; r < first + 16 && r < 30; r++) {
+    const int w[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+    int k = 0;
+    for (int i = 0; i < 8; i++)
+      k += w[i] <= r;
+    v |= (uint64_t) k << (4 * (r - first));
+  }
+  return v;
+}
+constexpr uint64_t kLeLo = le_nibbles(0);
+constexpr uint64_t kLeHi = le_nibbles(16);
+
+/// Bits of a sieve word that stand for the offsets <= r (r < 240).
+__device__ __forceinline__ uint64_t le_mask(uint32_t r)
+{
+  uint32_t j = r / 30;
+  uint32_t rem = r - 30 * j;
+  uint32_t k = (uint32_t) ((rem < 16 ? (kLeLo >> (4 * rem)) : (kLeHi >> (4 * (rem - 16)))) & 15);
+  uint64_t below = (1ull << (8 * j)) - 1;
+  return below | ((((uint64_t) 1 << k) - 1) << (8 * j));
+}
+
+/// LDS-only workgroup barrier: waits for this wave's LDS operations, not for
+/// its global loads and stores, which stay in flight across the barrier.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 /// floor(xp / d) for a quotient < 2^50 and d < 2^53.  xd is xp rounded to
 /// double; the IEEE division estimate is within one of the true quotient
@@ -78,14 +121,13 @@ __device__ __forceinline__ uint64_t ft_to_index(uint64_t n)
 
 /// pi(n) for n within the prime table.
 __device__ __forceinline__ uint64_t pi_lookup(const uint64_t* __restrict__ bits,
-                                              const uint32_t* __restrict__ cnt,
-                                              const uint64_t* lemask, uint64_t n)
+                                              const uint32_t* __restrict__ cnt, uint64_t n)
 {
   if (n < 7)
     return n < 2 ? 0 : (n < 3 ? 1 : (n < 5 ? 2 : 3));
   uint64_t blk = n / 240;
   uint32_t r = (uint32_t) (n - blk * 240);
-  return 3 + (uint64_t) cnt[blk] + (uint64_t) __popcll(bits[blk] & lemask[r]);
+  return 3 + (uint64_t) cnt[blk] + (uint64_t) __popcll(bits[blk] & le_mask(r));
 }
 
 __device__ __forceinline__ int wave_sum_i32(int v)
@@ -102,31 +144,58 @@ __device__ __forceinline__ long long wave_sum_i64(long long v)
   return v;
 }
 
-/// Unsieved numbers in [low, low + stop] of the segment.
+/// Inclusive prefix sum over the 64 lanes of a wave with DPP row shifts and
+/// row broadcasts (no LDS round trips).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
+{
+  // row_shr:1,2,4,8 inside each row of 16 lanes (lanes without a source add 0)
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false);
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false);
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false);
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false);
+  // row_bcast:15 adds lane 15 of a row to the next row, row_bcast:31 adds
+  // lane 31 to rows 2 and 3
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false);
+  v += (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t sum8(uint4 q)
+{
+  return (q.x & 0xffff) + (q.x >> 16) + (q.y & 0xffff) + (q.y >> 16) +
+         (q.z & 0xffff) + (q.z >> 16) + (q.w & 0xffff) + (q.w >> 16);
+}
+
+/// Unsieved numbers in [low, low + stop] of the segment.  pre[sg] is the
+/// count before super-group sg (64 words); the group counters of sg and the
+/// words of the group are read with 128-bit LDS loads, all independent.
 __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t* cnt1,
-                                              const uint32_t* pre2, const uint64_t* lemask,
-                                              uint32_t stop)
+                                              const uint32_t* pre, uint32_t stop)
 {
   uint32_t w = stop / 240;
   uint32_t r = stop - w * 240;
   uint32_t g = w >> 3;
   uint32_t sg = w >> 6;
-  uint32_t c = pre2[sg];
+  uint32_t c = pre[sg];
   uint4 v = reinterpret_cast<const uint4*>(cnt1)[sg];
+  const uint4* gw = reinterpret_cast<const uint4*>(bm) + 4 * g;   // 8 words of group g
+  uint4 w01 = gw[0], w23 = gw[1], w45 = gw[2], w67 = gw[3];
   uint32_t ng = g & 7;
-  uint32_t h[8] = {v.x & 0xffff, v.x >> 16, v.y & 0xffff, v.y >> 16,
-                   v.z & 0xffff, v.z >> 16, v.w & 0xffff, v.w >> 16};
+  uint32_t h[7] = {v.x & 0xffff, v.x >> 16, v.y & 0xffff, v.y >> 16,
+                   v.z & 0xffff, v.z >> 16, v.w & 0xffff};
 #pragma unroll
   for (uint32_t i = 0; i < 7; i++)
     c += i < ng ? h[i] : 0;
-  const uint2* words = reinterpret_cast<const uint2*>(bm);
-  for (uint32_t u = g << 3; u < w; u++) {
-    uint2 x = words[u];
-    c += __popc(x.x) + __popc(x.y);
+  uint32_t wi = w & 7;
+  uint32_t lo[8] = {w01.x, w01.z, w23.x, w23.z, w45.x, w45.z, w67.x, w67.z};
+  uint32_t hi[8] = {w01.y, w01.w, w23.y, w23.w, w45.y, w45.w, w67.y, w67.w};
+  uint64_t m = le_mask(r);
+#pragma unroll
+  for (uint32_t u = 0; u < 8; u++) {
+    uint32_t ml = u < wi ? ~0u : (u == wi ? (uint32_t) m : 0u);
+    uint32_t mh = u < wi ? ~0u : (u == wi ? (uint32_t) (m >> 32) : 0u);
+    c += __popc(lo[u] & ml) + __popc(hi[u] & mh);
   }
-  uint2 x = words[w];
-  uint64_t m = lemask[r];
-  c += __popc(x.x & (uint32_t) m) + __popc(x.y & (uint32_t) (m >> 32));
   return c;
 }
 
@@ -134,37 +203,36 @@ __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t
 /// coprime to 30, as Sieve::add/cross_off, src/sieve/Sieve.cpp:192-216).
 /// Threads split the 8 residue classes of q mod 30 and stride inside a class:
 /// consecutive multiples of one class are p bytes apart.  With COUNT the
-/// first-time removals decrement the group counters and are returned.
+/// first-time removals decrement the group counters.
 template <bool COUNT>
-__device__ __forceinline__ uint32_t cross_off(uint32_t p, uint64_t low, uint32_t seg_len,
-                                              uint32_t* bm, uint32_t* cnt1,
-                                              unsigned long long& hits)
+__device__ __forceinline__ void cross_off(uint32_t p, uint64_t low, uint32_t seg_len,
+                                          uint32_t* bm, uint32_t* cnt1,
+                                          unsigned long long& hits)
 {
   if (p < 7)
-    return 0;
+    return;
   const uint32_t cls = threadIdx.x & 7;
   const uint32_t j0 = threadIdx.x >> 3;
   uint64_t q_lo = div_exact((double) low, low, p) + 1;
-  uint64_t q_hi = div_exact((double) (low + seg_len - 1), low + seg_len - 1, p);
+  uint64_t last = low + seg_len - 1;
+  uint64_t q_hi = div_exact((double) last, last, p);
   if (q_hi < q_lo)
-    return 0;
+    return;
   uint64_t q = q_lo - q_lo % 30 + c_wheel[cls];
   if (q < q_lo)
     q += 30;
   if (q > q_hi)
-    return 0;
+    return;
   uint32_t n = (uint32_t) ((q_hi - q) / 30) + 1;
-  uint64_t off = (uint64_t) p * q - low;
-  uint32_t byte0 = (uint32_t) (off / 30);
-  uint32_t bib = c_wheel_bit[off - (uint64_t) byte0 * 30];
-  uint32_t removed = 0;
+  uint32_t off = (uint32_t) ((uint64_t) p * q - low);
+  uint32_t byte0 = off / 30;
+  uint32_t bib = c_wheel_bit[off - byte0 * 30];
   for (uint32_t t = j0; t < n; t += kBlock / 8) {
     uint32_t B = byte0 + t * p;
     uint32_t mask = 1u << (((B & 3) << 3) | bib);
     if (COUNT) {
       uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
       if (old & mask) {
-        removed++;
         uint32_t g = B >> 6;
         atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
       }
@@ -174,246 +242,266 @@ __device__ __forceinline__ uint32_t cross_off(uint32_t p, uint64_t low, uint32_t
   }
   if (COUNT && j0 < n)
     hits += (n - j0 + kBlock / 8 - 1) / (kBlock / 8);
-  return removed;
 }
 
 } // namespace
 
+/// Read-only tables and written arrays come in as separate __restrict__
+/// pointers so the compiler knows the tables are never written by the kernel
+/// and can read uniform entries (BInfo, work items) through the scalar cache.
 template <typename FT>
-__global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(HslArgs a)
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void hsl_sieve_kernel(
+    const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
+    const uint32_t* __restrict__ order, const BInfo* __restrict__ binfo_all,
+    const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
+    const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,
+    uint32_t* __restrict__ T_C, int32_t* __restrict__ T_M, long long* __restrict__ S_out,
+    unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,
+    uint32_t nitems, uint32_t W)
 {
   extern __shared__ __align__(16) uint32_t lds[];
-  const uint32_t W = a.W;
-  uint32_t* bm = lds;                                        // 2W u32
-  uint32_t* cnt1 = bm + 2 * W;                               // W/16 u32
-  uint32_t* pre2 = cnt1 + W / 16;                            // 64 u32
-  uint64_t* lemask = reinterpret_cast<uint64_t*>(pre2 + 64); // 240 u64
-  uint16_t* cop = reinterpret_cast<uint16_t*>(lemask + 240); // 480 u16
+  const uint32_t NS = W >> 6;                                 // super-groups
+  uint32_t* bm = lds;                                         // 2W u32
+  uint32_t* cnt1 = bm + 2 * W;                                // W/16 u32
+  uint16_t* cop = reinterpret_cast<uint16_t*>(cnt1 + W / 16); // 480 u16
 
-  __shared__ int sh_acc[4];             // [parity] leaf sign sum, [2+parity] removals
-  __shared__ uint32_t sh_chunk;
+  __shared__ uint32_t w_lo[kWin], w_hi[kWin];   // leaf index ranges of the window
+  __shared__ uint32_t pre_w[kWaves][64];        // per-wave super-group prefixes
+  __shared__ uint32_t sh_brk[kWaves];
+  __shared__ int sh_m[2];
+  __shared__ uint32_t sh_item;
   __shared__ long long sh_red[kWaves];
 
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63;
   const uint32_t wave = tid >> 6;
 
-  for (uint32_t r = tid; r < 240; r += kBlock) {
-    uint64_t m = 0;
-    for (int j = 0; j < 8; j++)
-      for (int i = 0; i < 8; i++)
-        if ((uint32_t) (30 * j + c_wheel[i]) <= r)
-          m |= 1ull << (8 * j + i);
-    lemask[r] = m;
-  }
   for (uint32_t i = tid; i < 480; i += kBlock)
     cop[i] = c_cop[i];
-  if (tid < 4)
-    sh_acc[tid] = 0;
+  if (tid < 2)
+    sh_m[tid] = 0;
   __syncthreads();
 
   const uint64_t SEG = 240ull * W;
-  const FT* __restrict__ factor = reinterpret_cast<const FT*>(a.factor);
-  unsigned long long n_leaves = 0, n_hits = 0, n_entries = 0, n_pairs = 0;
+  unsigned long long n_leaves = 0, n_hits = 0, n_entries = 0, n_slots = 0;
 
   for (;;) {
     if (tid == 0)
-      sh_chunk = atomicAdd(a.queue, 1u);
+      sh_item = atomicAdd(queue, 1u);
     __syncthreads();
-    const uint32_t qi = sh_chunk;
+    // values read from LDS are uniform; readfirstlane lets the compiler keep
+    // them (and everything derived from them) in scalar registers
+    const uint32_t qi = __builtin_amdgcn_readfirstlane(sh_item);
     __syncthreads();
-    if (qi >= a.nchunks)
+    if (qi >= nitems)
       break;
-    const uint32_t ci = a.order[qi];
+    const uint32_t item = order[qi];
+    const SegItem it = items[item];
+    const QueryDesc qd = queries[it.query];
+    const BInfo* binfo = binfo_all + qd.binfo_off;
+    uint32_t* TC = T_C + it.t_off;
+    int* TM = T_M + it.t_off;
+    const uint32_t b_stop = qd.b_first + it.nbs - 1;
+    const uint64_t low = (uint64_t) it.seg * SEG;
+    const uint64_t high = min(low + SEG, qd.limit);
+    const uint32_t seg_len = (uint32_t) (high - low);
+    const uint64_t low1 = low > 0 ? low : 1;
+    long long S = 0;
 
-    const ChunkDesc ch = a.chunks[ci];
-    const QueryDesc qd = a.queries[ch.query];
-    const BInfo* __restrict__ binfo = a.binfo + qd.binfo_off;
-    uint32_t* TC = a.T_C + ch.tc_off;
-    int* TM = a.T_M + ch.tc_off;
-    const uint32_t b_stop = min(qd.b_last, qd.b_first + ch.nbc - 1);
-    long long S = 0;         // this thread: sum of sign * in-segment count
-    __int128 Sacc = 0;       // thread 0: sum of M(s, b) * phi_local(b)
-
-    for (uint32_t s = ch.seg_begin; s < ch.seg_end; s++) {
-      const uint64_t low = (uint64_t) s * SEG;
-      const uint64_t high = min(low + SEG, qd.limit);
-      const uint32_t seg_len = (uint32_t) (high - low);
-      const uint64_t low1 = low > 0 ? low : 1;
-
-      // numbers coprime to 30 in [low, high)
-      for (uint32_t w = tid; w < W; w += kBlock) {
-        uint32_t base = w * 240;
-        uint64_t v;
-        if (base + 240 <= seg_len)
-          v = ~0ull;
-        else if (base >= seg_len)
-          v = 0;
-        else
-          v = lemask[seg_len - base - 1];
-        bm[2 * w] = (uint32_t) v;
-        bm[2 * w + 1] = (uint32_t) (v >> 32);
-      }
-      __syncthreads();
-
-      // pre-sieve p_4 .. p_{b_first - 1} (2, 3, 5 are outside the wheel)
-      for (uint32_t j = 4; j <= qd.pre_last; j++)
-        cross_off<false>(a.primes[j], low, seg_len, bm, cnt1, n_hits);
-      __syncthreads();
-
-      // group counters and the segment total
-      uint32_t my_total = 0;
-      {
-        const uint2* words = reinterpret_cast<const uint2*>(bm);
-        for (uint32_t gp = tid; gp < W / 16; gp += kBlock) {
-          uint32_t c0 = 0, c1 = 0;
+    // numbers coprime to 30 in [low, high)
+    for (uint32_t w = tid; w < W; w += kBlock) {
+      uint32_t base = w * 240;
+      uint64_t v = base + 240 <= seg_len ? ~0ull : (base >= seg_len ? 0ull : le_mask(seg_len - base - 1));
+      bm[2 * w] = (uint32_t) v;
+      bm[2 * w + 1] = (uint32_t) (v >> 32);
+    }
+    lds_barrier();
+    // pre-sieve p_4 .. p_{b_first - 1} (2, 3, 5 are outside the wheel)
+    for (uint32_t j = 4; j <= qd.pre_last; j++)
+      cross_off<false>(primes[j], low, seg_len, bm, cnt1, n_hits);
+    lds_barrier();
+    {
+      const uint4* w4 = reinterpret_cast<const uint4*>(bm);
+      for (uint32_t gp = tid; gp < W / 16; gp += kBlock) {
+        uint32_t c0 = 0, c1 = 0;
 #pragma unroll
-          for (int u = 0; u < 8; u++) {
-            uint2 x0 = words[16 * gp + u];
-            uint2 x1 = words[16 * gp + 8 + u];
-            c0 += __popc(x0.x) + __popc(x0.y);
-            c1 += __popc(x1.x) + __popc(x1.y);
-          }
-          cnt1[gp] = c0 | (c1 << 16);
-          my_total += c0 + c1;
+        for (int u = 0; u < 4; u++) {
+          uint4 x0 = w4[8 * gp + u];
+          uint4 x1 = w4[8 * gp + 4 + u];
+          c0 += __popc(x0.x) + __popc(x0.y) + __popc(x0.z) + __popc(x0.w);
+          c1 += __popc(x1.x) + __popc(x1.y) + __popc(x1.z) + __popc(x1.w);
         }
+        cnt1[gp] = c0 | (c1 << 16);
       }
-      long long tsum = wave_sum_i64((long long) my_total);
-      if (lane == 0)
-        sh_red[wave] = tsum;
-      __syncthreads();
-      uint32_t total = 0;      // meaningful in thread 0 only
-      if (tid == 0)
-        for (int i = 0; i < kWaves; i++)
-          total += (uint32_t) sh_red[i];
+    }
+    lds_barrier();
 
-      for (uint32_t b = qd.b_first; b <= b_stop; b++) {
-        const BInfo bi = binfo[b - qd.b_first];
-        const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
-        uint64_t lo_i, hi_i;
-        if (bi.type == 1) {
-          if (bi.p >= m_hi)
-            break;
+    uint32_t b = qd.b_first;
+    bool done = false;
+    while (!done) {
+      // leaf index ranges of b .. b + 255, and the first b that stops the segment
+      {
+        const uint32_t bb = b + tid;
+        uint32_t lo = 0, hi = 0;
+        bool brk = true;
+        if (bb <= b_stop) {
+          const BInfo bi = binfo[bb - qd.b_first];
+          const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
           uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
           m_lo = m_lo > bi.B ? m_lo : bi.B;
-          hi_i = ft_to_index(m_hi);
-          lo_i = m_lo < m_hi ? ft_to_index(m_lo) : hi_i;
-        } else {
-          hi_i = pi_lookup(a.pi_bits, a.pi_cnt, lemask, m_hi);
-          if (hi_i <= b)
-            break;
-          uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
-          m_lo = m_lo > bi.B ? m_lo : bi.B;
-          lo_i = m_lo < m_hi ? pi_lookup(a.pi_bits, a.pi_cnt, lemask, m_lo) : hi_i;
-        }
-        const uint32_t bidx = b - qd.b_first;
-        const int par = b & 1;
-        uint32_t phiL = 0;
-        int Mold = 0;
-        if (tid == 0) {
-          phiL = TC[bidx];
-          Mold = TM[bidx];
-        }
-
-        if (hi_i > lo_i) {
-          // super-group prefix over the group counters
-          if (wave == 0) {
-            uint32_t v = 0;
-            if (lane < W / 64) {
-              uint4 q = reinterpret_cast<const uint4*>(cnt1)[lane];
-              v = (q.x & 0xffff) + (q.x >> 16) + (q.y & 0xffff) + (q.y >> 16) +
-                  (q.z & 0xffff) + (q.z >> 16) + (q.w & 0xffff) + (q.w >> 16);
-            }
-            uint32_t incl = v;
-            for (int off = 1; off < 64; off <<= 1) {
-              uint32_t t = __shfl_up(incl, off, 64);
-              if ((int) lane >= off)
-                incl += t;
-            }
-            if (lane < W / 64)
-              pre2[lane] = incl - v;
-          }
-          __syncthreads();
-
-          int M = 0;
           if (bi.type == 1) {
-            n_entries += (hi_i - lo_i + kBlock - 1 - tid) / kBlock;
-            for (uint64_t idx = lo_i + 1 + tid; idx <= hi_i; idx += kBlock) {
-              uint32_t f = factor[idx];
-              if (f > bi.T) {
-                uint64_t qq = idx / 480;
-                uint64_t m = 2310 * qq + cop[idx - qq * 480];
-                uint64_t n = div_exact(bi.xp_d, bi.xp_lo, m);
-                uint32_t cnt = seg_count(bm, cnt1, pre2, lemask, (uint32_t) (n - low));
-                int sg = (f & 1) ? 1 : -1;
-                S += sg * (long long) cnt;
-                M += sg;
-                n_leaves++;
-              }
+            brk = bi.p >= m_hi;
+            if (!brk) {
+              hi = (uint32_t) ft_to_index(m_hi);
+              lo = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : hi;
             }
           } else {
-            for (uint64_t j = lo_i + 1 + tid; j <= hi_i; j += kBlock) {
-              uint64_t q = a.primes[j];
-              uint64_t n = div_exact(bi.xp_d, bi.xp_lo, q);
-              S += seg_count(bm, cnt1, pre2, lemask, (uint32_t) (n - low));
-              M += 1;
-              n_leaves++;
-            }
+            hi = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
+            brk = hi <= bb;
+            if (!brk)
+              lo = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : hi;
           }
-          M = wave_sum_i32(M);
-          if (lane == 0 && M != 0)
-            atomicAdd(&sh_acc[par], M);
-          __syncthreads();
         }
+        w_lo[tid] = lo;
+        w_hi[tid] = hi;
+        unsigned long long bal = __ballot(brk);
+        if (lane == 0)
+          sh_brk[wave] = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
+      }
+      lds_barrier();
+      uint32_t nwin = kWin;
+      for (int i = kWaves - 1; i >= 0; i--)
+        if (sh_brk[i] < 64)
+          nwin = 64 * i + sh_brk[i];
+      nwin = __builtin_amdgcn_readfirstlane(nwin);
+      if (nwin < kWin)
+        done = true;
 
-        uint32_t removed = cross_off<true>(bi.p, low, seg_len, bm, cnt1, n_hits);
-        int rem = wave_sum_i32((int) removed);
-        if (lane == 0 && rem != 0)
-          atomicAdd(&sh_acc[2 + par], rem);
-        __syncthreads();
-        if (tid == 0) {
-          int M = sh_acc[par];
-          int R = sh_acc[2 + par];
-          sh_acc[par] = 0;
-          sh_acc[2 + par] = 0;
-          Sacc += (__int128) M * (__int128) phiL;
-          TC[bidx] = phiL + total;
-          TM[bidx] = Mold + M;
-          total -= (uint32_t) R;
-          n_pairs++;
+      // factor entries (or primes) of the first prime, prefetched
+      uint32_t pf[kPf];
+      {
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(w_lo[0]);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(w_hi[0]);
+        const bool t1 = nwin > 0 && binfo[b - qd.b_first].type == 1;
+#pragma unroll
+        for (int k = 0; k < kPf; k++) {
+          uint32_t idx = lo + 1 + tid + k * kBlock;
+          pf[k] = (nwin > 0 && idx <= hi) ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
         }
       }
-      __syncthreads();
-    }
 
+      for (uint32_t j = 0; j < nwin; j++) {
+        const uint32_t bj = b + j;
+        const uint32_t br = bj - qd.b_first;
+        const BInfo bi = binfo[br];
+
+        // super-group prefixes and C(s, b)
+        uint32_t v = 0;
+        if (lane < NS)
+          v = sum8(reinterpret_cast<const uint4*>(cnt1)[lane]);
+        const uint32_t incl = wave_incl_scan(v);
+        if (lane < NS)
+          pre_w[wave][lane] = incl - v;
+        const uint32_t total = __shfl(incl, 63, 64);
+        if (tid == 0)
+          TC[br] = total;
+
+        const uint32_t lo = __builtin_amdgcn_readfirstlane(w_lo[j]);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane(w_hi[j]);
+        int M = 0;
+        if (hi > lo) {
+          const uint32_t* pre = pre_w[wave];
+          const bool t1 = bi.type == 1;
+          uint32_t k = 0;
+#pragma nounroll
+          for (uint32_t idx = lo + 1 + tid; idx <= hi; idx += kBlock, k++) {
+            // entry: prefetched for the first kPf strides, loaded otherwise
+            uint32_t f = k == 0 ? pf[0] : (k == 1 ? pf[1] : (t1 ? (uint32_t) factor[idx] : primes[idx]));
+            uint64_t m;
+            int sg;
+            if (t1) {
+              n_entries++;
+              if (f <= bi.T)
+                continue;
+              uint32_t qq = idx / 480;
+              m = 2310ull * qq + cop[idx - qq * 480];
+              sg = (f & 1) ? 1 : -1;
+            } else {
+              m = f;
+              sg = 1;
+            }
+            uint64_t n = div_exact(bi.xp_d, bi.xp_lo, m);
+            uint32_t cnt = seg_count(bm, cnt1, pre, (uint32_t) (n - low));
+            S += sg * (long long) cnt;
+            M += sg;
+            n_leaves++;
+          }
+        }
+
+        // prefetch the next prime's entries; they arrive during the cross-off
+        if (j + 1 < nwin) {
+          const uint32_t lo2 = __builtin_amdgcn_readfirstlane(w_lo[j + 1]);
+          const uint32_t hi2 = __builtin_amdgcn_readfirstlane(w_hi[j + 1]);
+          const bool t1 = binfo[br + 1].type == 1;
+#pragma unroll
+          for (int k = 0; k < kPf; k++) {
+            uint32_t idx = lo2 + 1 + tid + k * kBlock;
+            pf[k] = idx <= hi2 ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+          }
+        }
+
+        M = wave_sum_i32(M);
+        if (lane == 0 && M != 0)
+          atomicAdd(&sh_m[j & 1], M);
+        lds_barrier();
+        if (tid == 0) {
+          TM[br] = sh_m[j & 1];
+          sh_m[j & 1] = 0;
+          n_slots++;
+        }
+        cross_off<true>(bi.p, low, seg_len, bm, cnt1, n_hits);
+        lds_barrier();
+      }
+      b += nwin;
+    }
+    // slots the segment never reached hold zero
+    for (uint32_t bb = b + tid; bb <= b_stop; bb += kBlock) {
+      TC[bb - qd.b_first] = 0;
+      TM[bb - qd.b_first] = 0;
+    }
     long long ws = wave_sum_i64(S);
     if (lane == 0)
       sh_red[wave] = ws;
-    __syncthreads();
+    lds_barrier();
     if (tid == 0) {
-      __int128 tot = Sacc;
+      long long tot = 0;
       for (int i = 0; i < kWaves; i++)
         tot += sh_red[i];
-      a.S_out[2 * (size_t) ci] = (unsigned long long) tot;
-      a.S_out[2 * (size_t) ci + 1] = (unsigned long long) (tot >> 64);
+      S_out[item] = tot;
     }
-    __syncthreads();
   }
 
   long long l = wave_sum_i64((long long) n_leaves);
   long long h = wave_sum_i64((long long) n_hits);
   long long e = wave_sum_i64((long long) n_entries);
   if (lane == 0) {
-    atomicAdd(&a.counters[0], (unsigned long long) l);
-    atomicAdd(&a.counters[1], (unsigned long long) h);
-    atomicAdd(&a.counters[2], (unsigned long long) e);
+    atomicAdd(&counters[0], (unsigned long long) l);
+    atomicAdd(&counters[1], (unsigned long long) h);
+    atomicAdd(&counters[2], (unsigned long long) e);
   }
   if (tid == 0)
-    atomicAdd(&a.counters[3], n_pairs);
+    atomicAdd(&counters[3], n_slots);
 }
 
-template __global__ void hsl_sieve_kernel<uint16_t>(HslArgs);
-template __global__ void hsl_sieve_kernel<uint32_t>(HslArgs);
+template <typename FT>
+static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st)
+{
+  hipLaunchKernelGGL(hsl_sieve_kernel<FT>, dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
+                     a.queries, a.items, a.order, a.binfo, a.primes,
+                     reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_C, a.T_M,
+                     a.S_out, a.counters, a.queue, a.nitems, a.W);
+  return hipGetLastError();
+}
 
 /// Zero-fill (used instead of hipMemsetAsync so the sequence can be captured).
 __global__ void hsl_fill_zero(uint4* p, size_t n16)
@@ -424,54 +512,57 @@ __global__ void hsl_fill_zero(uint4* p, size_t n16)
     p[i] = make_uint4(0, 0, 0, 0);
 }
 
-/// Column = one (query, b).  Slice = a range of that query's chunks.
-/// For each column and slice: the slice's count total, sign total and the
-/// sum of M(chunk, b) * (counts of the earlier chunks inside the slice).
+/// Column = one (query, b); slice = a range of the column's segments.  For
+/// each column and slice: the slice's count total, sign total and the sum of
+/// M(s, b) * (counts of the earlier segments inside the slice).
 __global__ void hsl_combine_slices(const CombineCol* __restrict__ cols, uint32_t ncols,
-                                   const ChunkDesc* __restrict__ chunks,
+                                   const SegItem* __restrict__ items,
                                    const uint32_t* __restrict__ T_C, const int* __restrict__ T_M,
                                    uint32_t slice_len, uint32_t nslices,
                                    long long* sC, long long* sM, __int128* sMP)
 {
-  uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t sl = blockIdx.y;
-  if (col >= ncols)
+  if (c >= ncols)
     return;
-  const CombineCol cc = cols[col];
-  uint32_t c0 = sl * slice_len;
-  uint32_t c1 = min(cc.nch, c0 + slice_len);
+  const CombineCol cc = cols[c];
+  uint32_t i0 = sl * slice_len;
+  uint32_t i1 = min(cc.nitems, i0 + slice_len);
   long long P = 0, Ms = 0;
   __int128 MP = 0;
-  for (uint32_t c = c0; c < c1; c++) {
-    const ChunkDesc ch = chunks[cc.chunk_first + c];
-    uint64_t off = ch.tc_off + cc.b_rel;
+  for (uint32_t i = i0; i < i1; i++) {
+    uint64_t off = items[cc.item_first + i].t_off + cc.b_rel;
     long long tc = T_C[off];
     long long tm = T_M[off];
     MP += (__int128) tm * P;
     P += tc;
     Ms += tm;
   }
-  size_t o = (size_t) sl * ncols + col;
+  size_t o = (size_t) sl * ncols + c;
   sC[o] = P;
   sM[o] = Ms;
   sMP[o] = MP;
 }
 
-__global__ void hsl_combine_final(uint32_t ncols, uint32_t nslices,
-                                  const long long* sC, const long long* sM, const __int128* sMP,
-                                  __int128* out)
+/// Adds the column's carry (counts of all earlier segments, including those
+/// of earlier passes) and accumulates into out[col].
+__global__ void hsl_combine_final(const CombineCol* __restrict__ cols, uint32_t ncols,
+                                  uint32_t nslices, const long long* sC, const long long* sM,
+                                  const __int128* sMP, long long* carry, __int128* out)
 {
-  uint32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncols)
+  uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncols)
     return;
-  long long carry = 0;
+  const uint32_t g = cols[c].col;
+  long long cr = carry[g];
   __int128 acc = 0;
   for (uint32_t sl = 0; sl < nslices; sl++) {
-    size_t o = (size_t) sl * ncols + col;
-    acc += sMP[o] + (__int128) carry * sM[o];
-    carry += sC[o];
+    size_t o = (size_t) sl * ncols + c;
+    acc += sMP[o] + (__int128) cr * sM[o];
+    cr += sC[o];
   }
-  out[col] = acc;
+  carry[g] = cr;
+  out[g] += acc;
 }
 
 // ------------------------------------------------------------------------
@@ -502,12 +593,7 @@ hipError_t hsl_upload_constants()
 
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st)
 {
-  size_t lds = hsl_lds_bytes(a.W);
-  if (factor32)
-    hipLaunchKernelGGL(hsl_sieve_kernel<uint32_t>, dim3(grid), dim3(kBlock), lds, st, a);
-  else
-    hipLaunchKernelGGL(hsl_sieve_kernel<uint16_t>, dim3(grid), dim3(kBlock), lds, st, a);
-  return hipGetLastError();
+  return factor32 ? launch_sieve_t<uint32_t>(a, grid, st) : launch_sieve_t<uint16_t>(a, grid, st);
 }
 
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu)
@@ -527,19 +613,19 @@ hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const ChunkDesc* chunks,
+hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const SegItem* items,
                               const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
                               uint32_t nslices, long long* sC, long long* sM, void* sMP,
-                              void* out, hipStream_t st)
+                              long long* carry, void* out, hipStream_t st)
 {
   if (ncols == 0) return hipSuccess;
   dim3 g1((ncols + 255) / 256, nslices);
-  hipLaunchKernelGGL(hsl_combine_slices, g1, dim3(256), 0, st, cols, ncols, chunks, T_C, T_M,
+  hipLaunchKernelGGL(hsl_combine_slices, g1, dim3(256), 0, st, cols, ncols, items, T_C, T_M,
                      slice_len, nslices, sC, sM, (__int128*) sMP);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(hsl_combine_final, dim3((ncols + 255) / 256), dim3(256), 0, st, ncols,
-                     nslices, sC, sM, (const __int128*) sMP, (__int128*) out);
+  hipLaunchKernelGGL(hsl_combine_final, dim3((ncols + 255) / 256), dim3(256), 0, st, cols, ncols,
+                     nslices, sC, sM, (const __int128*) sMP, carry, (__int128*) out);
   return hipGetLastError();
 }
 

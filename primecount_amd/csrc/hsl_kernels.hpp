@@ -10,8 +10,8 @@
 //   leaf      x / (p_b * m) with a square-free m (type 1, factor-table loop)
 //             or a prime m (type 2, prime loop); it contributes
 //             sign * phi(x / (p_b * m), b - 1)
-//   chunk     a run of consecutive segments of one query, the unit a
-//             workgroup takes from the device work queue
+//   slot      one (segment, b) pair: C = unsieved numbers of the segment
+//             before p_b is crossed off, M = sum of the leaf signs
 #pragma once
 
 #include <stdint.h>
@@ -48,42 +48,47 @@ struct QueryDesc {
   uint32_t pre_last;     // primes p_4..p_pre_last are pre-sieved (pre_last = b_first-1)
 };
 
-/// One work-queue item: segments [seg_begin, seg_end) of query `query`.
-/// Its per-b partial sums live at tc_off + (b - b_first), b - b_first < nbc.
-struct ChunkDesc {
+/// One work-queue item: segment `seg` of query `query`.  Its slots
+/// (seg, b) live at t_off + (b - b_first) for b - b_first < nbs; nbs bounds
+/// the b the segment can reach (p_b <= sqrt(x / low)).
+struct alignas(16) SegItem {
   uint32_t query;
-  uint32_t seg_begin;
-  uint32_t seg_end;
-  uint32_t nbc;
-  uint64_t tc_off;
+  uint32_t seg;
+  uint32_t nbs;
+  uint32_t pad;
+  uint64_t t_off;
+  uint64_t pad1;
 };
 
 struct HslArgs {
   const QueryDesc* queries;
-  const ChunkDesc* chunks;
-  const uint32_t* order;        // work-queue position -> chunk index
+  const SegItem* items;
+  const uint32_t* order;        // work-queue position -> item index
   const BInfo* binfo;
   const uint32_t* primes;       // primes[0] = 0, primes[1] = 2, ...
   const void* factor;           // uint16_t or uint32_t factor table
   const uint64_t* pi_bits;      // prime bits per 240 numbers (primes >= 7)
   const uint32_t* pi_cnt;       // primes >= 7 below 240 * i
-  uint32_t* T_C;                // per (chunk, b): unsieved count summed over the chunk
-  int32_t* T_M;                 // per (chunk, b): sum of leaf signs over the chunk
-  unsigned long long* S_out;    // per chunk: 128-bit partial sum (lo, hi)
-  unsigned long long* counters; // leaves, hits, factor entries, pairs
-  unsigned int* queue;          // next chunk
-  uint32_t nchunks;
+  uint32_t* T_C;                // per slot: unsieved count before p_b is crossed off
+  int32_t* T_M;                 // per slot: sum of leaf signs
+  long long* S_out;             // per workgroup: sum of sign * in-segment count
+  unsigned long long* counters; // leaves, hits, factor entries, slots
+  unsigned int* queue;          // next work item
+  uint32_t nitems;
   uint32_t W;                   // words per segment (power of two, 256..4096)
 };
 
 constexpr int kBlock = 256;
 
-/// Column of the carry combine: one (query, b).
+/// Column of the carry combine: one (query, b) within one pass.
+/// Its items are items[item_first .. item_first + nitems) (consecutive
+/// segments of the query, in order); `col` indexes the pass-independent
+/// carry and output arrays.
 struct CombineCol {
   uint32_t b_rel;         // b - b_first
-  uint32_t chunk_first;   // first chunk (index into the chunk array) of the query
-  uint32_t nch;           // leading chunks of the query that store this b
-  uint32_t query;
+  uint32_t item_first;
+  uint32_t nitems;
+  uint32_t col;
 };
 
 /// LDS bytes the sieve kernel needs for W words per segment.
@@ -91,8 +96,6 @@ inline uint32_t hsl_lds_bytes(uint32_t W)
 {
   return 8 * W            // segment bitmap
        + W / 4            // group counters (16-bit, 8 words per group)
-       + 64 * 4           // super-group prefix (64 words per super-group)
-       + 240 * 8          // "offsets <= r" masks
        + 480 * 2;         // numbers coprime to 2310 in [0, 2310)
 }
 
@@ -101,9 +104,9 @@ hipError_t hsl_upload_constants();
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
-hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const ChunkDesc* chunks,
+hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const SegItem* items,
                               const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
                               uint32_t nslices, long long* sC, long long* sM, void* sMP,
-                              void* out, hipStream_t st);
+                              long long* carry, void* out, hipStream_t st);
 
 } // namespace pcg
