@@ -36,14 +36,11 @@ namespace pcg {
 
 __constant__ int16_t c_cop_index[2310];   // largest index of a coprime <= r
 __constant__ uint16_t c_cop[480];         // coprimes to 2310 in [0, 2310)
-__constant__ uint8_t c_wheel[8];          // 1, 7, 11, ..., 29
-__constant__ uint8_t c_wheel_bit[30];     // residue -> bit in byte (0xff: none)
 
 namespace {
 
 constexpr int kWaves = kBlock / 64;
 constexpr int kPf = 2;              // factor entries per thread prefetched one prime ahead
-constexpr uint32_t kWin = kBlock;   // primes per leaf-range window
 
 /// Nibble table: number of wheel residues {1,7,11,...,29} that are <= r.
 constexpr uint64_t le_nibbles(int first)
@@ -130,6 +127,20 @@ __device__ __forceinline__ uint64_t pi_lookup(const uint64_t* __restrict__ bits,
   return 3 + (uint64_t) cnt[blk] + (uint64_t) __popcll(bits[blk] & le_mask(r));
 }
 
+/// 64-bit v_readlane (the builtin returns int; the low word must not be
+/// sign-extended when the halves are joined).
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t lane)
+{
+  uint32_t lo = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) v, (int) lane);
+  uint32_t hi = (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) (v >> 32), (int) lane);
+  return ((uint64_t) hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane)
+{
+  return (uint32_t) __builtin_amdgcn_readlane((int) v, (int) lane);
+}
+
 __device__ __forceinline__ int wave_sum_i32(int v)
 {
   for (int off = 32; off > 0; off >>= 1)
@@ -199,26 +210,32 @@ __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t
   return c;
 }
 
-/// Cross off p and its multiples in the segment (numbers p*q > low with q
-/// coprime to 30, as Sieve::add/cross_off, src/sieve/Sieve.cpp:192-216).
-/// Threads split the 8 residue classes of q mod 30 and stride inside a class:
-/// consecutive multiples of one class are p bytes apart.  With COUNT the
-/// first-time removals decrement the group counters.
-template <bool COUNT>
-__device__ __forceinline__ void cross_off(uint32_t p, uint64_t low, uint32_t seg_len,
-                                          uint32_t* bm, uint32_t* cnt1,
-                                          unsigned long long& hits)
+/// Residues coprime to 30 in bit order, packed as bytes: 1, 7, 11, ..., 29.
+constexpr uint64_t kWheelBytes = 0x1D1713110D0B0701ull;
+
+/// Bit (0..7) of a wheel residue r in a sieve byte.
+__device__ __forceinline__ uint32_t wheel_bit(uint32_t r)
 {
-  if (p < 7)
+  uint32_t k = (uint32_t) ((r < 16 ? (kLeLo >> (4 * r)) : (kLeHi >> (4 * (r - 16)))) & 15);
+  return k - 1;
+}
+
+/// Cross off p and its multiples p*q, q in [q_lo, q_hi] coprime to 30, in the
+/// segment starting at low (Sieve::add/cross_off remove the multiples
+/// > low, src/sieve/Sieve.cpp:192-216).  Threads split the 8 residue classes
+/// of q mod 30 and stride inside a class: consecutive multiples of one class
+/// are p bytes apart.  With COUNT the first-time removals decrement the group
+/// counters.
+template <bool COUNT>
+__device__ __forceinline__ void cross_off_q(uint32_t p, uint64_t q_lo, uint64_t q_hi, uint64_t low,
+                                            uint32_t* bm, uint32_t* cnt1,
+                                            unsigned long long& hits)
+{
+  if (p < 7 || q_hi < q_lo)
     return;
   const uint32_t cls = threadIdx.x & 7;
   const uint32_t j0 = threadIdx.x >> 3;
-  uint64_t q_lo = div_exact((double) low, low, p) + 1;
-  uint64_t last = low + seg_len - 1;
-  uint64_t q_hi = div_exact((double) last, last, p);
-  if (q_hi < q_lo)
-    return;
-  uint64_t q = q_lo - q_lo % 30 + c_wheel[cls];
+  uint64_t q = q_lo - q_lo % 30 + ((kWheelBytes >> (8 * cls)) & 0xff);
   if (q < q_lo)
     q += 30;
   if (q > q_hi)
@@ -226,7 +243,7 @@ __device__ __forceinline__ void cross_off(uint32_t p, uint64_t low, uint32_t seg
   uint32_t n = (uint32_t) ((q_hi - q) / 30) + 1;
   uint32_t off = (uint32_t) ((uint64_t) p * q - low);
   uint32_t byte0 = off / 30;
-  uint32_t bib = c_wheel_bit[off - byte0 * 30];
+  uint32_t bib = wheel_bit(off - byte0 * 30);
   for (uint32_t t = j0; t < n; t += kBlock / 8) {
     uint32_t B = byte0 + t * p;
     uint32_t mask = 1u << (((B & 3) << 3) | bib);
@@ -242,6 +259,17 @@ __device__ __forceinline__ void cross_off(uint32_t p, uint64_t low, uint32_t seg
   }
   if (COUNT && j0 < n)
     hits += (n - j0 + kBlock / 8 - 1) / (kBlock / 8);
+}
+
+/// Pre-sieve form: q range computed here.
+__device__ __forceinline__ void cross_off_presieve(uint32_t p, uint64_t low, uint32_t seg_len,
+                                                   uint32_t* bm)
+{
+  unsigned long long dummy = 0;
+  uint64_t q_lo = div_exact((double) low, low, p) + 1;
+  uint64_t last = low + seg_len - 1;
+  uint64_t q_hi = div_exact((double) last, last, p);
+  cross_off_q<false>(p, q_lo, q_hi, low, bm, nullptr, dummy);
 }
 
 } // namespace
@@ -265,9 +293,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   uint32_t* cnt1 = bm + 2 * W;                                // W/16 u32
   uint16_t* cop = reinterpret_cast<uint16_t*>(cnt1 + W / 16); // 480 u16
 
-  __shared__ uint32_t w_lo[kWin], w_hi[kWin];   // leaf index ranges of the window
   __shared__ uint32_t pre_w[kWaves][64];        // per-wave super-group prefixes
-  __shared__ uint32_t sh_brk[kWaves];
   __shared__ int sh_m[2];
   __shared__ uint32_t sh_item;
   __shared__ long long sh_red[kWaves];
@@ -318,7 +344,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     lds_barrier();
     // pre-sieve p_4 .. p_{b_first - 1} (2, 3, 5 are outside the wheel)
     for (uint32_t j = 4; j <= qd.pre_last; j++)
-      cross_off<false>(primes[j], low, seg_len, bm, cnt1, n_hits);
+      cross_off_presieve(primes[j], low, seg_len, bm);
     lds_barrier();
     {
       const uint4* w4 = reinterpret_cast<const uint4*>(bm);
@@ -339,50 +365,53 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     uint32_t b = qd.b_first;
     bool done = false;
     while (!done) {
-      // leaf index ranges of b .. b + 255, and the first b that stops the segment
-      {
-        const uint32_t bb = b + tid;
-        uint32_t lo = 0, hi = 0;
-        bool brk = true;
-        if (bb <= b_stop) {
-          const BInfo bi = binfo[bb - qd.b_first];
-          const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
-          uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
-          m_lo = m_lo > bi.B ? m_lo : bi.B;
-          if (bi.type == 1) {
-            brk = bi.p >= m_hi;
-            if (!brk) {
-              hi = (uint32_t) ft_to_index(m_hi);
-              lo = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : hi;
-            }
-          } else {
-            hi = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
-            brk = hi <= bb;
-            if (!brk)
-              lo = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : hi;
+      // Window of up to 64 primes b .. b + 63.  Every wave computes the whole
+      // window (lane l <-> b + l) and keeps it in registers; an epoch reads
+      // its prime's values with v_readlane, so the epoch loop touches no
+      // memory for per-prime data.  The window ends before the first b whose
+      // leaf range is provably empty (that b and all later ones stop the
+      // segment, src/gourdon/D.cpp:131-132, 289-290).
+      const uint32_t bb = b + lane;
+      uint32_t r_lo = 0, r_hi = 0, r_p = 0, r_T = 0;
+      uint64_t r_xlo = 0, r_qlo = 0, r_qhi = 0;
+      double r_xd = 0;
+      bool brk = true;
+      if (bb <= b_stop) {
+        const BInfo bi = binfo[bb - qd.b_first];
+        const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
+        uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
+        m_lo = m_lo > bi.B ? m_lo : bi.B;
+        if (bi.type == 1) {
+          brk = bi.p >= m_hi;
+          if (!brk) {
+            r_hi = (uint32_t) ft_to_index(m_hi);
+            r_lo = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : r_hi;
           }
+        } else {
+          r_hi = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
+          brk = r_hi <= bb;
+          if (!brk)
+            r_lo = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : r_hi;
         }
-        w_lo[tid] = lo;
-        w_hi[tid] = hi;
-        unsigned long long bal = __ballot(brk);
-        if (lane == 0)
-          sh_brk[wave] = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
+        r_p = bi.p;
+        r_T = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
+        r_xlo = bi.xp_lo;
+        r_xd = bi.xp_d;
+        // first and last cofactor of p in the segment: p*q in (low, high)
+        r_qlo = div_exact((double) low, low, bi.p) + 1;
+        r_qhi = div_exact((double) (high - 1), high - 1, bi.p);
       }
-      lds_barrier();
-      uint32_t nwin = kWin;
-      for (int i = kWaves - 1; i >= 0; i--)
-        if (sh_brk[i] < 64)
-          nwin = 64 * i + sh_brk[i];
-      nwin = __builtin_amdgcn_readfirstlane(nwin);
-      if (nwin < kWin)
+      const unsigned long long bal = __ballot(brk);
+      const uint32_t nwin = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
+      if (nwin < 64)
         done = true;
 
-      // factor entries (or primes) of the first prime, prefetched
+      // factor entries (or primes) of the window's first prime
       uint32_t pf[kPf];
       {
-        const uint32_t lo = __builtin_amdgcn_readfirstlane(w_lo[0]);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane(w_hi[0]);
-        const bool t1 = nwin > 0 && binfo[b - qd.b_first].type == 1;
+        const uint32_t lo = readlane_u32(r_lo, 0);
+        const uint32_t hi = readlane_u32(r_hi, 0);
+        const bool t1 = (readlane_u32(r_T, 0) >> 31) != 0;
 #pragma unroll
         for (int k = 0; k < kPf; k++) {
           uint32_t idx = lo + 1 + tid + k * kBlock;
@@ -391,9 +420,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
       }
 
       for (uint32_t j = 0; j < nwin; j++) {
-        const uint32_t bj = b + j;
-        const uint32_t br = bj - qd.b_first;
-        const BInfo bi = binfo[br];
+        const uint32_t br = b + j - qd.b_first;
+        const uint32_t lo = readlane_u32(r_lo, j);
+        const uint32_t hi = readlane_u32(r_hi, j);
+        const uint32_t Tt = readlane_u32(r_T, j);
+        const uint32_t p = readlane_u32(r_p, j);
+        const uint64_t xlo = readlane_u64(r_xlo, j);
+        const double xd = __longlong_as_double(
+            (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
+        const bool t1 = (Tt >> 31) != 0;
+        const uint32_t T = Tt & 0x7fffffffu;
 
         // super-group prefixes and C(s, b)
         uint32_t v = 0;
@@ -402,16 +438,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const uint32_t incl = wave_incl_scan(v);
         if (lane < NS)
           pre_w[wave][lane] = incl - v;
-        const uint32_t total = __shfl(incl, 63, 64);
+        const uint32_t total = readlane_u32(incl, 63);
         if (tid == 0)
           TC[br] = total;
 
-        const uint32_t lo = __builtin_amdgcn_readfirstlane(w_lo[j]);
-        const uint32_t hi = __builtin_amdgcn_readfirstlane(w_hi[j]);
         int M = 0;
         if (hi > lo) {
           const uint32_t* pre = pre_w[wave];
-          const bool t1 = bi.type == 1;
           uint32_t k = 0;
 #pragma nounroll
           for (uint32_t idx = lo + 1 + tid; idx <= hi; idx += kBlock, k++) {
@@ -421,7 +454,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             int sg;
             if (t1) {
               n_entries++;
-              if (f <= bi.T)
+              if (f <= T)
                 continue;
               uint32_t qq = idx / 480;
               m = 2310ull * qq + cop[idx - qq * 480];
@@ -430,7 +463,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
               m = f;
               sg = 1;
             }
-            uint64_t n = div_exact(bi.xp_d, bi.xp_lo, m);
+            uint64_t n = div_exact(xd, xlo, m);
             uint32_t cnt = seg_count(bm, cnt1, pre, (uint32_t) (n - low));
             S += sg * (long long) cnt;
             M += sg;
@@ -440,18 +473,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
 
         // prefetch the next prime's entries; they arrive during the cross-off
         if (j + 1 < nwin) {
-          const uint32_t lo2 = __builtin_amdgcn_readfirstlane(w_lo[j + 1]);
-          const uint32_t hi2 = __builtin_amdgcn_readfirstlane(w_hi[j + 1]);
-          const bool t1 = binfo[br + 1].type == 1;
+          const uint32_t lo2 = readlane_u32(r_lo, j + 1);
+          const uint32_t hi2 = readlane_u32(r_hi, j + 1);
+          const bool t1n = (readlane_u32(r_T, j + 1) >> 31) != 0;
 #pragma unroll
           for (int k = 0; k < kPf; k++) {
             uint32_t idx = lo2 + 1 + tid + k * kBlock;
-            pf[k] = idx <= hi2 ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+            pf[k] = idx <= hi2 ? (t1n ? (uint32_t) factor[idx] : primes[idx]) : 0u;
           }
         }
 
-        M = wave_sum_i32(M);
-        if (lane == 0 && M != 0)
+        if (M != 0)
           atomicAdd(&sh_m[j & 1], M);
         lds_barrier();
         if (tid == 0) {
@@ -459,7 +491,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
           sh_m[j & 1] = 0;
           n_slots++;
         }
-        cross_off<true>(bi.p, low, seg_len, bm, cnt1, n_hits);
+        const uint64_t qlo = readlane_u64(r_qlo, j);
+        const uint64_t qhi = readlane_u64(r_qhi, j);
+        cross_off_q<true>(p, qlo, qhi, low, bm, cnt1, n_hits);
         lds_barrier();
       }
       b += nwin;
@@ -580,15 +614,9 @@ hipError_t hsl_upload_constants()
     }
     ci[n] = (int16_t) idx;
   }
-  const uint8_t wheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
-  uint8_t wb[30];
-  for (int r = 0; r < 30; r++) wb[r] = 0xff;
-  for (int i = 0; i < 8; i++) wb[wheel[i]] = (uint8_t) i;
   hipError_t e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cop_index), ci, sizeof(ci))) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cop), cp, sizeof(cp))) != hipSuccess) return e;
-  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_wheel), wheel, sizeof(wheel))) != hipSuccess) return e;
-  return hipMemcpyToSymbol(HIP_SYMBOL(c_wheel_bit), wb, sizeof(wb));
+  return hipMemcpyToSymbol(HIP_SYMBOL(c_cop), cp, sizeof(cp));
 }
 
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st)
