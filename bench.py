@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--x", type=float, default=1e19)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-timeout", type=int, default=120)
+    ap.add_argument("--segment-words", type=int, default=0,
+                    help="64-bit words per sieve segment (0 = automatic)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the hard-special-leaf engine runs on a single MI355X "
@@ -100,6 +102,8 @@ def main():
         sys.exit(2)
 
     import primecount_amd as pc
+    if a.segment_words:
+        pc.set_segment_words(a.segment_words)
     x = int(round(a.x))
     p = pc.gourdon_params(x)
     want, _ = expected_D(x)

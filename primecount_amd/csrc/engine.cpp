@@ -270,11 +270,16 @@ void plan_segments(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint3
   }
 }
 
+/// Segment size.  One wave owns a segment, so LDS per wave (8 W bytes) sets
+/// how many waves a CU keeps resident; on gfx950 W = 1024 (245,760 numbers)
+/// measured fastest for large sieve limits (W = 512 and 2048 were 8% and 90%
+/// slower at x = 1e19).  Small limits use smaller segments so that every
+/// resident wave gets work.
 uint32_t auto_segment_words(uint64_t limit, uint64_t slots)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
-  uint32_t W = 2048;
+  uint32_t W = 1024;
   while (W > 256 && (limit + 240ull * W - 1) / (240ull * W) < 2 * slots)
     W >>= 1;
   return W;
@@ -400,7 +405,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     if (!plans[i].empty) max_limit = std::max(max_limit, plans[i].qd.limit);
   }
   int bpc = 0;
-  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 6);
+  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 16);
   if (W < 256 || W > 4096 || (W & (W - 1)))
     throw pcg_error("segment words must be a power of two in [256, 4096]");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));

@@ -39,7 +39,6 @@ __constant__ uint16_t c_cop[480];         // coprimes to 2310 in [0, 2310)
 
 namespace {
 
-constexpr int kWaves = kBlock / 64;
 constexpr int kPf = 2;              // factor entries per thread prefetched one prime ahead
 
 /// Nibble table: number of wheel residues {1,7,11,...,29} that are <= r.
@@ -274,11 +273,21 @@ __device__ __forceinline__ void cross_off_presieve(uint32_t p, uint64_t low, uin
 
 } // namespace
 
+/// Inclusive-scan total of a wave (every lane contributes v).
+__device__ __forceinline__ uint32_t wave_total_u32(uint32_t v)
+{
+  return readlane_u32(wave_incl_scan(v), 63);
+}
+
+/// One wave owns one segment at a time: the epochs need no workgroup
+/// barriers (a wave's LDS operations execute in order), and the per-prime
+/// bookkeeping runs once per segment and prime.
+///
 /// Read-only tables and written arrays come in as separate __restrict__
 /// pointers so the compiler knows the tables are never written by the kernel
 /// and can read uniform entries (BInfo, work items) through the scalar cache.
 template <typename FT>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void hsl_sieve_kernel(
+__global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
     const uint32_t* __restrict__ order, const BInfo* __restrict__ binfo_all,
     const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
@@ -292,33 +301,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* cnt1 = bm + 2 * W;                                // W/16 u32
   uint16_t* cop = reinterpret_cast<uint16_t*>(cnt1 + W / 16); // 480 u16
+  __shared__ uint32_t pre_s[64];                              // super-group prefixes
 
-  __shared__ uint32_t pre_w[kWaves][64];        // per-wave super-group prefixes
-  __shared__ int sh_m[2];
-  __shared__ uint32_t sh_item;
-  __shared__ long long sh_red[kWaves];
-
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63;
-  const uint32_t wave = tid >> 6;
-
-  for (uint32_t i = tid; i < 480; i += kBlock)
+  const uint32_t lane = threadIdx.x;
+  for (uint32_t i = lane; i < 480; i += kBlock)
     cop[i] = c_cop[i];
-  if (tid < 2)
-    sh_m[tid] = 0;
-  __syncthreads();
 
   const uint64_t SEG = 240ull * W;
   unsigned long long n_leaves = 0, n_hits = 0, n_entries = 0, n_slots = 0;
 
   for (;;) {
-    if (tid == 0)
-      sh_item = atomicAdd(queue, 1u);
-    __syncthreads();
-    // values read from LDS are uniform; readfirstlane lets the compiler keep
-    // them (and everything derived from them) in scalar registers
-    const uint32_t qi = __builtin_amdgcn_readfirstlane(sh_item);
-    __syncthreads();
+    uint32_t qi = 0;
+    if (lane == 0)
+      qi = atomicAdd(queue, 1u);
+    qi = __builtin_amdgcn_readfirstlane(qi);
     if (qi >= nitems)
       break;
     const uint32_t item = order[qi];
@@ -335,20 +331,20 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     long long S = 0;
 
     // numbers coprime to 30 in [low, high)
-    for (uint32_t w = tid; w < W; w += kBlock) {
+    for (uint32_t w = lane; w < W; w += kBlock) {
       uint32_t base = w * 240;
       uint64_t v = base + 240 <= seg_len ? ~0ull : (base >= seg_len ? 0ull : le_mask(seg_len - base - 1));
       bm[2 * w] = (uint32_t) v;
       bm[2 * w + 1] = (uint32_t) (v >> 32);
     }
-    lds_barrier();
     // pre-sieve p_4 .. p_{b_first - 1} (2, 3, 5 are outside the wheel)
     for (uint32_t j = 4; j <= qd.pre_last; j++)
       cross_off_presieve(primes[j], low, seg_len, bm);
-    lds_barrier();
+    // group counters and the segment total
+    uint32_t my_total = 0;
     {
       const uint4* w4 = reinterpret_cast<const uint4*>(bm);
-      for (uint32_t gp = tid; gp < W / 16; gp += kBlock) {
+      for (uint32_t gp = lane; gp < W / 16; gp += kBlock) {
         uint32_t c0 = 0, c1 = 0;
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -358,22 +354,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
           c1 += __popc(x1.x) + __popc(x1.y) + __popc(x1.z) + __popc(x1.w);
         }
         cnt1[gp] = c0 | (c1 << 16);
+        my_total += c0 + c1;
       }
     }
-    lds_barrier();
+    uint32_t total = wave_total_u32(my_total);   // unsieved numbers at the current b
 
     uint32_t b = qd.b_first;
     bool done = false;
     while (!done) {
-      // Window of up to 64 primes b .. b + 63.  Every wave computes the whole
-      // window (lane l <-> b + l) and keeps it in registers; an epoch reads
-      // its prime's values with v_readlane, so the epoch loop touches no
-      // memory for per-prime data.  The window ends before the first b whose
-      // leaf range is provably empty (that b and all later ones stop the
-      // segment, src/gourdon/D.cpp:131-132, 289-290).
+      // Window of up to 64 primes b .. b + 63, lane l <-> b + l, kept in
+      // registers; an epoch reads its prime's values with v_readlane.  The
+      // window ends before the first b whose leaf range is provably empty
+      // (that b and all later ones stop the segment, src/gourdon/D.cpp:
+      // 131-132, 289-290).
       const uint32_t bb = b + lane;
       uint32_t r_lo = 0, r_hi = 0, r_p = 0, r_T = 0;
-      uint64_t r_xlo = 0, r_qlo = 0, r_qhi = 0;
+      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0;
+      uint64_t r_xlo = 0;
       double r_xd = 0;
       bool brk = true;
       if (bb <= b_stop) {
@@ -397,9 +394,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         r_T = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
         r_xlo = bi.xp_lo;
         r_xd = bi.xp_d;
-        // first and last cofactor of p in the segment: p*q in (low, high)
-        r_qlo = div_exact((double) low, low, bi.p) + 1;
-        r_qhi = div_exact((double) (high - 1), high - 1, bi.p);
+        // cofactors q of p with p*q in (low, high): q_lo .. q_lo + nq - 1
+        const uint64_t q_lo = div_exact((double) low, low, bi.p) + 1;
+        const uint64_t q_hi = div_exact((double) (high - 1), high - 1, bi.p);
+        r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
+        r_qm = (uint32_t) (q_lo % 30);
+        r_off0 = (uint32_t) (bi.p * q_lo - low);
       }
       const unsigned long long bal = __ballot(brk);
       const uint32_t nwin = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
@@ -414,40 +414,35 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         const bool t1 = (readlane_u32(r_T, 0) >> 31) != 0;
 #pragma unroll
         for (int k = 0; k < kPf; k++) {
-          uint32_t idx = lo + 1 + tid + k * kBlock;
+          uint32_t idx = lo + 1 + lane + k * kBlock;
           pf[k] = (nwin > 0 && idx <= hi) ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
         }
       }
 
+      // C(s, b) and M(s, b) of the window, lane j <-> b + j; written to HBM
+      // once per window so no store is pending inside the epochs
+      uint32_t w_C = 0;
+      int w_M = 0;
       for (uint32_t j = 0; j < nwin; j++) {
-        const uint32_t br = b + j - qd.b_first;
         const uint32_t lo = readlane_u32(r_lo, j);
         const uint32_t hi = readlane_u32(r_hi, j);
-        const uint32_t Tt = readlane_u32(r_T, j);
-        const uint32_t p = readlane_u32(r_p, j);
-        const uint64_t xlo = readlane_u64(r_xlo, j);
-        const double xd = __longlong_as_double(
-            (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
-        const bool t1 = (Tt >> 31) != 0;
-        const uint32_t T = Tt & 0x7fffffffu;
-
-        // super-group prefixes and C(s, b)
-        uint32_t v = 0;
-        if (lane < NS)
-          v = sum8(reinterpret_cast<const uint4*>(cnt1)[lane]);
-        const uint32_t incl = wave_incl_scan(v);
-        if (lane < NS)
-          pre_w[wave][lane] = incl - v;
-        const uint32_t total = readlane_u32(incl, 63);
-        if (tid == 0)
-          TC[br] = total;
+        w_C = lane == j ? total : w_C;
 
         int M = 0;
         if (hi > lo) {
-          const uint32_t* pre = pre_w[wave];
+          const uint32_t Tt = readlane_u32(r_T, j);
+          const bool t1 = (Tt >> 31) != 0;
+          const uint32_t T = Tt & 0x7fffffffu;
+          const uint64_t xlo = readlane_u64(r_xlo, j);
+          const double xd = __longlong_as_double(
+              (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
+          // super-group prefixes for the count queries
+          uint32_t v = lane < NS ? sum8(reinterpret_cast<const uint4*>(cnt1)[lane]) : 0u;
+          uint32_t incl = wave_incl_scan(v);
+          pre_s[lane] = incl - v;
           uint32_t k = 0;
 #pragma nounroll
-          for (uint32_t idx = lo + 1 + tid; idx <= hi; idx += kBlock, k++) {
+          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock, k++) {
             // entry: prefetched for the first kPf strides, loaded otherwise
             uint32_t f = k == 0 ? pf[0] : (k == 1 ? pf[1] : (t1 ? (uint32_t) factor[idx] : primes[idx]));
             uint64_t m;
@@ -464,55 +459,81 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
               sg = 1;
             }
             uint64_t n = div_exact(xd, xlo, m);
-            uint32_t cnt = seg_count(bm, cnt1, pre, (uint32_t) (n - low));
+            uint32_t cnt = seg_count(bm, cnt1, pre_s, (uint32_t) (n - low));
             S += sg * (long long) cnt;
             M += sg;
             n_leaves++;
           }
+          M = (int) wave_total_u32((uint32_t) M);
         }
+        // Every load of this prime has been consumed.  Saying so with an
+        // explicit vmcnt(0) lets the compiler's wait analysis see that only
+        // the prefetch below is in flight during the cross-off (otherwise a
+        // conservative wait there would expose the prefetch latency).
+        __builtin_amdgcn_s_waitcnt(0x0F70);
+        w_M = lane == j ? M : w_M;
 
         // prefetch the next prime's entries; they arrive during the cross-off
         if (j + 1 < nwin) {
           const uint32_t lo2 = readlane_u32(r_lo, j + 1);
           const uint32_t hi2 = readlane_u32(r_hi, j + 1);
-          const bool t1n = (readlane_u32(r_T, j + 1) >> 31) != 0;
+          if (hi2 > lo2) {
+            const bool t1n = (readlane_u32(r_T, j + 1) >> 31) != 0;
 #pragma unroll
-          for (int k = 0; k < kPf; k++) {
-            uint32_t idx = lo2 + 1 + tid + k * kBlock;
-            pf[k] = idx <= hi2 ? (t1n ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+            for (int k = 0; k < kPf; k++) {
+              uint32_t idx = lo2 + 1 + lane + k * kBlock;
+              pf[k] = idx <= hi2 ? (t1n ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+            }
           }
         }
 
-        if (M != 0)
-          atomicAdd(&sh_m[j & 1], M);
-        lds_barrier();
-        if (tid == 0) {
-          TM[br] = sh_m[j & 1];
-          sh_m[j & 1] = 0;
-          n_slots++;
+        // cross off p_b; first-time removals leave C(s, b + 1)
+        const uint32_t p = readlane_u32(r_p, j);
+        const uint32_t nq = readlane_u32(r_nq, j);
+        uint32_t removed = 0;
+        if (nq > 0 && p >= 7) {
+          const uint32_t qm = readlane_u32(r_qm, j);
+          const uint32_t off0 = readlane_u32(r_off0, j);
+          const uint32_t cls = lane & 7;
+          const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
+          const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
+          if (delta < nq) {
+            const uint32_t n = (nq - 1 - delta) / 30 + 1;
+            const uint32_t off = off0 + p * delta;
+            const uint32_t byte0 = off / 30;
+            const uint32_t bib = wheel_bit(off - byte0 * 30);
+            const uint32_t j0 = lane >> 3;
+            for (uint32_t t = j0; t < n; t += 8) {
+              uint32_t B = byte0 + t * p;
+              uint32_t mask = 1u << (((B & 3) << 3) | bib);
+              uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
+              if (old & mask) {
+                removed++;
+                uint32_t g = B >> 6;
+                atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
+              }
+            }
+            if (j0 < n)
+              n_hits += (n - j0 + 7) / 8;
+          }
         }
-        const uint64_t qlo = readlane_u64(r_qlo, j);
-        const uint64_t qhi = readlane_u64(r_qhi, j);
-        cross_off_q<true>(p, qlo, qhi, low, bm, cnt1, n_hits);
-        lds_barrier();
+        total -= wave_total_u32(removed);
       }
+      if (lane < nwin) {
+        TC[b + lane - qd.b_first] = w_C;
+        TM[b + lane - qd.b_first] = w_M;
+      }
+      n_slots += nwin;
       b += nwin;
     }
     // slots the segment never reached hold zero
-    for (uint32_t bb = b + tid; bb <= b_stop; bb += kBlock) {
-      TC[bb - qd.b_first] = 0;
-      TM[bb - qd.b_first] = 0;
+    for (uint32_t bq = b + lane; bq <= b_stop; bq += kBlock) {
+      TC[bq - qd.b_first] = 0;
+      TM[bq - qd.b_first] = 0;
     }
-    long long ws = wave_sum_i64(S);
+    long long tot = wave_sum_i64(S);
     if (lane == 0)
-      sh_red[wave] = ws;
-    lds_barrier();
-    if (tid == 0) {
-      long long tot = 0;
-      for (int i = 0; i < kWaves; i++)
-        tot += sh_red[i];
       S_out[item] = tot;
-    }
   }
 
   long long l = wave_sum_i64((long long) n_leaves);
@@ -522,9 +543,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     atomicAdd(&counters[0], (unsigned long long) l);
     atomicAdd(&counters[1], (unsigned long long) h);
     atomicAdd(&counters[2], (unsigned long long) e);
-  }
-  if (tid == 0)
     atomicAdd(&counters[3], n_slots);
+  }
 }
 
 template <typename FT>

@@ -71,14 +71,14 @@ struct HslArgs {
   const uint32_t* pi_cnt;       // primes >= 7 below 240 * i
   uint32_t* T_C;                // per slot: unsieved count before p_b is crossed off
   int32_t* T_M;                 // per slot: sum of leaf signs
-  long long* S_out;             // per workgroup: sum of sign * in-segment count
+  long long* S_out;             // per item: sum of sign * in-segment count
   unsigned long long* counters; // leaves, hits, factor entries, slots
   unsigned int* queue;          // next work item
   uint32_t nitems;
   uint32_t W;                   // words per segment (power of two, 256..4096)
 };
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 64;     // one wave per segment
 
 /// Column of the carry combine: one (query, b) within one pass.
 /// Its items are items[item_first .. item_first + nitems) (consecutive
