@@ -94,6 +94,9 @@ def main():
     ap.add_argument("--cpu-timeout", type=int, default=120)
     ap.add_argument("--segment-words", type=int, default=0,
                     help="64-bit words per sieve segment (0 = automatic)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
+                         "FETCH_SIZE/WRITE_SIZE calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the hard-special-leaf engine runs on a single MI355X "
@@ -189,6 +192,15 @@ def main():
         "roofline_lds": roofline_lds,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),
     }
+    if a.calibrate_copy:
+        import torch
+        n = 256 << 20                     # 2^28 float32 = 1 GiB read + 1 GiB written
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        out["calibration_copy_bytes"] = 4 * n
     print(json.dumps(out), flush=True)
 
 

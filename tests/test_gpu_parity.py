@@ -107,3 +107,34 @@ def test_errors_are_reported(pc):
         pc.D(10 ** 12, 100, 50, 8)            # z < y is outside the domain
     # the engine still works after an error
     assert pc.D(10 ** 6, 207, 207, 8) == 2465
+
+
+def test_S2_hard_1e19(pc):
+    log("S2_hard default 1e19")
+    r = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}[10 ** 19]
+    assert pc.S2_hard_default(10 ** 19) == int(r["value"])
+
+
+def test_int128_path_1e21(pc):
+    # x > 2^64: x / p_b needs more than 64 bits (128-bit path of D.cpp:456-505)
+    big = golden("configs_big.json")
+    r = big["D"][0]
+    log("D default 1e21 (128-bit)")
+    x = int(r["x"])
+    assert x > 2 ** 64
+    assert pc.D_default(x) == int(r["value"])
+    assert pc.D128(x, int(r["y"]), int(r["z"]), int(r["k"])) == int(r["value"])
+    r = big["S2_hard"][0]
+    log("S2_hard default 1e21 (128-bit)")
+    assert pc.S2_hard_default(x) == int(r["value"])
+
+
+def test_int128_entry_near_2_63(pc):
+    # x just below and above 2^63: the 64-bit and the 128-bit entry points agree
+    for x in (2 ** 63 - 1, 2 ** 63 + 12345):
+        p = pc.gourdon_params(x)
+        log("D near 2^63: x=%d" % x)
+        got = pc.D128(x, p["y"], p["z"], p["k"])
+        if x < 2 ** 63:
+            assert pc.D(x, p["y"], p["z"], p["k"]) == got
+        assert got > 0
