@@ -58,6 +58,23 @@ def traffic_from_profile(x):
     return None
 
 
+def device_copy(nbytes, reps=2):
+    """A plain device-to-device copy of `nbytes` (HIP's own copy kernel), used
+    to calibrate the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(reps):
+        assert hip.hipMemcpyDtoD(dst, src, ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    return nbytes
+
+
 def cpu_baseline(x, timeout):
     cli = os.path.join(REPO, "oracle", "_ref", "primecount")
     if not os.path.exists(cli):
@@ -193,14 +210,7 @@ def main():
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),
     }
     if a.calibrate_copy:
-        import torch
-        n = 256 << 20                     # 2^28 float32 = 1 GiB read + 1 GiB written
-        src = torch.ones(n, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        out["calibration_copy_bytes"] = 4 * n
+        out["calibration_copy_bytes"] = device_copy(1 << 30)
     print(json.dumps(out), flush=True)
 
 
