@@ -1,0 +1,63 @@
+"""Host-side logic and the C-ABI boundary, on the CPU (no GPU calls).
+
+- the table builders against the arithmetic definitions (a C++ harness
+  compiled here from primecount_amd/csrc/tables.cpp);
+- the shared library loads and exports every function that
+  include/primecount_mi355x.h declares; the C++ overload header compiles;
+- without a device the engine refuses to run (no silent CPU fallback).
+"""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_table_builders_match_definitions(tmp_path):
+    exe = tmp_path / "tables_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-o", str(exe),
+                    os.path.join(REPO, "tests", "native", "tables_check.cpp"),
+                    os.path.join(REPO, "primecount_amd", "csrc", "tables.cpp")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+def declared_functions():
+    text = open(os.path.join(REPO, "include", "primecount_mi355x.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(pcg_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol(pc):
+    lib = ctypes.CDLL(pc.library_path())
+    names = declared_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_cpp_header_compiles(tmp_path):
+    src = tmp_path / "use.cpp"
+    src.write_text('#include "primecount_mi355x.hpp"\n'
+                   'int main() { return (int) primecount_mi355x::D((int64_t) 1000, 10, 10, 2); }\n')
+    subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(REPO, "include"),
+                    str(src)], check=True)
+
+
+def test_version_and_no_silent_fallback(pc):
+    lib = pc.load()
+    assert b"gfx950" in lib.pcg_version()
+    if pc.device_available():
+        pytest.skip("a GPU is present; the no-device path is covered on CPU hosts")
+    with pytest.raises(pc.PrimecountError, match="HIP device|gfx950"):
+        pc.D(10 ** 12, 76500, 76500, 8)
+
+
+def test_argument_errors(pc):
+    with pytest.raises(pc.PrimecountError):
+        pc.set_segment_words(1000)            # not a power of two
+    with pytest.raises(pc.PrimecountError):
+        pc.set_host_threads(-1)
+    pc.set_segment_words(0)
