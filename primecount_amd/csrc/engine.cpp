@@ -271,15 +271,15 @@ void plan_segments(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint3
 }
 
 /// Segment size.  One wave owns a segment, so LDS per wave (8 W bytes) sets
-/// how many waves a CU keeps resident; on gfx950 W = 1024 (245,760 numbers)
-/// measured fastest for large sieve limits (W = 512 and 2048 were 8% and 90%
-/// slower at x = 1e19).  Small limits use smaller segments so that every
-/// resident wave gets work.
+/// how many waves a CU keeps resident; on gfx950 W = 512 (122,880 numbers)
+/// measured fastest for large sieve limits (at x = 1e19 the sieve kernel took
+/// 300 ms at W = 512, 373 ms at 1024 and 400 ms at 256).  Small limits use
+/// smaller segments so that every resident wave gets work.
 uint32_t auto_segment_words(uint64_t limit, uint64_t slots)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
-  uint32_t W = 1024;
+  uint32_t W = 512;
   while (W > 256 && (limit + 240ull * W - 1) / (240ull * W) < 2 * slots)
     W >>= 1;
   return W;

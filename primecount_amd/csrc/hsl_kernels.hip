@@ -1,24 +1,28 @@
 // hsl_kernels.hip -- CDNA4 (gfx950) kernels of the hard-special-leaf engine.
 //
-// A persistent grid of 256-thread workgroups takes segments from a device
-// work queue.  A workgroup holds its segment in LDS as a wheel-30 bitmap (240
+// A persistent grid of single-wave workgroups takes segments from a device
+// work queue.  A wave holds its segment in LDS as a wheel-30 bitmap (240
 // numbers per 64-bit word, the layout of src/sieve/Sieve.cpp:62-66) plus
 // 16-bit counters of the unsieved numbers per group of 8 words, and walks the
-// sieving primes b = b_first, b_first + 1, ... in epochs:
-//   1. every wave scans the group counters into super-group (64 word)
-//      prefixes; the total is C(s, b), the count of unsieved numbers before
-//      p_b is crossed off;
-//   2. the leaves of p_b in the segment are answered: factor-table entries
-//      (or primes) of the leaf range are filtered, the exact quotient
-//      n = x / (p_b * m) is formed, and the unsieved numbers <= n are counted
-//      from the prefixes, the counters and popcounts of at most 8 words;
-//      M(s, b) is the sum of the leaf signs;
-//   3. p_b and its multiples are crossed off with LDS atomics, decrementing
-//      the group counter of every number removed for the first time
-//      (src/sieve/Sieve.cpp:497-845 counts the same first-time removals).
-// Epochs are separated by LDS-only barriers, so the global loads of the next
-// prime's factor-table entries stay in flight across them.  The leaf ranges of
-// up to 256 consecutive primes are computed in one parallel step per window.
+// sieving primes b = b_first, b_first + 1, ... in windows of up to 64 primes
+// (lane l holds the per-prime values of b + l):
+//   * leaves of p_b: factor-table entries (or primes) of the leaf range are
+//     filtered, the exact quotient n = x / (p_b * m) is formed, and the
+//     unsieved numbers <= n are counted from super-group (64 word) prefixes,
+//     the group counters and popcounts of at most 8 words; M(s, b) is the sum
+//     of the leaf signs;
+//   * cross-off of p_b: p_b and its multiples are cleared with LDS atomics;
+//     every number removed for the first time decrements its group counter
+//     and is charged to p_b (src/sieve/Sieve.cpp:497-845 counts the same
+//     first-time removals), which gives C(s, b), the count of unsieved numbers
+//     before p_b is crossed off.
+// Small primes go one at a time: leaves, then cross-off.  Once every cofactor
+// q of p_b in the segment is below p_b^2 (from then on for all larger primes
+// of the segment), a multiple p_b * q that is still unsieved has q = 1 or q
+// prime, so its least prime factor is min(p_b, q) and the first-time charge
+// no longer depends on the order of the cross-offs.  The cross-offs of such
+// primes run together across lanes, and only a prime that has leaves forces
+// the pending ones to be applied before its count.
 //
 // The reference carries phi(low, b-1) sequentially through its segments
 // (src/gourdon/D.cpp:96, 271).  Here each segment writes its C(s, b) and
@@ -57,6 +61,12 @@ constexpr uint64_t le_nibbles(int first)
 constexpr uint64_t kLeLo = le_nibbles(0);
 constexpr uint64_t kLeHi = le_nibbles(16);
 
+/// Number of wheel residues {1, 7, 11, ..., 29} that are <= r (r < 30).
+__device__ __forceinline__ uint32_t le_count(uint32_t r)
+{
+  return (uint32_t) ((r < 16 ? (kLeLo >> (4 * r)) : (kLeHi >> (4 * (r - 16)))) & 15);
+}
+
 /// Bits of a sieve word that stand for the offsets <= r (r < 240).
 __device__ __forceinline__ uint64_t le_mask(uint32_t r)
 {
@@ -74,6 +84,18 @@ __device__ __forceinline__ void lds_barrier()
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
   __builtin_amdgcn_s_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+/// The lanes of a wave exchange values through LDS.  Without a fence the
+/// compiler may reuse what a lane itself stored to an address (another lane's
+/// store in between is invisible to it in the per-thread model); this fence
+/// orders the wave's LDS accesses for the compiler.  The hardware executes a
+/// wave's LDS operations in order, so no wait is needed at wave scope.
+__device__ __forceinline__ void wave_lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 /// floor(xp / d) for a quotient < 2^50 and d < 2^53.  xd is xp rounded to
@@ -209,8 +231,25 @@ __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t
   return c;
 }
 
+/// Inclusive prefix maximum over the 64 lanes of a wave (same DPP pattern as
+/// wave_incl_scan).
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t v)
+{
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x111, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x112, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x114, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x118, 0xf, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x142, 0xa, 0xf, false));
+  v = max(v, (uint32_t) __builtin_amdgcn_update_dpp(0, (int) v, 0x143, 0xc, 0xf, false));
+  return v;
+}
+
 /// Residues coprime to 30 in bit order, packed as bytes: 1, 7, 11, ..., 29.
 constexpr uint64_t kWheelBytes = 0x1D1713110D0B0701ull;
+
+/// Bit r set for the residues r in [0, 30) coprime to 30.
+constexpr uint32_t kCop30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) |
+                            (1u << 17) | (1u << 19) | (1u << 23) | (1u << 29);
 
 /// Bit (0..7) of a wheel residue r in a sieve byte.
 __device__ __forceinline__ uint32_t wheel_bit(uint32_t r)
@@ -302,6 +341,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   uint32_t* cnt1 = bm + 2 * W;                                // W/16 u32
   uint16_t* cop = reinterpret_cast<uint16_t*>(cnt1 + W / 16); // 480 u16
   __shared__ uint32_t pre_s[64];                              // super-group prefixes
+  __shared__ uint32_t w_rem[64];                              // per window prime: removals
+  __shared__ int w_m[64];                                     // per window prime: leaf signs
+  __shared__ uint4 wp[64];                                    // per window prime: cross-off data
 
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 480; i += kBlock)
@@ -340,6 +382,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     // pre-sieve p_4 .. p_{b_first - 1} (2, 3, 5 are outside the wheel)
     for (uint32_t j = 4; j <= qd.pre_last; j++)
       cross_off_presieve(primes[j], low, seg_len, bm);
+    wave_lds_sync();
     // group counters and the segment total
     uint32_t my_total = 0;
     {
@@ -358,6 +401,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       }
     }
     uint32_t total = wave_total_u32(my_total);   // unsieved numbers at the current b
+    wave_lds_sync();
 
     uint32_t b = qd.b_first;
     bool done = false;
@@ -369,10 +413,11 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       // 131-132, 289-290).
       const uint32_t bb = b + lane;
       uint32_t r_lo = 0, r_hi = 0, r_p = 0, r_T = 0;
-      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0;
+      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0, r_h = 0;
       uint64_t r_xlo = 0;
       double r_xd = 0;
-      bool brk = true;
+      bool brk = true, par = false;
+      uint4 prm = make_uint4(0, 0, 0, 0);
       if (bb <= b_stop) {
         const BInfo bi = binfo[bb - qd.b_first];
         const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
@@ -400,35 +445,117 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
         r_qm = (uint32_t) (q_lo % 30);
         r_off0 = (uint32_t) (bi.p * q_lo - low);
+        // Every cofactor is below p^2: a number of the segment that is still
+        // unsieved when p is crossed off and is a multiple of p is p * q with
+        // q = 1 or q prime, and its least prime factor is min(p, q).
+        par = q_hi < (uint64_t) bi.p * bi.p;
+        if (r_nq > 0 && bi.p >= 7) {
+          // hits: cofactors coprime to 30; c0 = wheel residues below q_lo % 30
+          // (2, 3 and 5 are not in the wheel and have no hits)
+          const uint32_t c0 = r_qm ? le_count(r_qm - 1) : 0u;
+          const uint32_t v = r_qm + r_nq - 1;
+          const uint32_t vb = v / 30;
+          r_h = 8 * vb + le_count(v - 30 * vb) - c0;
+          prm.y = r_off0 | (r_qm << 20) | (c0 << 25);   // off0 < 2^20 when nq > 0
+        }
+        prm.x = bi.p;
+        prm.z = q_lo < bi.p ? (uint32_t) (bi.p - q_lo) : 0u;
       }
       const unsigned long long bal = __ballot(brk);
       const uint32_t nwin = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
       if (nwin < 64)
         done = true;
+      const unsigned long long in_win = nwin >= 64 ? ~0ull : ((1ull << nwin) - 1);
+      // Primes [0, seq_end) of the window are processed one at a time; from
+      // seq_end on (the condition above is monotone in p) the cross-offs of
+      // consecutive primes run together, and only a prime with factor
+      // entries forces the pending ones to be applied before its count.
+      const unsigned long long pbal = __ballot(par) & in_win;
+      const uint32_t seq_end = pbal ? (uint32_t) __ffsll((long long) pbal) - 1 : nwin;
+      const bool in_par = lane >= seq_end && lane < nwin;
+      const unsigned long long lpar = __ballot(in_par && r_hi > r_lo);
+      // lanes per prime in the parallel cross-off: a power of two >= the
+      // largest hit count of the window's parallel primes (at most 64)
+      const uint32_t hmax = readlane_u32(wave_incl_max(in_par ? r_h : 0u), 63);
+      const uint32_t lg = hmax <= 1 ? 0u : (hmax >= 64 ? 6u : 32u - (uint32_t) __clz(hmax - 1));
+      prm.w = r_h;
+      wp[lane] = prm;
+      w_rem[lane] = 0;
+      w_m[lane] = 0;
+      wave_lds_sync();
 
-      // factor entries (or primes) of the window's first prime
-      uint32_t pf[kPf];
-      {
-        const uint32_t lo = readlane_u32(r_lo, 0);
-        const uint32_t hi = readlane_u32(r_hi, 0);
-        const bool t1 = (readlane_u32(r_T, 0) >> 31) != 0;
-#pragma unroll
-        for (int k = 0; k < kPf; k++) {
-          uint32_t idx = lo + 1 + lane + k * kBlock;
-          pf[k] = (nwin > 0 && idx <= hi) ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+      // Cross off primes [r0, r1) of the parallel part together: 2^lg lanes
+      // per prime, lane k of a prime's group takes its hits k, k + 2^lg, ...
+      // (a hit is the k-th cofactor coprime to 30).  A number removed for the
+      // first time is charged to its least prime factor: p, or the cofactor q
+      // when 1 < q < p (q is then a prime of this run).
+      auto flush = [&](uint32_t r0, uint32_t r1) {
+        const uint32_t G = 1u << lg;
+        for (uint32_t jb = r0; jb < r1; jb += kBlock >> lg) {
+          const uint32_t jo = jb + (lane >> lg);
+          if (jo < r1) {
+            const uint4 pp = wp[jo];
+            const uint32_t c0 = pp.y >> 25, qm = (pp.y >> 20) & 31, off0 = pp.y & 0xFFFFFu;
+            for (uint32_t kk = lane & (G - 1); kk < pp.w; kk += G) {
+              const uint32_t idx = c0 + kk;
+              const uint32_t delta = 30 * (idx >> 3) +
+                                     (uint32_t) ((kWheelBytes >> (8 * (idx & 7))) & 0xff) - qm;
+              const uint32_t off = off0 + pp.x * delta;
+              const uint32_t B = off / 30;
+              const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
+              const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
+              n_hits++;
+              if (old & mask) {
+                const uint32_t g = B >> 6;
+                atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
+                uint32_t attr = jo;
+                if (delta < pp.z) {                     // q = q_lo + delta < p
+                  const uint32_t q = pp.x - pp.z + delta;
+                  if (q > 1)
+                    attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
+                }
+                atomicAdd(&w_rem[attr], 1u);
+              }
+            }
+          }
         }
-      }
+      };
 
-      // C(s, b) and M(s, b) of the window, lane j <-> b + j; written to HBM
-      // once per window so no store is pending inside the epochs
-      uint32_t w_C = 0;
-      int w_M = 0;
-      for (uint32_t j = 0; j < nwin; j++) {
+      // factor entries (or primes) of the next prime with a leaf range
+      uint32_t pf[kPf];
+      auto prefetch = [&](uint32_t jj) {
+        const uint32_t lo = readlane_u32(r_lo, jj);
+        const uint32_t hi = readlane_u32(r_hi, jj);
+        if (hi > lo) {
+          const bool t1 = (readlane_u32(r_T, jj) >> 31) != 0;
+#pragma unroll
+          for (int k = 0; k < kPf; k++) {
+            uint32_t idx = lo + 1 + lane + k * kBlock;
+            pf[k] = idx <= hi ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
+          }
+        }
+      };
+      // processing order: every prime below seq_end, then the primes of the
+      // parallel part that have a leaf range
+      auto next_prime = [&](uint32_t c) -> uint32_t {
+        if (c < seq_end)
+          return c;
+        const unsigned long long m = c >= 64 ? 0ull : (lpar & (~0ull << c));
+        return m ? (uint32_t) __ffsll((long long) m) - 1 : 64u;
+      };
+
+      uint32_t j = next_prime(0);
+      if (j < nwin)
+        prefetch(j);
+      uint32_t run = seq_end;           // first prime whose cross-off is pending
+      while (j < nwin) {
+        const uint32_t jn = next_prime(j + 1);
+        if (j >= seq_end) {
+          flush(run, j);
+          run = j;
+        }
         const uint32_t lo = readlane_u32(r_lo, j);
         const uint32_t hi = readlane_u32(r_hi, j);
-        w_C = lane == j ? total : w_C;
-
-        int M = 0;
         if (hi > lo) {
           const uint32_t Tt = readlane_u32(r_T, j);
           const bool t1 = (Tt >> 31) != 0;
@@ -440,6 +567,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           uint32_t v = lane < NS ? sum8(reinterpret_cast<const uint4*>(cnt1)[lane]) : 0u;
           uint32_t incl = wave_incl_scan(v);
           pre_s[lane] = incl - v;
+          wave_lds_sync();
+          int M = 0;
           uint32_t k = 0;
 #pragma nounroll
           for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock, k++) {
@@ -464,64 +593,84 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             M += sg;
             n_leaves++;
           }
-          M = (int) wave_total_u32((uint32_t) M);
+          if (M != 0)
+            atomicAdd(&w_m[j], M);
         }
         // Every load of this prime has been consumed.  Saying so with an
         // explicit vmcnt(0) lets the compiler's wait analysis see that only
         // the prefetch below is in flight during the cross-off (otherwise a
         // conservative wait there would expose the prefetch latency).
         __builtin_amdgcn_s_waitcnt(0x0F70);
-        w_M = lane == j ? M : w_M;
+        if (jn < nwin)
+          prefetch(jn);
 
-        // prefetch the next prime's entries; they arrive during the cross-off
-        if (j + 1 < nwin) {
-          const uint32_t lo2 = readlane_u32(r_lo, j + 1);
-          const uint32_t hi2 = readlane_u32(r_hi, j + 1);
-          if (hi2 > lo2) {
-            const bool t1n = (readlane_u32(r_T, j + 1) >> 31) != 0;
-#pragma unroll
-            for (int k = 0; k < kPf; k++) {
-              uint32_t idx = lo2 + 1 + lane + k * kBlock;
-              pf[k] = idx <= hi2 ? (t1n ? (uint32_t) factor[idx] : primes[idx]) : 0u;
-            }
-          }
-        }
-
-        // cross off p_b; first-time removals leave C(s, b + 1)
-        const uint32_t p = readlane_u32(r_p, j);
-        const uint32_t nq = readlane_u32(r_nq, j);
-        uint32_t removed = 0;
-        if (nq > 0 && p >= 7) {
-          const uint32_t qm = readlane_u32(r_qm, j);
-          const uint32_t off0 = readlane_u32(r_off0, j);
-          const uint32_t cls = lane & 7;
-          const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
-          const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
-          if (delta < nq) {
-            const uint32_t n = (nq - 1 - delta) / 30 + 1;
-            const uint32_t off = off0 + p * delta;
-            const uint32_t byte0 = off / 30;
-            const uint32_t bib = wheel_bit(off - byte0 * 30);
-            const uint32_t j0 = lane >> 3;
-            for (uint32_t t = j0; t < n; t += 8) {
-              uint32_t B = byte0 + t * p;
-              uint32_t mask = 1u << (((B & 3) << 3) | bib);
-              uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-              if (old & mask) {
-                removed++;
-                uint32_t g = B >> 6;
-                atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
+        if (j < seq_end) {
+          // cross off p_b; the first-time removals are C(s, b) - C(s, b + 1)
+          const uint32_t p = readlane_u32(r_p, j);
+          const uint32_t nq = readlane_u32(r_nq, j);
+          uint32_t removed = 0;
+          if (nq > 0 && p >= 7) {
+            const uint32_t qm = readlane_u32(r_qm, j);
+            const uint32_t off0 = readlane_u32(r_off0, j);
+            if (nq <= kBlock) {
+              // sparse prime: lane l takes cofactor q_lo + l (if coprime to 30)
+              const uint32_t t = (qm + lane) % 30;
+              if (lane < nq && ((kCop30 >> t) & 1)) {
+                const uint32_t off = off0 + p * lane;
+                const uint32_t B = off / 30;
+                const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
+                const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
+                n_hits++;
+                if (old & mask) {
+                  removed = 1;
+                  const uint32_t g = B >> 6;
+                  atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
+                }
+              }
+            } else {
+              // dense prime: lanes split the 8 residue classes of q mod 30 and
+              // stride 8 multiples apart inside a class (p bytes per multiple)
+              const uint32_t cls = lane & 7;
+              const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
+              const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
+              if (delta < nq) {
+                const uint32_t n = (nq - 1 - delta) / 30 + 1;
+                const uint32_t off = off0 + p * delta;
+                const uint32_t byte0 = off / 30;
+                const uint32_t bib = wheel_bit(off - byte0 * 30);
+                const uint32_t j0 = lane >> 3;
+                for (uint32_t t = j0; t < n; t += 8) {
+                  uint32_t B = byte0 + t * p;
+                  uint32_t mask = 1u << (((B & 3) << 3) | bib);
+                  uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
+                  if (old & mask) {
+                    removed++;
+                    uint32_t g = B >> 6;
+                    atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
+                  }
+                }
+                if (j0 < n)
+                  n_hits += (n - j0 + 7) / 8;
               }
             }
-            if (j0 < n)
-              n_hits += (n - j0 + 7) / 8;
           }
+          if (removed != 0)
+            atomicAdd(&w_rem[j], removed);
         }
-        total -= wave_total_u32(removed);
+        j = jn;
       }
-      if (lane < nwin) {
-        TC[b + lane - qd.b_first] = w_C;
-        TM[b + lane - qd.b_first] = w_M;
+      flush(run, nwin);
+      wave_lds_sync();
+      {
+        // C(s, b + l) = count at the window start minus the removals of the
+        // window's earlier primes
+        const uint32_t rem = lane < nwin ? w_rem[lane] : 0u;
+        const uint32_t incl = wave_incl_scan(rem);
+        if (lane < nwin) {
+          TC[b + lane - qd.b_first] = total - (incl - rem);
+          TM[b + lane - qd.b_first] = w_m[lane];
+        }
+        total -= readlane_u32(incl, 63);
       }
       n_slots += nwin;
       b += nwin;
