@@ -2,20 +2,19 @@
 //
 // A persistent grid of single-wave workgroups takes segments from a device
 // work queue.  A wave holds its segment in LDS as a wheel-30 bitmap (240
-// numbers per 64-bit word, the layout of src/sieve/Sieve.cpp:62-66) plus
-// 16-bit counters of the unsieved numbers per group of 8 words, and walks the
-// sieving primes b = b_first, b_first + 1, ... in windows of up to 64 primes
+// numbers per 64-bit word, the layout of src/sieve/Sieve.cpp:62-66) and walks
+// the sieving primes b = b_first, b_first + 1, ... in windows of up to 64 primes
 // (lane l holds the per-prime values of b + l):
 //   * leaves of p_b: factor-table entries (or primes) of the leaf range are
 //     filtered, the exact quotient n = x / (p_b * m) is formed, and the
-//     unsieved numbers <= n are counted from super-group (64 word) prefixes,
-//     the group counters and popcounts of at most 8 words; M(s, b) is the sum
-//     of the leaf signs;
+//     unsieved numbers <= n are counted from word-pair prefix counts (rebuilt
+//     for every prime that has leaves in the segment) and popcounts of at most
+//     2 words; M(s, b) is the sum of the leaf signs;
 //   * cross-off of p_b: p_b and its multiples are cleared with LDS atomics;
-//     every number removed for the first time decrements its group counter
-//     and is charged to p_b (src/sieve/Sieve.cpp:497-845 counts the same
-//     first-time removals), which gives C(s, b), the count of unsieved numbers
-//     before p_b is crossed off.
+//     every number removed for the first time is charged to p_b
+//     (src/sieve/Sieve.cpp:497-845 counts the same first-time removals),
+//     which gives C(s, b), the count of unsieved numbers before p_b is
+//     crossed off.
 // Small primes go one at a time: leaves, then cross-off.  Once every cofactor
 // q of p_b in the segment is below p_b^2 (from then on for all larger primes
 // of the segment), a multiple p_b * q that is still unsieved has q = 1 or q
@@ -192,43 +191,67 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
   return v;
 }
 
-__device__ __forceinline__ uint32_t sum8(uint4 q)
+/// Prefix counts of the segment at word-pair granularity: pre2[g] = unsieved
+/// numbers in words [0, 2 g).  Lane l takes W / 128 consecutive pairs (one
+/// 128-bit LDS load each), then a wave scan gives every lane its base.
+__device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* pre2, uint32_t W,
+                                                  uint32_t lane)
 {
-  return (q.x & 0xffff) + (q.x >> 16) + (q.y & 0xffff) + (q.y >> 16) +
-         (q.z & 0xffff) + (q.z >> 16) + (q.w & 0xffff) + (q.w >> 16);
+  const uint32_t per = W >> 7;
+  const uint4* w4 = reinterpret_cast<const uint4*>(bm) + lane * per;
+  uint32_t* out = pre2 + lane * per;
+  if (per <= 8) {
+    uint32_t c[8];
+    uint32_t tot = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) {
+      uint32_t v = 0;
+      if (i < per) {
+        const uint4 q = w4[i];
+        v = __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+      }
+      c[i] = v;
+      tot += v;
+    }
+    uint32_t run = wave_incl_scan(tot) - tot;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; i++) {
+      if (i < per)
+        out[i] = run;
+      run += c[i];
+    }
+  } else {
+    uint32_t tot = 0;
+    for (uint32_t i = 0; i < per; i++) {
+      const uint4 q = w4[i];
+      tot += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+    }
+    uint32_t run = wave_incl_scan(tot) - tot;
+    for (uint32_t i = 0; i < per; i++) {
+      const uint4 q = w4[i];
+      out[i] = run;
+      run += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
+    }
+  }
 }
 
-/// Unsieved numbers in [low, low + stop] of the segment.  pre[sg] is the
-/// count before super-group sg (64 words); the group counters of sg and the
-/// words of the group are read with 128-bit LDS loads, all independent.
-__device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t* cnt1,
-                                              const uint32_t* pre, uint32_t stop)
+/// Unsieved numbers in [low, low + stop] of the segment (count_stop.hpp:
+/// the count is inclusive and relative to low): the prefix of the word pair,
+/// the whole first word of the pair when stop lies in its second word, and
+/// the masked word of stop.
+__device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t* pre2,
+                                              uint32_t stop)
 {
-  uint32_t w = stop / 240;
-  uint32_t r = stop - w * 240;
-  uint32_t g = w >> 3;
-  uint32_t sg = w >> 6;
-  uint32_t c = pre[sg];
-  uint4 v = reinterpret_cast<const uint4*>(cnt1)[sg];
-  const uint4* gw = reinterpret_cast<const uint4*>(bm) + 4 * g;   // 8 words of group g
-  uint4 w01 = gw[0], w23 = gw[1], w45 = gw[2], w67 = gw[3];
-  uint32_t ng = g & 7;
-  uint32_t h[7] = {v.x & 0xffff, v.x >> 16, v.y & 0xffff, v.y >> 16,
-                   v.z & 0xffff, v.z >> 16, v.w & 0xffff};
-#pragma unroll
-  for (uint32_t i = 0; i < 7; i++)
-    c += i < ng ? h[i] : 0;
-  uint32_t wi = w & 7;
-  uint32_t lo[8] = {w01.x, w01.z, w23.x, w23.z, w45.x, w45.z, w67.x, w67.z};
-  uint32_t hi[8] = {w01.y, w01.w, w23.y, w23.w, w45.y, w45.w, w67.y, w67.w};
-  uint64_t m = le_mask(r);
-#pragma unroll
-  for (uint32_t u = 0; u < 8; u++) {
-    uint32_t ml = u < wi ? ~0u : (u == wi ? (uint32_t) m : 0u);
-    uint32_t mh = u < wi ? ~0u : (u == wi ? (uint32_t) (m >> 32) : 0u);
-    c += __popc(lo[u] & ml) + __popc(hi[u] & mh);
-  }
-  return c;
+  const uint32_t w = stop / 240;
+  const uint32_t r = stop - w * 240;
+  const uint32_t g = w >> 1;
+  const uint4 q = reinterpret_cast<const uint4*>(bm)[g];
+  const uint64_t m = le_mask(r);
+  const bool odd = (w & 1) != 0;
+  uint32_t c = pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
+  const uint32_t lo = odd ? q.z : q.x;
+  const uint32_t hi = odd ? q.w : q.y;
+  return c + __popc(lo & (uint32_t) m) + __popc(hi & (uint32_t) (m >> 32));
 }
 
 /// Inclusive prefix maximum over the 64 lanes of a wave (same DPP pattern as
@@ -260,14 +283,11 @@ __device__ __forceinline__ uint32_t wheel_bit(uint32_t r)
 
 /// Cross off p and its multiples p*q, q in [q_lo, q_hi] coprime to 30, in the
 /// segment starting at low (Sieve::add/cross_off remove the multiples
-/// > low, src/sieve/Sieve.cpp:192-216).  Threads split the 8 residue classes
+/// > low, src/sieve/Sieve.cpp:192-216).  Lanes split the 8 residue classes
 /// of q mod 30 and stride inside a class: consecutive multiples of one class
-/// are p bytes apart.  With COUNT the first-time removals decrement the group
-/// counters.
-template <bool COUNT>
+/// are p bytes apart.
 __device__ __forceinline__ void cross_off_q(uint32_t p, uint64_t q_lo, uint64_t q_hi, uint64_t low,
-                                            uint32_t* bm, uint32_t* cnt1,
-                                            unsigned long long& hits)
+                                            uint32_t* bm)
 {
   if (p < 7 || q_hi < q_lo)
     return;
@@ -284,30 +304,18 @@ __device__ __forceinline__ void cross_off_q(uint32_t p, uint64_t q_lo, uint64_t 
   uint32_t bib = wheel_bit(off - byte0 * 30);
   for (uint32_t t = j0; t < n; t += kBlock / 8) {
     uint32_t B = byte0 + t * p;
-    uint32_t mask = 1u << (((B & 3) << 3) | bib);
-    if (COUNT) {
-      uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-      if (old & mask) {
-        uint32_t g = B >> 6;
-        atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
-      }
-    } else {
-      atomicAnd(&bm[B >> 2], ~mask);
-    }
+    atomicAnd(&bm[B >> 2], ~(1u << (((B & 3) << 3) | bib)));
   }
-  if (COUNT && j0 < n)
-    hits += (n - j0 + kBlock / 8 - 1) / (kBlock / 8);
 }
 
 /// Pre-sieve form: q range computed here.
 __device__ __forceinline__ void cross_off_presieve(uint32_t p, uint64_t low, uint32_t seg_len,
                                                    uint32_t* bm)
 {
-  unsigned long long dummy = 0;
   uint64_t q_lo = div_exact((double) low, low, p) + 1;
   uint64_t last = low + seg_len - 1;
   uint64_t q_hi = div_exact((double) last, last, p);
-  cross_off_q<false>(p, q_lo, q_hi, low, bm, nullptr, dummy);
+  cross_off_q(p, q_lo, q_hi, low, bm);
 }
 
 } // namespace
@@ -336,11 +344,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     uint32_t nitems, uint32_t W)
 {
   extern __shared__ __align__(16) uint32_t lds[];
-  const uint32_t NS = W >> 6;                                 // super-groups
   uint32_t* bm = lds;                                         // 2W u32
-  uint32_t* cnt1 = bm + 2 * W;                                // W/16 u32
-  uint16_t* cop = reinterpret_cast<uint16_t*>(cnt1 + W / 16); // 480 u16
-  __shared__ uint32_t pre_s[64];                              // super-group prefixes
+  uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
+  uint16_t* cop = reinterpret_cast<uint16_t*>(pre2 + W / 2);  // 480 u16
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
   __shared__ int w_m[64];                                     // per window prime: leaf signs
   __shared__ uint4 wp[64];                                    // per window prime: cross-off data
@@ -383,21 +389,13 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     for (uint32_t j = 4; j <= qd.pre_last; j++)
       cross_off_presieve(primes[j], low, seg_len, bm);
     wave_lds_sync();
-    // group counters and the segment total
+    // the segment total
     uint32_t my_total = 0;
     {
       const uint4* w4 = reinterpret_cast<const uint4*>(bm);
-      for (uint32_t gp = lane; gp < W / 16; gp += kBlock) {
-        uint32_t c0 = 0, c1 = 0;
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          uint4 x0 = w4[8 * gp + u];
-          uint4 x1 = w4[8 * gp + 4 + u];
-          c0 += __popc(x0.x) + __popc(x0.y) + __popc(x0.z) + __popc(x0.w);
-          c1 += __popc(x1.x) + __popc(x1.y) + __popc(x1.z) + __popc(x1.w);
-        }
-        cnt1[gp] = c0 | (c1 << 16);
-        my_total += c0 + c1;
+      for (uint32_t i = lane; i < W / 2; i += kBlock) {
+        const uint4 q = w4[i];
+        my_total += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
       }
     }
     uint32_t total = wave_total_u32(my_total);   // unsieved numbers at the current b
@@ -506,8 +504,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
               n_hits++;
               if (old & mask) {
-                const uint32_t g = B >> 6;
-                atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
                 uint32_t attr = jo;
                 if (delta < pp.z) {                     // q = q_lo + delta < p
                   const uint32_t q = pp.x - pp.z + delta;
@@ -563,10 +559,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           const uint64_t xlo = readlane_u64(r_xlo, j);
           const double xd = __longlong_as_double(
               (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
-          // super-group prefixes for the count queries
-          uint32_t v = lane < NS ? sum8(reinterpret_cast<const uint4*>(cnt1)[lane]) : 0u;
-          uint32_t incl = wave_incl_scan(v);
-          pre_s[lane] = incl - v;
+          // word-pair prefixes of the current sieve state for the count queries
+          build_pair_prefix(bm, pre2, W, lane);
           wave_lds_sync();
           int M = 0;
           uint32_t k = 0;
@@ -588,7 +582,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               sg = 1;
             }
             uint64_t n = div_exact(xd, xlo, m);
-            uint32_t cnt = seg_count(bm, cnt1, pre_s, (uint32_t) (n - low));
+            uint32_t cnt = seg_count(bm, pre2, (uint32_t) (n - low));
             S += sg * (long long) cnt;
             M += sg;
             n_leaves++;
@@ -621,11 +615,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
                 const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
                 const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
                 n_hits++;
-                if (old & mask) {
+                if (old & mask)
                   removed = 1;
-                  const uint32_t g = B >> 6;
-                  atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
-                }
               }
             } else {
               // dense prime: lanes split the 8 residue classes of q mod 30 and
@@ -643,11 +634,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
                   uint32_t B = byte0 + t * p;
                   uint32_t mask = 1u << (((B & 3) << 3) | bib);
                   uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-                  if (old & mask) {
+                  if (old & mask)
                     removed++;
-                    uint32_t g = B >> 6;
-                    atomicSub(&cnt1[g >> 1], 1u << ((g & 1) << 4));
-                  }
                 }
                 if (j0 < n)
                   n_hits += (n - j0 + 7) / 8;

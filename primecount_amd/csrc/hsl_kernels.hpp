@@ -95,7 +95,7 @@ struct CombineCol {
 inline uint32_t hsl_lds_bytes(uint32_t W)
 {
   return 8 * W            // segment bitmap
-       + W / 4            // group counters (16-bit, 8 words per group)
+       + 2 * W            // word-pair prefix counts (32-bit, W / 2 pairs)
        + 480 * 2;         // numbers coprime to 2310 in [0, 2310)
 }
 
