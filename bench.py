@@ -144,7 +144,7 @@ def main():
     except Exception:
         sync = lambda: None
 
-    kernel_s, combine_s, wall = [], [], []
+    kernel_s, combine_s, plan_s, wall = [], [], [], []
     sync()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -154,6 +154,7 @@ def main():
         st = pc.last_stats()
         kernel_s.append(st["seconds_kernel"])
         combine_s.append(st["seconds_combine"])
+        plan_s.append(st["seconds_plan"])
         if want is not None and got != want:
             raise SystemExit("step %d result %d != reference %d" % (i, got, want))
     sync()
@@ -201,6 +202,7 @@ def main():
         "bit_exact": want is not None,
         "kernel_ms": round(1000 * k_avg, 3),
         "combine_ms": round(1000 * sum(combine_s) / len(combine_s), 3),
+        "host_plan_ms": round(1000 * sum(plan_s) / len(plan_s), 3),
         "table_build_s_first_call": round(first["seconds_tables"], 3),
         "end_to_end_s_first_call": round(first["seconds_total"], 3),
         "work": {"leaves": st["leaves"], "hits": st["hits"], "factor_entries": st["factor_entries"],

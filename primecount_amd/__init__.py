@@ -95,6 +95,7 @@ def load():
         "pcg_D_batch": ([PI128, ctypes.c_size_t, PI128], ctypes.c_int),
         "pcg_set_segment_words": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
+        "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
     }
@@ -210,6 +211,11 @@ def set_alpha(a):
 
 def set_segment_words(w):
     _check(load().pcg_set_segment_words(int(w)))
+
+
+def set_pass_slots(n):
+    """(segment, prime) slots per kernel pass; 0 restores the default."""
+    _check(load().pcg_set_pass_slots(ctypes.c_uint64(int(n))))
 
 
 def set_host_threads(t):

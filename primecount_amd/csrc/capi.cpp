@@ -196,6 +196,15 @@ int pcg_set_segment_words(uint32_t words)
   });
 }
 
+int pcg_set_pass_slots(uint64_t slots)
+{
+  return guarded([&] {
+    if (slots != 0 && slots < 64)
+      throw pcg_error("pass slots must be 0 or >= 64");
+    engine_config().max_slots_per_pass = slots;
+  });
+}
+
 int pcg_set_host_threads(int threads)
 {
   return guarded([&] {

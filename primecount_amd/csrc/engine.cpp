@@ -72,14 +72,55 @@ struct DevBuf {
   }
 };
 
+/// Page-locked host staging memory (asynchronous uploads).
+struct HostBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+  void* ensure(size_t bytes)
+  {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes > cap) {
+      if (p) (void) hipHostFree(p);
+      p = nullptr;
+      cap = 0;
+      hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        (void) hipGetLastError();
+        throw pcg_error("pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+      }
+      cap = bytes;
+    }
+    return p;
+  }
+  void release()
+  {
+    if (p) (void) hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 struct Device {
   bool ready = false;
   int dev = 0;
   int cus = 0;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {};
+  std::vector<hipEvent_t> ev;         // timing events, 3 per pass
+  hipEvent_t staged[2] = {};          // staging slot s may be overwritten
   DevBuf queries, items, order, binfo, primes, factor, pi_bits, pi_cnt;
   DevBuf tc, tm, s_out, counters, queue, cols, sc, sm, smp, out, carry;
+  HostBuf stage[2];                   // per pass: items, order, columns
+  HostBuf s_host;                     // per item in-segment sums
+
+  hipEvent_t event(size_t i)
+  {
+    while (ev.size() <= i) {
+      hipEvent_t e;
+      PCG_HIP(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+    return ev[i];
+  }
 
   void init()
   {
@@ -96,7 +137,7 @@ struct Device {
       throw pcg_error(std::string("unsupported device ") + prop.gcnArchName + " (need gfx950)");
     cus = prop.multiProcessorCount;
     PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto& e : ev) PCG_HIP(hipEventCreate(&e));
+    for (auto& e : staged) PCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     PCG_HIP(hsl_upload_constants());
     ready = true;
   }
@@ -105,6 +146,8 @@ struct Device {
   {
     for (DevBuf* b : {&queries, &items, &order, &binfo, &primes, &factor, &pi_bits, &pi_cnt,
                       &tc, &tm, &s_out, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry})
+      b->release();
+    for (HostBuf* b : {&stage[0], &stage[1], &s_host})
       b->release();
   }
 };
@@ -254,19 +297,23 @@ void plan_segments(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint3
 {
   const uint64_t SEG = 240ull * W;
   const uint64_t nseg = (qp.qd.limit + SEG - 1) / SEG;
-  const uint64_t nb = qp.qd.b_last - qp.qd.b_first + 1;
-  const uint64_t p_last = P.primes[qp.qd.b_last];
+  const uint32_t b_first = qp.qd.b_first, b_last = qp.qd.b_last;
+  // Segment s (low = max(1, s * SEG)) reaches b iff p_b <= isqrt(x / low),
+  // i.e. iff low <= x / p_b^2.  smax[b] = last segment that reaches b; it
+  // does not increase with b.
+  std::vector<int64_t> smax(b_last - b_first + 1);
+  for (uint32_t b = b_first; b <= b_last; b++) {
+    const uint64_t p = P.primes[b];
+    const u128 L = (u128) q.x / ((u128) p * p);
+    smax[b - b_first] = L < 1 ? -1 : (int64_t) std::min<u128>(L / SEG, nseg);
+  }
   qp.nbs.clear();
   qp.nbs.reserve(nseg);
-  uint64_t bb = qp.qd.b_last;                 // walks down as low grows
+  int64_t hi = (int64_t) b_last;               // largest b the segment reaches
   for (uint64_t s = 0; s < nseg; s++) {
-    uint64_t low1 = std::max<uint64_t>(1, s * SEG);
-    uint64_t r = isqrt((u128) q.x / low1);
-    if (r < p_last)
-      while (bb > 0 && P.primes[bb] > r) bb--;
-    uint64_t e = bb >= qp.qd.b_first ? std::min<uint64_t>(nb, bb - qp.qd.b_first + 1) : 0;
-    if (e == 0) break;
-    qp.nbs.push_back((uint32_t) e);
+    while (hi >= (int64_t) b_first && smax[hi - b_first] < (int64_t) s) hi--;
+    if (hi < (int64_t) b_first) break;
+    qp.nbs.push_back((uint32_t) (hi - b_first + 1));
   }
 }
 
@@ -446,20 +493,58 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   PCG_HIP(hsl_launch_fill_zero(D.out.p, (size_t) ncols_total * 16, s));
   PCG_HIP(hsl_launch_fill_zero(D.counters.p, 64, s));
 
-  std::vector<long long> s_item(all.size(), 0);
-  double t_plan = 0, ms_kernel = 0, ms_combine = 0;
+  double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
   uint64_t max_slots = engine_config().max_slots_per_pass;
   if (max_slots == 0) max_slots = kMaxSlotsPerPass;
 
-  for (size_t pass_begin = 0; pass_begin < all.size();) {
-    double tp = now();
-    // this pass: items [pass_begin, pass_end)
-    size_t pass_end = pass_begin;
+  // passes: consecutive items with at most max_slots slots (at least one item)
+  std::vector<size_t> pass_at{0};
+  uint64_t max_pass_slots = 0, max_pass_items = 0;
+  {
     uint64_t slots = 0;
-    while (pass_end < all.size() && (pass_end == pass_begin || slots + all[pass_end].nbs <= max_slots))
-      slots += all[pass_end++].nbs;
+    for (size_t i = 0; i < all.size(); i++) {
+      if (i > pass_at.back() && slots + all[i].nbs > max_slots) {
+        max_pass_slots = std::max(max_pass_slots, slots);
+        max_pass_items = std::max<uint64_t>(max_pass_items, i - pass_at.back());
+        pass_at.push_back(i);
+        slots = 0;
+      }
+      slots += all[i].nbs;
+    }
+    max_pass_slots = std::max(max_pass_slots, slots);
+    max_pass_items = std::max<uint64_t>(max_pass_items, all.size() - pass_at.back());
+    pass_at.push_back(all.size());
+  }
+  const size_t npass = pass_at.size() - 1;
+
+  // Device buffers shared by the passes (the stream orders their reuse);
+  // host staging alternates between two pinned slots so that the host
+  // prepares pass i + 1 while the device runs pass i.
+  const size_t items_bytes = (size_t) max_pass_items * sizeof(SegItem);
+  const size_t order_bytes = (size_t) max_pass_items * 4;
+  const size_t cols_bytes = (size_t) ncols_total * sizeof(CombineCol);
+  D.items.ensure(items_bytes);
+  D.order.ensure(order_bytes);
+  D.cols.ensure(cols_bytes);
+  D.tc.ensure(max_pass_slots * 4);
+  D.tm.ensure(max_pass_slots * 4);
+  D.s_out.ensure(all.size() * 8);
+  D.queue.ensure(npass * 4);
+  PCG_HIP(hsl_launch_fill_zero(D.queue.p, npass * 4, s));
+  for (auto& hb : D.stage) hb.ensure(items_bytes + order_bytes + cols_bytes);
+  long long* s_item = (long long*) D.s_host.ensure(all.size() * 8);
+
+  for (size_t pi = 0; pi < npass; pi++) {
+    double tp = now();
+    const size_t pass_begin = pass_at[pi], pass_end = pass_at[pi + 1];
     const uint32_t nitems = (uint32_t) (pass_end - pass_begin);
-    std::vector<SegItem> items(nitems);
+    const int slot = (int) (pi & 1);
+    if (pi >= 2) PCG_HIP(hipEventSynchronize(D.staged[slot]));
+    char* stage = (char*) D.stage[slot].p;
+    SegItem* items = (SegItem*) stage;
+    uint32_t* order = (uint32_t*) (stage + items_bytes);
+    CombineCol* cols = (CombineCol*) (stage + items_bytes + order_bytes);
+
     uint64_t off = 0;
     for (uint32_t i = 0; i < nitems; i++) {
       const ItemRef& r = all[pass_begin + i];
@@ -471,43 +556,40 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
       it.t_off = off;
       off += r.nbs;
     }
+    const uint64_t slots = off;
     // schedule: segments with the most slots first (one query: natural order)
-    std::vector<uint32_t> order(nitems);
-    std::iota(order.begin(), order.end(), 0);
+    std::iota(order, order + nitems, 0u);
     if (qdesc.size() > 1)
-      std::stable_sort(order.begin(), order.end(),
+      std::stable_sort(order, order + nitems,
                        [&](uint32_t a, uint32_t b) { return items[a].nbs > items[b].nbs; });
     // combine columns of this pass
-    std::vector<CombineCol> cols;
+    uint32_t ncols = 0;
     uint32_t max_n = 1;
     for (uint32_t i = 0; i < nitems;) {
-      uint32_t slot = items[i].query, j = i;
-      while (j < nitems && items[j].query == slot) j++;
+      uint32_t slot_q = items[i].query, j = i;
+      while (j < nitems && items[j].query == slot_q) j++;
       // items [i, j) are consecutive segments of one query; nbs non-increasing
       uint32_t n = j - i;
       for (uint32_t br = 0; br < items[i].nbs; br++) {
         while (n > 0 && items[i + n - 1].nbs <= br) n--;
         if (n == 0) break;
-        cols.push_back(CombineCol{br, i, n, colbase[slot] + br});
+        cols[ncols++] = CombineCol{br, i, n, colbase[slot_q] + br};
         max_n = std::max(max_n, n);
       }
       i = j;
     }
     uint32_t slice_len = std::max<uint32_t>(1, (max_n + 127) / 128);
     uint32_t nslices = (max_n + slice_len - 1) / slice_len;
-    const uint32_t ncols = (uint32_t) cols.size();
 
-    upload(D.items, items.data(), items.size() * sizeof(SegItem));
-    upload(D.order, order.data(), order.size() * 4);
-    upload(D.cols, cols.data(), cols.size() * sizeof(CombineCol));
-    D.tc.ensure(slots * 4);
-    D.tm.ensure(slots * 4);
-    D.s_out.ensure((size_t) nitems * 8);
-    D.queue.ensure(64);
+    PCG_HIP(hipMemcpyAsync(D.items.p, items, (size_t) nitems * sizeof(SegItem), hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(D.order.p, order, (size_t) nitems * 4, hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(D.cols.p, cols, (size_t) ncols * sizeof(CombineCol), hipMemcpyHostToDevice, s));
+    PCG_HIP(hipEventRecord(D.staged[slot], s));
+    if ((size_t) nslices * ncols * 16 > D.smp.cap)
+      PCG_HIP(hipStreamSynchronize(s));          // earlier passes may still read them
     D.sc.ensure((size_t) nslices * ncols * 8);
     D.sm.ensure((size_t) nslices * ncols * 8);
     D.smp.ensure((size_t) nslices * ncols * 16);
-    PCG_HIP(hsl_launch_fill_zero(D.queue.p, 64, s));
     t_plan += now() - tp;
 
     HslArgs a{};
@@ -521,37 +603,37 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
     a.T_C = (uint32_t*) D.tc.p;
     a.T_M = (int32_t*) D.tm.p;
-    a.S_out = (long long*) D.s_out.p;
+    a.S_out = (long long*) D.s_out.p + pass_begin;
     a.counters = (unsigned long long*) D.counters.p;
-    a.queue = (unsigned int*) D.queue.p;
+    a.queue = (unsigned int*) D.queue.p + pi;
     a.nitems = nitems;
     a.W = W;
+    (void) slots;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, nitems);
 
-    PCG_HIP(hipEventRecord(D.ev[0], s));
+    PCG_HIP(hipEventRecord(D.event(3 * pi), s));
     PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
-    PCG_HIP(hipEventRecord(D.ev[1], s));
+    PCG_HIP(hipEventRecord(D.event(3 * pi + 1), s));
     PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const SegItem*) D.items.p,
                                (const uint32_t*) D.tc.p, (const int32_t*) D.tm.p, slice_len,
                                nslices, (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p,
                                (long long*) D.carry.p, D.out.p, s));
-    PCG_HIP(hipEventRecord(D.ev[2], s));
-    PCG_HIP(hipMemcpyAsync(s_item.data() + pass_begin, D.s_out.p, (size_t) nitems * 8,
-                           hipMemcpyDeviceToHost, s));
-    PCG_HIP(hipStreamSynchronize(s));
-    float mk = 0, mc = 0;
-    PCG_HIP(hipEventElapsedTime(&mk, D.ev[0], D.ev[1]));
-    PCG_HIP(hipEventElapsedTime(&mc, D.ev[1], D.ev[2]));
-    ms_kernel += mk;
-    ms_combine += mc;
-    pass_begin = pass_end;
+    PCG_HIP(hipEventRecord(D.event(3 * pi + 2), s));
   }
+  PCG_HIP(hipMemcpyAsync(s_item, D.s_out.p, all.size() * 8, hipMemcpyDeviceToHost, s));
 
   std::vector<i128> col_out(ncols_total);
   uint64_t counters[4] = {0, 0, 0, 0};
   PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, D.counters.p, 32, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
+  for (size_t pi = 0; pi < npass; pi++) {
+    float mk = 0, mc = 0;
+    PCG_HIP(hipEventElapsedTime(&mk, D.ev[3 * pi], D.ev[3 * pi + 1]));
+    PCG_HIP(hipEventElapsedTime(&mc, D.ev[3 * pi + 1], D.ev[3 * pi + 2]));
+    ms_kernel += mk;
+    ms_combine += mc;
+  }
 
   std::vector<i128> per_slot(qdesc.size(), 0);
   for (size_t i = 0; i < all.size(); i++) per_slot[all[i].slot] += s_item[i];

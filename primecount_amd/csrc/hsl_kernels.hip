@@ -464,54 +464,109 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       if (nwin < 64)
         done = true;
       const unsigned long long in_win = nwin >= 64 ? ~0ull : ((1ull << nwin) - 1);
-      // Primes [0, seq_end) of the window are processed one at a time; from
-      // seq_end on (the condition above is monotone in p) the cross-offs of
-      // consecutive primes run together, and only a prime with factor
-      // entries forces the pending ones to be applied before its count.
+      // par (monotone in p) holds for the primes [seq_end, nwin) of the window
       const unsigned long long pbal = __ballot(par) & in_win;
       const uint32_t seq_end = pbal ? (uint32_t) __ffsll((long long) pbal) - 1 : nwin;
-      const bool in_par = lane >= seq_end && lane < nwin;
-      const unsigned long long lpar = __ballot(in_par && r_hi > r_lo);
-      // lanes per prime in the parallel cross-off: a power of two >= the
-      // largest hit count of the window's parallel primes (at most 64)
-      const uint32_t hmax = readlane_u32(wave_incl_max(in_par ? r_h : 0u), 63);
-      const uint32_t lg = hmax <= 1 ? 0u : (hmax >= 64 ? 6u : 32u - (uint32_t) __clz(hmax - 1));
+      // primes with factor entries; the cross-offs between two of them are
+      // applied together just before the count of the second
+      const unsigned long long lmask = __ballot(r_hi > r_lo) & in_win;
+      const uint32_t h_win = lane < nwin ? r_h : 0u;
       prm.w = r_h;
       wp[lane] = prm;
       w_rem[lane] = 0;
       w_m[lane] = 0;
       wave_lds_sync();
 
-      // Cross off primes [r0, r1) of the parallel part together: 2^lg lanes
-      // per prime, lane k of a prime's group takes its hits k, k + 2^lg, ...
-      // (a hit is the k-th cofactor coprime to 30).  A number removed for the
-      // first time is charged to its least prime factor: p, or the cofactor q
-      // when 1 < q < p (q is then a prime of this run).
+      // Cross off primes [r0, r1) of the window, in two parts:
+      //  * primes up to the last one with more than 32 hits go one at a time,
+      //    lanes split the 8 residue classes of q mod 30 and stride 8
+      //    multiples apart inside a class (p bytes per multiple);
+      //  * the rest together, 2^lg lanes per prime (2^lg >= hits), lane k of a
+      //    prime's group taking its k-th cofactor coprime to 30.  If the part
+      //    holds a prime below seq_end, the atomics of the primes are issued
+      //    one prime group after the other, which keeps the first-time
+      //    removals of the sequential order; otherwise they are issued at once
+      //    and a number removed for the first time is charged to its least
+      //    prime factor: p, or the cofactor q when 1 < q < p (q is then a
+      //    prime of this part).
       auto flush = [&](uint32_t r0, uint32_t r1) {
-        const uint32_t G = 1u << lg;
-        for (uint32_t jb = r0; jb < r1; jb += kBlock >> lg) {
-          const uint32_t jo = jb + (lane >> lg);
-          if (jo < r1) {
-            const uint4 pp = wp[jo];
-            const uint32_t c0 = pp.y >> 25, qm = (pp.y >> 20) & 31, off0 = pp.y & 0xFFFFFu;
-            for (uint32_t kk = lane & (G - 1); kk < pp.w; kk += G) {
-              const uint32_t idx = c0 + kk;
-              const uint32_t delta = 30 * (idx >> 3) +
-                                     (uint32_t) ((kWheelBytes >> (8 * (idx & 7))) & 0xff) - qm;
-              const uint32_t off = off0 + pp.x * delta;
-              const uint32_t B = off / 30;
-              const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
-              const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-              n_hits++;
-              if (old & mask) {
-                uint32_t attr = jo;
-                if (delta < pp.z) {                     // q = q_lo + delta < p
-                  const uint32_t q = pp.x - pp.z + delta;
-                  if (q > 1)
-                    attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
-                }
-                atomicAdd(&w_rem[attr], 1u);
+        if (r0 >= r1)
+          return;
+        const unsigned long long run = (r1 >= 64 ? ~0ull : ((1ull << r1) - 1)) & ~((1ull << r0) - 1);
+        const unsigned long long big = __ballot(h_win > 32) & run;
+        const uint32_t d_end = big ? 64u - (uint32_t) __clzll((long long) big) : r0;
+        for (uint32_t j = r0; j < d_end; j++) {
+          const uint32_t p = readlane_u32(r_p, j);
+          const uint32_t nq = readlane_u32(r_nq, j);
+          uint32_t removed = 0;
+          if (nq > 0 && p >= 7) {
+            const uint32_t qm = readlane_u32(r_qm, j);
+            const uint32_t off0 = readlane_u32(r_off0, j);
+            const uint32_t cls = lane & 7;
+            const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
+            const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
+            if (delta < nq) {
+              const uint32_t n = (nq - 1 - delta) / 30 + 1;
+              const uint32_t off = off0 + p * delta;
+              const uint32_t byte0 = off / 30;
+              const uint32_t bib = wheel_bit(off - byte0 * 30);
+              const uint32_t j0 = lane >> 3;
+              for (uint32_t t = j0; t < n; t += 8) {
+                const uint32_t B = byte0 + t * p;
+                const uint32_t mask = 1u << (((B & 3) << 3) | bib);
+                const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
+                if (old & mask)
+                  removed++;
               }
+              if (j0 < n)
+                n_hits += (n - j0 + 7) / 8;
+            }
+          }
+          if (removed != 0)
+            atomicAdd(&w_rem[j], removed);
+        }
+        if (d_end >= r1)
+          return;
+        const bool ordered = d_end < seq_end;
+        const unsigned long long rest = run & ~((1ull << d_end) - 1);
+        const uint32_t hm = readlane_u32(wave_incl_max((rest >> lane) & 1 ? h_win : 0u), 63);
+        const uint32_t lg = hm <= 1 ? 0u : 32u - (uint32_t) __clz(hm - 1);   // hm <= 32
+        const uint32_t G = 1u << lg;
+        const uint32_t grp = lane >> lg;
+        const uint32_t ngrp = kBlock >> lg;
+        for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
+          const uint32_t jo = jb + grp;
+          const uint4 pp = wp[jo < r1 ? jo : r0];
+          const uint32_t kk = lane & (G - 1);
+          const bool has = jo < r1 && kk < pp.w;
+          const uint32_t idx = (pp.y >> 25) + kk;
+          const uint32_t delta = 30 * (idx >> 3) +
+                                 (uint32_t) ((kWheelBytes >> (8 * (idx & 7))) & 0xff) -
+                                 ((pp.y >> 20) & 31);
+          const uint32_t off = (pp.y & 0xFFFFFu) + pp.x * delta;
+          const uint32_t B = off / 30;
+          const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
+          uint32_t old = 0;
+          if (!ordered) {
+            if (has)
+              old = atomicAnd(&bm[B >> 2], ~mask);
+          } else {
+            for (uint32_t u = 0; u < ngrp; u++) {
+              if (has && grp == u)
+                old = atomicAnd(&bm[B >> 2], ~mask);
+              wave_lds_sync();
+            }
+          }
+          if (has) {
+            n_hits++;
+            if (old & mask) {
+              uint32_t attr = jo;
+              if (!ordered && delta < pp.z) {           // q = q_lo + delta < p
+                const uint32_t q = pp.x - pp.z + delta;
+                if (q > 1)
+                  attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
+              }
+              atomicAdd(&w_rem[attr], 1u);
             }
           }
         }
@@ -522,129 +577,71 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       auto prefetch = [&](uint32_t jj) {
         const uint32_t lo = readlane_u32(r_lo, jj);
         const uint32_t hi = readlane_u32(r_hi, jj);
-        if (hi > lo) {
-          const bool t1 = (readlane_u32(r_T, jj) >> 31) != 0;
+        const bool t1 = (readlane_u32(r_T, jj) >> 31) != 0;
 #pragma unroll
-          for (int k = 0; k < kPf; k++) {
-            uint32_t idx = lo + 1 + lane + k * kBlock;
-            pf[k] = idx <= hi ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
-          }
+        for (int k = 0; k < kPf; k++) {
+          uint32_t idx = lo + 1 + lane + k * kBlock;
+          pf[k] = idx <= hi ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
         }
       };
-      // processing order: every prime below seq_end, then the primes of the
-      // parallel part that have a leaf range
-      auto next_prime = [&](uint32_t c) -> uint32_t {
-        if (c < seq_end)
-          return c;
-        const unsigned long long m = c >= 64 ? 0ull : (lpar & (~0ull << c));
+      auto next_leaf = [&](uint32_t c) -> uint32_t {
+        const unsigned long long m = c >= 64 ? 0ull : (lmask & (~0ull << c));
         return m ? (uint32_t) __ffsll((long long) m) - 1 : 64u;
       };
 
-      uint32_t j = next_prime(0);
+      uint32_t j = next_leaf(0);
       if (j < nwin)
         prefetch(j);
-      uint32_t run = seq_end;           // first prime whose cross-off is pending
+      uint32_t run = 0;                 // first prime whose cross-off is pending
       while (j < nwin) {
-        const uint32_t jn = next_prime(j + 1);
-        if (j >= seq_end) {
-          flush(run, j);
-          run = j;
-        }
+        const uint32_t jn = next_leaf(j + 1);
+        flush(run, j);
+        run = j;
         const uint32_t lo = readlane_u32(r_lo, j);
         const uint32_t hi = readlane_u32(r_hi, j);
-        if (hi > lo) {
-          const uint32_t Tt = readlane_u32(r_T, j);
-          const bool t1 = (Tt >> 31) != 0;
-          const uint32_t T = Tt & 0x7fffffffu;
-          const uint64_t xlo = readlane_u64(r_xlo, j);
-          const double xd = __longlong_as_double(
-              (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
-          // word-pair prefixes of the current sieve state for the count queries
-          build_pair_prefix(bm, pre2, W, lane);
-          wave_lds_sync();
-          int M = 0;
-          uint32_t k = 0;
+        const uint32_t Tt = readlane_u32(r_T, j);
+        const bool t1 = (Tt >> 31) != 0;
+        const uint32_t T = Tt & 0x7fffffffu;
+        const uint64_t xlo = readlane_u64(r_xlo, j);
+        const double xd = __longlong_as_double(
+            (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
+        // word-pair prefixes of the current sieve state for the count queries
+        build_pair_prefix(bm, pre2, W, lane);
+        wave_lds_sync();
+        int M = 0;
+        uint32_t k = 0;
 #pragma nounroll
-          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock, k++) {
-            // entry: prefetched for the first kPf strides, loaded otherwise
-            uint32_t f = k == 0 ? pf[0] : (k == 1 ? pf[1] : (t1 ? (uint32_t) factor[idx] : primes[idx]));
-            uint64_t m;
-            int sg;
-            if (t1) {
-              n_entries++;
-              if (f <= T)
-                continue;
-              uint32_t qq = idx / 480;
-              m = 2310ull * qq + cop[idx - qq * 480];
-              sg = (f & 1) ? 1 : -1;
-            } else {
-              m = f;
-              sg = 1;
-            }
-            uint64_t n = div_exact(xd, xlo, m);
-            uint32_t cnt = seg_count(bm, pre2, (uint32_t) (n - low));
-            S += sg * (long long) cnt;
-            M += sg;
-            n_leaves++;
+        for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock, k++) {
+          // entry: prefetched for the first kPf strides, loaded otherwise
+          uint32_t f = k == 0 ? pf[0] : (k == 1 ? pf[1] : (t1 ? (uint32_t) factor[idx] : primes[idx]));
+          uint64_t m;
+          int sg;
+          if (t1) {
+            n_entries++;
+            if (f <= T)
+              continue;
+            uint32_t qq = idx / 480;
+            m = 2310ull * qq + cop[idx - qq * 480];
+            sg = (f & 1) ? 1 : -1;
+          } else {
+            m = f;
+            sg = 1;
           }
-          if (M != 0)
-            atomicAdd(&w_m[j], M);
+          uint64_t n = div_exact(xd, xlo, m);
+          uint32_t cnt = seg_count(bm, pre2, (uint32_t) (n - low));
+          S += sg * (long long) cnt;
+          M += sg;
+          n_leaves++;
         }
+        if (M != 0)
+          atomicAdd(&w_m[j], M);
         // Every load of this prime has been consumed.  Saying so with an
         // explicit vmcnt(0) lets the compiler's wait analysis see that only
-        // the prefetch below is in flight during the cross-off (otherwise a
+        // the prefetch below is in flight during the cross-offs (otherwise a
         // conservative wait there would expose the prefetch latency).
         __builtin_amdgcn_s_waitcnt(0x0F70);
         if (jn < nwin)
           prefetch(jn);
-
-        if (j < seq_end) {
-          // cross off p_b; the first-time removals are C(s, b) - C(s, b + 1)
-          const uint32_t p = readlane_u32(r_p, j);
-          const uint32_t nq = readlane_u32(r_nq, j);
-          uint32_t removed = 0;
-          if (nq > 0 && p >= 7) {
-            const uint32_t qm = readlane_u32(r_qm, j);
-            const uint32_t off0 = readlane_u32(r_off0, j);
-            if (nq <= kBlock) {
-              // sparse prime: lane l takes cofactor q_lo + l (if coprime to 30)
-              const uint32_t t = (qm + lane) % 30;
-              if (lane < nq && ((kCop30 >> t) & 1)) {
-                const uint32_t off = off0 + p * lane;
-                const uint32_t B = off / 30;
-                const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
-                const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-                n_hits++;
-                if (old & mask)
-                  removed = 1;
-              }
-            } else {
-              // dense prime: lanes split the 8 residue classes of q mod 30 and
-              // stride 8 multiples apart inside a class (p bytes per multiple)
-              const uint32_t cls = lane & 7;
-              const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
-              const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
-              if (delta < nq) {
-                const uint32_t n = (nq - 1 - delta) / 30 + 1;
-                const uint32_t off = off0 + p * delta;
-                const uint32_t byte0 = off / 30;
-                const uint32_t bib = wheel_bit(off - byte0 * 30);
-                const uint32_t j0 = lane >> 3;
-                for (uint32_t t = j0; t < n; t += 8) {
-                  uint32_t B = byte0 + t * p;
-                  uint32_t mask = 1u << (((B & 3) << 3) | bib);
-                  uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-                  if (old & mask)
-                    removed++;
-                }
-                if (j0 < n)
-                  n_hits += (n - j0 + 7) / 8;
-              }
-            }
-          }
-          if (removed != 0)
-            atomicAdd(&w_rem[j], removed);
-        }
         j = jn;
       }
       flush(run, nwin);
@@ -721,10 +718,29 @@ __global__ void hsl_combine_slices(const CombineCol* __restrict__ cols, uint32_t
   uint32_t i1 = min(cc.nitems, i0 + slice_len);
   long long P = 0, Ms = 0;
   __int128 MP = 0;
-  for (uint32_t i = i0; i < i1; i++) {
-    uint64_t off = items[cc.item_first + i].t_off + cc.b_rel;
-    long long tc = T_C[off];
-    long long tm = T_M[off];
+  uint32_t i = i0;
+  // 8 items per step: the 16 loads of a step are independent and in flight
+  // together
+  for (; i + 8 <= i1; i += 8) {
+    uint32_t tc[8];
+    int tm[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const uint64_t off = items[cc.item_first + i + u].t_off + cc.b_rel;
+      tc[u] = T_C[off];
+      tm[u] = T_M[off];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      MP += (__int128) tm[u] * P;
+      P += tc[u];
+      Ms += tm[u];
+    }
+  }
+  for (; i < i1; i++) {
+    const uint64_t off = items[cc.item_first + i].t_off + cc.b_rel;
+    const long long tc = T_C[off];
+    const long long tm = T_M[off];
     MP += (__int128) tm * P;
     P += tc;
     Ms += tm;

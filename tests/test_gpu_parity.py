@@ -138,3 +138,37 @@ def test_int128_entry_near_2_63(pc):
         if x < 2 ** 63:
             assert pc.D(x, p["y"], p["z"], p["k"]) == got
         assert got > 0
+
+
+def test_batch_matches_reference(pc):
+    # 256 seeded queries x = 10^(12 + 6u) in one launch, against the values the
+    # reference build printed for each x
+    fx = golden("batch_D.json")
+    xs = [int(r["x"]) for r in fx["rows"]]
+    want = [int(r["D"]) for r in fx["rows"]]
+    log("D batch of %d queries" % len(xs))
+    assert pc.D_batch(xs) == want
+    # order and duplicates do not matter
+    sub = xs[::-7] + xs[:3]
+    assert pc.D_batch(sub) == [want[xs.index(v)] for v in sub]
+
+
+@pytest.mark.parametrize("slots", [1 << 14, 1 << 20])
+def test_pass_size_invariance(pc, slots):
+    # many kernel passes (carry kept on the device between them) give the
+    # same sums as one pass, for single queries and for a batch
+    rows = {int(v["x"]): v for v in golden("configs.json")["D"]}
+    fx = [r for r in golden("batch_D.json")["rows"] if int(r["x"]) <= 10 ** 15][:24]
+    xs = [int(r["x"]) for r in fx]
+    want = [int(r["D"]) for r in fx]
+    try:
+        pc.set_pass_slots(slots)
+        log("pass slots %d: D default 1e13 and 1e15" % slots)
+        for e in (13, 15):
+            assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+        s2 = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}[10 ** 15]
+        assert pc.S2_hard_default(10 ** 15) == int(s2["value"])
+        log("pass slots %d: batch of %d" % (slots, len(xs)))
+        assert pc.D_batch(xs) == want
+    finally:
+        pc.set_pass_slots(0)
