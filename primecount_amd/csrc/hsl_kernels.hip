@@ -113,6 +113,26 @@ __device__ __forceinline__ uint64_t div_exact(double xd, uint64_t xlo, uint64_t 
   return q;
 }
 
+/// u64 -> f32 within 2^-22 relative error (two conversions and one fma).
+__device__ __forceinline__ float u64_to_f32(uint64_t v)
+{
+  return __builtin_fmaf((float) (uint32_t) (v >> 32), 4294967296.0f, (float) (uint32_t) v);
+}
+
+/// floor(xp / m) - low for a leaf whose quotient lies in [low, low + 2^20)
+/// (the leaf ranges guarantee low <= xp / m < high).  r = xp - low * m is
+/// exact modulo 2^64 from the low word of xp because 0 <= r < m * 2^20 <
+/// 2^60; the f32 estimate of r / m has relative error < 2^-21, so it is
+/// within 1/2 of r / m < 2^20 and its truncation within one of the floor,
+/// which the integer residual then fixes.
+__device__ __forceinline__ uint32_t leaf_stop(uint64_t xlo, uint64_t low, uint64_t m)
+{
+  const uint64_t r = xlo - low * m;
+  uint32_t s = (uint32_t) (u64_to_f32(r) * __builtin_amdgcn_rcpf(u64_to_f32(m)));
+  const int64_t t = (int64_t) (r - (uint64_t) s * m);
+  return t < 0 ? s - 1 : ((uint64_t) t >= m ? s + 1 : s);
+}
+
 /// min(floor(xp / d), cap) for cap < 2^48.
 __device__ __forceinline__ uint64_t div_cap(double xd, uint64_t xlo, uint64_t d, uint64_t cap)
 {
@@ -413,7 +433,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       uint32_t r_lo = 0, r_hi = 0, r_p = 0, r_T = 0;
       uint32_t r_nq = 0, r_qm = 0, r_off0 = 0, r_h = 0;
       uint64_t r_xlo = 0;
-      double r_xd = 0;
       bool brk = true, par = false;
       uint4 prm = make_uint4(0, 0, 0, 0);
       if (bb <= b_stop) {
@@ -436,7 +455,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         r_p = bi.p;
         r_T = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
         r_xlo = bi.xp_lo;
-        r_xd = bi.xp_d;
         // cofactors q of p with p*q in (low, high): q_lo .. q_lo + nq - 1
         const uint64_t q_lo = div_exact((double) low, low, bi.p) + 1;
         const uint64_t q_hi = div_exact((double) (high - 1), high - 1, bi.p);
@@ -522,8 +540,11 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
                 n_hits += (n - j0 + 7) / 8;
             }
           }
-          if (removed != 0)
-            atomicAdd(&w_rem[j], removed);
+          // wave total with DPP (a divergent atomic on one address would be
+          // expanded into a loop over the active lanes)
+          const uint32_t rtot = readlane_u32(wave_incl_scan(removed), 63);
+          if (lane == 0)
+            w_rem[j] = rtot;
         }
         if (d_end >= r1)
           return;
@@ -603,8 +624,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         const bool t1 = (Tt >> 31) != 0;
         const uint32_t T = Tt & 0x7fffffffu;
         const uint64_t xlo = readlane_u64(r_xlo, j);
-        const double xd = __longlong_as_double(
-            (long long) readlane_u64((uint64_t) __double_as_longlong(r_xd), j));
         // word-pair prefixes of the current sieve state for the count queries
         build_pair_prefix(bm, pre2, W, lane);
         wave_lds_sync();
@@ -627,14 +646,14 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             m = f;
             sg = 1;
           }
-          uint64_t n = div_exact(xd, xlo, m);
-          uint32_t cnt = seg_count(bm, pre2, (uint32_t) (n - low));
+          const uint32_t cnt = seg_count(bm, pre2, leaf_stop(xlo, low, m));
           S += sg * (long long) cnt;
           M += sg;
           n_leaves++;
         }
-        if (M != 0)
-          atomicAdd(&w_m[j], M);
+        const uint32_t mtot = readlane_u32(wave_incl_scan((uint32_t) M), 63);
+        if (lane == 0)
+          w_m[j] = (int) mtot;
         // Every load of this prime has been consumed.  Saying so with an
         // explicit vmcnt(0) lets the compiler's wait analysis see that only
         // the prefetch below is in flight during the cross-offs (otherwise a
