@@ -201,8 +201,8 @@ void validate(const HslQuery& q)
       throw pcg_error("D(x, y, z, k) needs z >= y");
     if (q.x / q.z > ((i128) 1 << 50))
       throw pcg_error("D(x, y, z, k): x / z must be < 2^50");
-    if (q.z > ((int64_t) 1 << 40))
-      throw pcg_error("D(x, y, z, k): z must be <= 2^40");
+    if (q.z >= ((int64_t) 1 << 32))
+      throw pcg_error("D(x, y, z, k): z must be < 2^32");
   } else {
     if ((i128) q.z > ((i128) 1 << 50))
       throw pcg_error("S2_hard(x, y, z, c): z must be < 2^50");

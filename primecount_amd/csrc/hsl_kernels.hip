@@ -42,7 +42,6 @@ __constant__ uint16_t c_cop[480];         // coprimes to 2310 in [0, 2310)
 
 namespace {
 
-constexpr int kPf = 2;              // factor entries per thread prefetched one prime ahead
 
 /// Nibble table: number of wheel residues {1,7,11,...,29} that are <= r.
 constexpr uint64_t le_nibbles(int first)
@@ -113,22 +112,18 @@ __device__ __forceinline__ uint64_t div_exact(double xd, uint64_t xlo, uint64_t 
   return q;
 }
 
-/// u64 -> f32 within 2^-22 relative error (two conversions and one fma).
-__device__ __forceinline__ float u64_to_f32(uint64_t v)
-{
-  return __builtin_fmaf((float) (uint32_t) (v >> 32), 4294967296.0f, (float) (uint32_t) v);
-}
-
 /// floor(xp / m) - low for a leaf whose quotient lies in [low, low + 2^20)
-/// (the leaf ranges guarantee low <= xp / m < high).  r = xp - low * m is
-/// exact modulo 2^64 from the low word of xp because 0 <= r < m * 2^20 <
-/// 2^60; the f32 estimate of r / m has relative error < 2^-21, so it is
-/// within 1/2 of r / m < 2^20 and its truncation within one of the floor,
-/// which the integer residual then fixes.
-__device__ __forceinline__ uint32_t leaf_stop(uint64_t xlo, uint64_t low, uint64_t m)
+/// (the leaf ranges guarantee low <= xp / m < high) and m < 2^32.
+/// r = xp - low * m is exact modulo 2^64 from the low word of xp because
+/// 0 <= r < m * 2^20 < 2^52; r / m is estimated in double with the hardware
+/// reciprocal (any relative error below 2^-21 keeps the estimate within 1/2
+/// of r / m < 2^20, so its truncation is within one of the floor), and the
+/// integer residual r - s * m fixes the last unit.
+__device__ __forceinline__ uint32_t leaf_stop(uint64_t xlo, uint64_t low, uint32_t m)
 {
   const uint64_t r = xlo - low * m;
-  uint32_t s = (uint32_t) (u64_to_f32(r) * __builtin_amdgcn_rcpf(u64_to_f32(m)));
+  const double md = (double) m;
+  uint32_t s = (uint32_t) ((double) r * __builtin_amdgcn_rcp(md));
   const int64_t t = (int64_t) (r - (uint64_t) s * m);
   return t < 0 ? s - 1 : ((uint64_t) t >= m ? s + 1 : s);
 }
@@ -258,20 +253,20 @@ __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* 
 /// Unsieved numbers in [low, low + stop] of the segment (count_stop.hpp:
 /// the count is inclusive and relative to low): the prefix of the word pair,
 /// the whole first word of the pair when stop lies in its second word, and
-/// the masked word of stop.
+/// the low lut[stop % 240] bits of the word of stop (lut[r] = sieve bits of
+/// the offsets <= r in a word).
 __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t* pre2,
-                                              uint32_t stop)
+                                              const uint8_t* lut, uint32_t stop)
 {
   const uint32_t w = stop / 240;
   const uint32_t r = stop - w * 240;
   const uint32_t g = w >> 1;
   const uint4 q = reinterpret_cast<const uint4*>(bm)[g];
-  const uint64_t m = le_mask(r);
+  const uint32_t n = lut[r];
   const bool odd = (w & 1) != 0;
-  uint32_t c = pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
-  const uint32_t lo = odd ? q.z : q.x;
-  const uint32_t hi = odd ? q.w : q.y;
-  return c + __popc(lo & (uint32_t) m) + __popc(hi & (uint32_t) (m >> 32));
+  const uint32_t c = pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
+  const uint64_t word = odd ? (((uint64_t) q.w << 32) | q.z) : (((uint64_t) q.y << 32) | q.x);
+  return c + (n ? (uint32_t) __popcll(word << (64 - n)) : 0u);
 }
 
 /// Inclusive prefix maximum over the 64 lanes of a wave (same DPP pattern as
@@ -367,6 +362,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
   uint16_t* cop = reinterpret_cast<uint16_t*>(pre2 + W / 2);  // 480 u16
+  uint8_t* lut = reinterpret_cast<uint8_t*>(cop + 480);       // 240 u8
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
   __shared__ int w_m[64];                                     // per window prime: leaf signs
   __shared__ uint4 wp[64];                                    // per window prime: cross-off data
@@ -374,9 +370,12 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   const uint32_t lane = threadIdx.x;
   for (uint32_t i = lane; i < 480; i += kBlock)
     cop[i] = c_cop[i];
+  for (uint32_t r = lane; r < 240; r += kBlock)
+    lut[r] = (uint8_t) (8 * (r / 30) + le_count(r % 30));
 
   const uint64_t SEG = 240ull * W;
-  unsigned long long n_leaves = 0, n_hits = 0, n_entries = 0, n_slots = 0;
+  unsigned long long n_hits = 0, n_slots = 0;
+  unsigned long long u_leaves = 0, u_entries = 0;   // wave-uniform
 
   for (;;) {
     uint32_t qi = 0;
@@ -593,75 +592,58 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         }
       };
 
-      // factor entries (or primes) of the next prime with a leaf range
-      uint32_t pf[kPf];
-      auto prefetch = [&](uint32_t jj) {
-        const uint32_t lo = readlane_u32(r_lo, jj);
-        const uint32_t hi = readlane_u32(r_hi, jj);
-        const bool t1 = (readlane_u32(r_T, jj) >> 31) != 0;
-#pragma unroll
-        for (int k = 0; k < kPf; k++) {
-          uint32_t idx = lo + 1 + lane + k * kBlock;
-          pf[k] = idx <= hi ? (t1 ? (uint32_t) factor[idx] : primes[idx]) : 0u;
-        }
-      };
       auto next_leaf = [&](uint32_t c) -> uint32_t {
         const unsigned long long m = c >= 64 ? 0ull : (lmask & (~0ull << c));
         return m ? (uint32_t) __ffsll((long long) m) - 1 : 64u;
       };
 
-      uint32_t j = next_leaf(0);
-      if (j < nwin)
-        prefetch(j);
       uint32_t run = 0;                 // first prime whose cross-off is pending
-      while (j < nwin) {
-        const uint32_t jn = next_leaf(j + 1);
+      for (uint32_t j = next_leaf(0); j < nwin; j = next_leaf(j + 1)) {
         flush(run, j);
         run = j;
         const uint32_t lo = readlane_u32(r_lo, j);
         const uint32_t hi = readlane_u32(r_hi, j);
         const uint32_t Tt = readlane_u32(r_T, j);
-        const bool t1 = (Tt >> 31) != 0;
         const uint32_t T = Tt & 0x7fffffffu;
         const uint64_t xlo = readlane_u64(r_xlo, j);
         // word-pair prefixes of the current sieve state for the count queries
         build_pair_prefix(bm, pre2, W, lane);
         wave_lds_sync();
         int M = 0;
-        uint32_t k = 0;
+        if (Tt >> 31) {
+          // factor-table entries: leaves -mu(m) phi(xp / m, b - 1) for the m
+          // whose entry exceeds T (src/gourdon/D.cpp:148-188)
+          u_entries += hi - lo;
 #pragma nounroll
-        for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock, k++) {
-          // entry: prefetched for the first kPf strides, loaded otherwise
-          uint32_t f = k == 0 ? pf[0] : (k == 1 ? pf[1] : (t1 ? (uint32_t) factor[idx] : primes[idx]));
-          uint64_t m;
-          int sg;
-          if (t1) {
-            n_entries++;
-            if (f <= T)
-              continue;
-            uint32_t qq = idx / 480;
-            m = 2310ull * qq + cop[idx - qq * 480];
-            sg = (f & 1) ? 1 : -1;
-          } else {
-            m = f;
-            sg = 1;
+          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock) {
+            const uint32_t f = factor[idx];
+            const bool leaf = f > T;
+            u_leaves += __popcll(__ballot(leaf));
+            if (leaf) {
+              const uint32_t qq = idx / 480;
+              const uint32_t m = 2310 * qq + cop[idx - qq * 480];
+              const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
+              if (f & 1) {
+                S += cnt;
+                M++;
+              } else {
+                S -= cnt;
+                M--;
+              }
+            }
           }
-          const uint32_t cnt = seg_count(bm, pre2, leaf_stop(xlo, low, m));
-          S += sg * (long long) cnt;
-          M += sg;
-          n_leaves++;
+        } else {
+          // primes q: leaves +phi(xp / q, b - 1) (src/gourdon/D.cpp:279-302)
+          u_leaves += hi - lo;
+#pragma nounroll
+          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock) {
+            S += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
+            M++;
+          }
         }
         const uint32_t mtot = readlane_u32(wave_incl_scan((uint32_t) M), 63);
         if (lane == 0)
           w_m[j] = (int) mtot;
-        // Every load of this prime has been consumed.  Saying so with an
-        // explicit vmcnt(0) lets the compiler's wait analysis see that only
-        // the prefetch below is in flight during the cross-offs (otherwise a
-        // conservative wait there would expose the prefetch latency).
-        __builtin_amdgcn_s_waitcnt(0x0F70);
-        if (jn < nwin)
-          prefetch(jn);
-        j = jn;
       }
       flush(run, nwin);
       wave_lds_sync();
@@ -689,13 +671,13 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       S_out[item] = tot;
   }
 
-  long long l = wave_sum_i64((long long) n_leaves);
+  // u_leaves and u_entries are complete in lane 0 (the lowest entry index of
+  // every leaf-loop step belongs to lane 0)
   long long h = wave_sum_i64((long long) n_hits);
-  long long e = wave_sum_i64((long long) n_entries);
   if (lane == 0) {
-    atomicAdd(&counters[0], (unsigned long long) l);
+    atomicAdd(&counters[0], u_leaves);
     atomicAdd(&counters[1], (unsigned long long) h);
-    atomicAdd(&counters[2], (unsigned long long) e);
+    atomicAdd(&counters[2], u_entries);
     atomicAdd(&counters[3], n_slots);
   }
 }
