@@ -205,8 +205,9 @@ def main():
         "host_plan_ms": round(1000 * sum(plan_s) / len(plan_s), 3),
         "table_build_s_first_call": round(first["seconds_tables"], 3),
         "end_to_end_s_first_call": round(first["seconds_total"], 3),
-        "work": {"leaves": st["leaves"], "hits": st["hits"], "factor_entries": st["factor_entries"],
-                 "pairs": st["pairs"]},
+        "work": {k: st[k] for k in ("leaves", "hits", "factor_entries", "pairs", "windows",
+                                    "leaf_primes", "leaf_steps", "dense_primes", "dense_steps",
+                                    "group_steps")},
         "roofline": roofline,
         "roofline_lds": roofline_lds,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),

@@ -136,6 +136,13 @@ typedef struct pcg_stats {
   uint64_t hits;             /* cross-off hits (prime multiples visited)      */
   uint64_t factor_entries;   /* factor-table entries scanned                  */
   uint64_t pairs;            /* (segment, sieving prime) steps                */
+  /* wave-level work of the sieve kernel */
+  uint64_t windows;          /* windows of up to 64 sieving primes            */
+  uint64_t leaf_primes;      /* (segment, prime) pairs with factor entries    */
+  uint64_t leaf_steps;       /* leaf-loop steps (64 entries each)             */
+  uint64_t dense_primes;     /* primes crossed off one at a time              */
+  uint64_t dense_steps;      /* their cross-off steps                         */
+  uint64_t group_steps;      /* grouped cross-off steps                       */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

@@ -56,6 +56,12 @@ class Stats(ctypes.Structure):
         ("hits", ctypes.c_uint64),
         ("factor_entries", ctypes.c_uint64),
         ("pairs", ctypes.c_uint64),
+        ("windows", ctypes.c_uint64),
+        ("leaf_primes", ctypes.c_uint64),
+        ("leaf_steps", ctypes.c_uint64),
+        ("dense_primes", ctypes.c_uint64),
+        ("dense_steps", ctypes.c_uint64),
+        ("group_steps", ctypes.c_uint64),
     ]
 
     def as_dict(self):

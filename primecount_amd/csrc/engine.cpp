@@ -487,11 +487,11 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
   D.carry.ensure((size_t) ncols_total * 8);
   D.out.ensure((size_t) ncols_total * 16);
-  D.counters.ensure(64);
+  D.counters.ensure(128);
   hipStream_t s = D.stream;
   PCG_HIP(hsl_launch_fill_zero(D.carry.p, (size_t) ncols_total * 8, s));
   PCG_HIP(hsl_launch_fill_zero(D.out.p, (size_t) ncols_total * 16, s));
-  PCG_HIP(hsl_launch_fill_zero(D.counters.p, 64, s));
+  PCG_HIP(hsl_launch_fill_zero(D.counters.p, 128, s));
 
   double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
   uint64_t max_slots = engine_config().max_slots_per_pass;
@@ -623,9 +623,9 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   PCG_HIP(hipMemcpyAsync(s_item, D.s_out.p, all.size() * 8, hipMemcpyDeviceToHost, s));
 
   std::vector<i128> col_out(ncols_total);
-  uint64_t counters[4] = {0, 0, 0, 0};
+  uint64_t counters[10] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
-  PCG_HIP(hipMemcpyAsync(counters, D.counters.p, 32, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(counters, D.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
   for (size_t pi = 0; pi < npass; pi++) {
     float mk = 0, mc = 0;
@@ -660,6 +660,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     st->hits = counters[1];
     st->factor_entries = counters[2];
     st->pairs = counters[3];
+    st->windows = counters[4];
+    st->leaf_primes = counters[5];
+    st->leaf_steps = counters[6];
+    st->dense_primes = counters[7];
+    st->dense_steps = counters[8];
+    st->group_steps = counters[9];
   }
   return result;
 }

@@ -374,8 +374,12 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     lut[r] = (uint8_t) (8 * (r / 30) + le_count(r % 30));
 
   const uint64_t SEG = 240ull * W;
-  unsigned long long n_hits = 0, n_slots = 0;
+  unsigned long long n_slots = 0, u_hits = 0;
   unsigned long long u_leaves = 0, u_entries = 0;   // wave-uniform
+  // wave-level step counts: windows, primes with leaves, leaf-loop steps,
+  // one-at-a-time cross-off primes and steps, grouped cross-off steps
+  unsigned long long u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0,
+                     u_gsteps = 0;
 
   for (;;) {
     uint32_t qi = 0;
@@ -488,6 +492,18 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       // applied together just before the count of the second
       const unsigned long long lmask = __ballot(r_hi > r_lo) & in_win;
       const uint32_t h_win = lane < nwin ? r_h : 0u;
+      // every prime of the window is crossed off by the window's end: its
+      // hits are its cofactors coprime to 30
+      u_hits += readlane_u32(wave_incl_scan(h_win), 63);
+      // primes [0, d_win) include every one with more than 32 hits; the
+      // others take 2^lg lanes each in a grouped cross-off, 2^lg >= hits
+      const unsigned long long big = __ballot(h_win > 32);
+      const uint32_t d_win = big ? 64u - (uint32_t) __clzll((long long) big) : 0u;
+      const uint32_t hm = readlane_u32(wave_incl_max(lane >= d_win ? h_win : 0u), 63);
+      const uint32_t lg = hm <= 1 ? 0u : 32u - (uint32_t) __clz(hm - 1);      // hm <= 32
+      const uint32_t G = 1u << lg;
+      const uint32_t grp = lane >> lg;
+      const uint32_t ngrp = kBlock >> lg;
       prm.w = r_h;
       wp[lane] = prm;
       w_rem[lane] = 0;
@@ -509,52 +525,44 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       auto flush = [&](uint32_t r0, uint32_t r1) {
         if (r0 >= r1)
           return;
-        const unsigned long long run = (r1 >= 64 ? ~0ull : ((1ull << r1) - 1)) & ~((1ull << r0) - 1);
-        const unsigned long long big = __ballot(h_win > 32) & run;
-        const uint32_t d_end = big ? 64u - (uint32_t) __clzll((long long) big) : r0;
+        const uint32_t d_end = min(max(d_win, r0), r1);
         for (uint32_t j = r0; j < d_end; j++) {
           const uint32_t p = readlane_u32(r_p, j);
           const uint32_t nq = readlane_u32(r_nq, j);
-          uint32_t removed = 0;
+          u_dprimes++;
+          uint32_t removed = 0;                         // wave-uniform
           if (nq > 0 && p >= 7) {
             const uint32_t qm = readlane_u32(r_qm, j);
             const uint32_t off0 = readlane_u32(r_off0, j);
             const uint32_t cls = lane & 7;
             const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
             const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
-            if (delta < nq) {
-              const uint32_t n = (nq - 1 - delta) / 30 + 1;
-              const uint32_t off = off0 + p * delta;
-              const uint32_t byte0 = off / 30;
-              const uint32_t bib = wheel_bit(off - byte0 * 30);
-              const uint32_t j0 = lane >> 3;
-              for (uint32_t t = j0; t < n; t += 8) {
+            const uint32_t n = delta < nq ? (nq - 1 - delta) / 30 + 1 : 0u;   // class count
+            const uint32_t off = off0 + p * delta;
+            const uint32_t byte0 = off / 30;
+            const uint32_t bib = wheel_bit(off - byte0 * 30);
+            const uint32_t j0 = lane >> 3;
+            const uint32_t nmax = (nq + 29) / 30;       // >= every class count
+            u_dsteps += (nmax + 7) / 8;
+            for (uint32_t t0 = 0; t0 < nmax; t0 += 8) {
+              const uint32_t t = t0 + j0;
+              bool first = false;
+              if (t < n) {
                 const uint32_t B = byte0 + t * p;
                 const uint32_t mask = 1u << (((B & 3) << 3) | bib);
-                const uint32_t old = atomicAnd(&bm[B >> 2], ~mask);
-                if (old & mask)
-                  removed++;
+                first = (atomicAnd(&bm[B >> 2], ~mask) & mask) != 0;
               }
-              if (j0 < n)
-                n_hits += (n - j0 + 7) / 8;
+              removed += (uint32_t) __popcll(__ballot(first));
             }
           }
-          // wave total with DPP (a divergent atomic on one address would be
-          // expanded into a loop over the active lanes)
-          const uint32_t rtot = readlane_u32(wave_incl_scan(removed), 63);
           if (lane == 0)
-            w_rem[j] = rtot;
+            w_rem[j] = removed;
         }
         if (d_end >= r1)
           return;
         const bool ordered = d_end < seq_end;
-        const unsigned long long rest = run & ~((1ull << d_end) - 1);
-        const uint32_t hm = readlane_u32(wave_incl_max((rest >> lane) & 1 ? h_win : 0u), 63);
-        const uint32_t lg = hm <= 1 ? 0u : 32u - (uint32_t) __clz(hm - 1);   // hm <= 32
-        const uint32_t G = 1u << lg;
-        const uint32_t grp = lane >> lg;
-        const uint32_t ngrp = kBlock >> lg;
         for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
+          u_gsteps++;
           const uint32_t jo = jb + grp;
           const uint4 pp = wp[jo < r1 ? jo : r0];
           const uint32_t kk = lane & (G - 1);
@@ -577,17 +585,14 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               wave_lds_sync();
             }
           }
-          if (has) {
-            n_hits++;
-            if (old & mask) {
-              uint32_t attr = jo;
-              if (!ordered && delta < pp.z) {           // q = q_lo + delta < p
-                const uint32_t q = pp.x - pp.z + delta;
-                if (q > 1)
-                  attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
-              }
-              atomicAdd(&w_rem[attr], 1u);
+          if (has && (old & mask)) {
+            uint32_t attr = jo;
+            if (!ordered && delta < pp.z) {             // q = q_lo + delta < p
+              const uint32_t q = pp.x - pp.z + delta;
+              if (q > 1)
+                attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
             }
+            atomicAdd(&w_rem[attr], 1u);
           }
         }
       };
@@ -601,6 +606,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       for (uint32_t j = next_leaf(0); j < nwin; j = next_leaf(j + 1)) {
         flush(run, j);
         run = j;
+        u_lprimes++;
         const uint32_t lo = readlane_u32(r_lo, j);
         const uint32_t hi = readlane_u32(r_hi, j);
         const uint32_t Tt = readlane_u32(r_T, j);
@@ -609,41 +615,43 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         // word-pair prefixes of the current sieve state for the count queries
         build_pair_prefix(bm, pre2, W, lane);
         wave_lds_sync();
-        int M = 0;
+        int M = 0;                                      // wave-uniform
         if (Tt >> 31) {
           // factor-table entries: leaves -mu(m) phi(xp / m, b - 1) for the m
           // whose entry exceeds T (src/gourdon/D.cpp:148-188)
           u_entries += hi - lo;
+          u_lsteps += (hi - lo + kBlock - 1) / kBlock;
 #pragma nounroll
-          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock) {
-            const uint32_t f = factor[idx];
+          for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+            const uint32_t idx = base + lane;
+            const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
             const bool leaf = f > T;
-            u_leaves += __popcll(__ballot(leaf));
+            const bool pos = leaf && (f & 1);
+            const uint32_t nl = (uint32_t) __popcll(__ballot(leaf));
+            const uint32_t np = (uint32_t) __popcll(__ballot(pos));
+            u_leaves += nl;
+            M += (int) (2 * np) - (int) nl;
             if (leaf) {
               const uint32_t qq = idx / 480;
               const uint32_t m = 2310 * qq + cop[idx - qq * 480];
               const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
-              if (f & 1) {
-                S += cnt;
-                M++;
-              } else {
-                S -= cnt;
-                M--;
-              }
+              S += pos ? (long long) cnt : -(long long) cnt;
             }
           }
         } else {
           // primes q: leaves +phi(xp / q, b - 1) (src/gourdon/D.cpp:279-302)
           u_leaves += hi - lo;
+          u_lsteps += (hi - lo + kBlock - 1) / kBlock;
+          M = (int) (hi - lo);
 #pragma nounroll
-          for (uint32_t idx = lo + 1 + lane; idx <= hi; idx += kBlock) {
-            S += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
-            M++;
+          for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+            const uint32_t idx = base + lane;
+            if (idx <= hi)
+              S += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
           }
         }
-        const uint32_t mtot = readlane_u32(wave_incl_scan((uint32_t) M), 63);
         if (lane == 0)
-          w_m[j] = (int) mtot;
+          w_m[j] = M;
       }
       flush(run, nwin);
       wave_lds_sync();
@@ -659,6 +667,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         total -= readlane_u32(incl, 63);
       }
       n_slots += nwin;
+      u_win++;
       b += nwin;
     }
     // slots the segment never reached hold zero
@@ -671,14 +680,18 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       S_out[item] = tot;
   }
 
-  // u_leaves and u_entries are complete in lane 0 (the lowest entry index of
-  // every leaf-loop step belongs to lane 0)
-  long long h = wave_sum_i64((long long) n_hits);
+  // the u_* counters are updated in wave-uniform control flow only
   if (lane == 0) {
     atomicAdd(&counters[0], u_leaves);
-    atomicAdd(&counters[1], (unsigned long long) h);
+    atomicAdd(&counters[1], u_hits);
     atomicAdd(&counters[2], u_entries);
     atomicAdd(&counters[3], n_slots);
+    atomicAdd(&counters[4], u_win);
+    atomicAdd(&counters[5], u_lprimes);
+    atomicAdd(&counters[6], u_lsteps);
+    atomicAdd(&counters[7], u_dprimes);
+    atomicAdd(&counters[8], u_dsteps);
+    atomicAdd(&counters[9], u_gsteps);
   }
 }
 
