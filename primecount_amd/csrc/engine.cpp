@@ -107,10 +107,9 @@ struct Device {
   hipStream_t stream = nullptr;
   std::vector<hipEvent_t> ev;         // timing events, 3 per pass
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
-  DevBuf queries, items, order, binfo, primes, factor, pi_bits, pi_cnt;
-  DevBuf tc, tm, s_out, counters, queue, cols, sc, sm, smp, out, carry;
-  HostBuf stage[2];                   // per pass: items, order, columns
-  HostBuf s_host;                     // per item in-segment sums
+  DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
+  DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans;
+  HostBuf stage[2];                   // per pass: spans, columns
 
   hipEvent_t event(size_t i)
   {
@@ -144,10 +143,11 @@ struct Device {
 
   void release()
   {
-    for (DevBuf* b : {&queries, &items, &order, &binfo, &primes, &factor, &pi_bits, &pi_cnt,
-                      &tc, &tm, &s_out, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry})
+    for (DevBuf* b : {&queries, &items, &binfo, &primes, &factor, &pi_bits, &pi_cnt, &tc, &tm,
+                      &sacc, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry, &reach,
+                      &sfx, &spans})
       b->release();
-    for (HostBuf* b : {&stage[0], &stage[1], &s_host})
+    for (HostBuf* b : {&stage[0], &stage[1]})
       b->release();
   }
 };
@@ -170,7 +170,23 @@ struct QueryPlan {
   bool empty = true;
   QueryDesc qd{};
   std::vector<BInfo> binfo;
-  std::vector<uint32_t> nbs;       // per segment: slots (b values) it can reach
+  std::vector<uint32_t> reach;     // per b: segments that reach b (non-increasing)
+  std::vector<uint64_t> sfx;       // suffix sums of reach
+  uint32_t nseg = 0;               // segments that reach at least b_first
+
+  /// b values segment s reaches: #{b : reach_b > s}
+  uint32_t nbs(uint64_t s) const
+  {
+    return (uint32_t) (std::upper_bound(reach.begin(), reach.end(), s,
+                                        [](uint64_t v, uint32_t r) { return v >= r; }) -
+                       reach.begin());
+  }
+  /// slots of segments [0, s)
+  uint64_t F(uint64_t s) const
+  {
+    const uint32_t k = nbs(s);
+    return s * k + sfx[k];
+  }
 };
 
 uint64_t imin(uint64_t a, uint64_t b) { return a < b ? a : b; }
@@ -290,31 +306,26 @@ QueryPlan plan_query(const HslQuery& q, const PrimeTables& P)
 
 /// Slots per segment.  A segment starting at `low` can only process b with
 /// p_b <= sqrt(x / low) (beyond that its leaf range is empty and the kernel
-/// stops), so pi(min(p_{b_last}, sqrt(x / low))) - b_first + 1 bounds the b
-/// it reaches.  The bound decreases with low; trailing segments that reach
-/// no b hold no leaves and are dropped.
-void plan_segments(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W)
+/// stops): segment s reaches b iff max(1, s * 240 W) <= x / p_b^2.  reach_b
+/// counts those segments; it does not increase with b, and segments beyond
+/// reach_{b_first} hold no leaves and are dropped.
+void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W)
 {
   const uint64_t SEG = 240ull * W;
-  const uint64_t nseg = (qp.qd.limit + SEG - 1) / SEG;
+  const uint64_t nseg_all = (qp.qd.limit + SEG - 1) / SEG;
   const uint32_t b_first = qp.qd.b_first, b_last = qp.qd.b_last;
-  // Segment s (low = max(1, s * SEG)) reaches b iff p_b <= isqrt(x / low),
-  // i.e. iff low <= x / p_b^2.  smax[b] = last segment that reaches b; it
-  // does not increase with b.
-  std::vector<int64_t> smax(b_last - b_first + 1);
+  const uint32_t nb = b_last - b_first + 1;
+  qp.reach.assign(nb, 0);
   for (uint32_t b = b_first; b <= b_last; b++) {
     const uint64_t p = P.primes[b];
     const u128 L = (u128) q.x / ((u128) p * p);
-    smax[b - b_first] = L < 1 ? -1 : (int64_t) std::min<u128>(L / SEG, nseg);
+    if (L >= 1)
+      qp.reach[b - b_first] = (uint32_t) std::min<u128>(L / SEG + 1, nseg_all);
   }
-  qp.nbs.clear();
-  qp.nbs.reserve(nseg);
-  int64_t hi = (int64_t) b_last;               // largest b the segment reaches
-  for (uint64_t s = 0; s < nseg; s++) {
-    while (hi >= (int64_t) b_first && smax[hi - b_first] < (int64_t) s) hi--;
-    if (hi < (int64_t) b_first) break;
-    qp.nbs.push_back((uint32_t) (hi - b_first + 1));
-  }
+  qp.sfx.assign(nb + 1, 0);
+  for (uint32_t k = nb; k-- > 0;)
+    qp.sfx[k] = qp.sfx[k + 1] + qp.reach[k];
+  qp.nseg = qp.reach[0];
 }
 
 /// Segment size.  One wave owns a segment, so LDS per wave (8 W bytes) sets
@@ -453,136 +464,154 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   }
   int bpc = 0;
   uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 16);
-  if (W < 256 || W > 4096 || (W & (W - 1)))
-    throw pcg_error("segment words must be a power of two in [256, 4096]");
+  if (W < 256 || W > 4096 || (W % 128) != 0)
+    throw pcg_error("segment words must be a multiple of 128 in [256, 4096]");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
   bpc = std::max(bpc, 1);
   const uint64_t slots_wg = (uint64_t) D.cus * bpc;
 
-  // queries, per-b constants, items and columns
-  std::vector<QueryDesc> qdesc;
-  std::vector<BInfo> binfo;
-  std::vector<uint32_t> qslot(qs.size(), UINT32_MAX);
-  std::vector<uint32_t> colbase;                 // per slot: first global column
-  struct ItemRef { uint32_t slot, seg, nbs; };
-  std::vector<ItemRef> all;                      // (query, segment) in order
-  uint32_t ncols_total = 0;
+  // Queries with work, largest first: items are taken from the work queue in
+  // (query, segment) order, so the heaviest items (the first segments of the
+  // largest queries) start first and the light tails end the launch.
+  std::vector<size_t> live;
   for (size_t i = 0; i < qs.size(); i++) {
     if (plans[i].empty) continue;
-    plan_segments(plans[i], qs[i], T.P, W);
-    if (plans[i].nbs.empty()) continue;
+    plan_reach(plans[i], qs[i], T.P, W);
+    if (plans[i].nseg > 0) live.push_back(i);
+  }
+  if (live.empty()) return result;
+  std::stable_sort(live.begin(), live.end(), [&](size_t a, size_t b) {
+    return plans[a].sfx[0] > plans[b].sfx[0];
+  });
+
+  std::vector<QueryDesc> qdesc;
+  std::vector<BInfo> binfo;
+  std::vector<uint32_t> reach;
+  std::vector<uint64_t> sfx;
+  std::vector<uint32_t> qslot(qs.size(), UINT32_MAX);
+  std::vector<size_t> slot_query;                // query slot -> index into qs
+  std::vector<uint32_t> colbase;                 // per slot: first global column
+  uint32_t ncols_total = 0;
+  uint64_t nsegments = 0;
+  for (size_t i : live) {
     QueryDesc qd = plans[i].qd;
     qd.binfo_off = (uint32_t) binfo.size();
+    qd.reach_off = (uint32_t) reach.size();
+    qd.sfx_off = (uint32_t) sfx.size();
     qslot[i] = (uint32_t) qdesc.size();
+    slot_query.push_back(i);
     qdesc.push_back(qd);
     binfo.insert(binfo.end(), plans[i].binfo.begin(), plans[i].binfo.end());
+    reach.insert(reach.end(), plans[i].reach.begin(), plans[i].reach.end());
+    sfx.insert(sfx.end(), plans[i].sfx.begin(), plans[i].sfx.end());
     colbase.push_back(ncols_total);
-    ncols_total += plans[i].nbs[0];
-    for (uint32_t sgi = 0; sgi < plans[i].nbs.size(); sgi++)
-      all.push_back(ItemRef{qslot[i], sgi, plans[i].nbs[sgi]});
+    ncols_total += plans[i].nbs(0);
+    nsegments += plans[i].nseg;
   }
-  if (all.empty()) return result;
 
   upload(D.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
   upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
+  upload(D.reach, reach.data(), reach.size() * 4);
+  upload(D.sfx, sfx.data(), sfx.size() * 8);
+  const size_t nq = qdesc.size();
   D.carry.ensure((size_t) ncols_total * 8);
   D.out.ensure((size_t) ncols_total * 16);
+  D.sacc.ensure(nq * 16);
   D.counters.ensure(128);
   hipStream_t s = D.stream;
   PCG_HIP(hsl_launch_fill_zero(D.carry.p, (size_t) ncols_total * 8, s));
   PCG_HIP(hsl_launch_fill_zero(D.out.p, (size_t) ncols_total * 16, s));
+  PCG_HIP(hsl_launch_fill_zero(D.sacc.p, nq * 16, s));
   PCG_HIP(hsl_launch_fill_zero(D.counters.p, 128, s));
 
-  double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
   uint64_t max_slots = engine_config().max_slots_per_pass;
   if (max_slots == 0) max_slots = kMaxSlotsPerPass;
 
-  // passes: consecutive items with at most max_slots slots (at least one item)
-  std::vector<size_t> pass_at{0};
-  uint64_t max_pass_slots = 0, max_pass_items = 0;
-  {
-    uint64_t slots = 0;
-    for (size_t i = 0; i < all.size(); i++) {
-      if (i > pass_at.back() && slots + all[i].nbs > max_slots) {
-        max_pass_slots = std::max(max_pass_slots, slots);
-        max_pass_items = std::max<uint64_t>(max_pass_items, i - pass_at.back());
-        pass_at.push_back(i);
-        slots = 0;
+  // Passes: runs of (query, segment) items with at most max_slots slots (a
+  // single segment may exceed it).  A pass is a list of spans, one per query
+  // it touches; boundaries are found by binary search on the slot count F.
+  struct Pass { std::vector<PassSpan> spans; uint32_t nitems = 0; uint64_t slots = 0; };
+  std::vector<Pass> passes(1);
+  for (uint32_t qsl = 0; qsl < nq; qsl++) {
+    const QueryPlan& qp = plans[slot_query[qsl]];
+    uint64_t s0 = 0;
+    while (s0 < qp.nseg) {
+      Pass& ps = passes.back();
+      const uint64_t f0 = qp.F(s0);
+      const uint64_t cap = max_slots > ps.slots ? max_slots - ps.slots : 0;
+      // largest s1 in (s0, nseg] with F(s1) - F(s0) <= cap
+      uint64_t lo = s0, hi = qp.nseg;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi + 1) / 2;
+        if (qp.F(mid) - f0 <= cap) lo = mid;
+        else hi = mid - 1;
       }
-      slots += all[i].nbs;
+      uint64_t s1 = lo;
+      if (s1 == s0) {
+        if (ps.nitems > 0) {           // full: start a new pass
+          passes.emplace_back();
+          continue;
+        }
+        s1 = s0 + 1;                   // one segment larger than a pass
+      }
+      PassSpan sp{};
+      sp.query = qsl;
+      sp.s0 = (uint32_t) s0;
+      sp.s1 = (uint32_t) s1;
+      sp.item_base = ps.nitems;
+      sp.t_base = ps.slots - f0;       // modulo 2^64
+      ps.spans.push_back(sp);
+      ps.nitems += (uint32_t) (s1 - s0);
+      ps.slots += qp.F(s1) - f0;
+      s0 = s1;
+      if (s0 < qp.nseg) passes.emplace_back();
     }
-    max_pass_slots = std::max(max_pass_slots, slots);
-    max_pass_items = std::max<uint64_t>(max_pass_items, all.size() - pass_at.back());
-    pass_at.push_back(all.size());
   }
-  const size_t npass = pass_at.size() - 1;
+  if (passes.back().nitems == 0) passes.pop_back();
+  const size_t npass = passes.size();
 
-  // Device buffers shared by the passes (the stream orders their reuse);
-  // host staging alternates between two pinned slots so that the host
-  // prepares pass i + 1 while the device runs pass i.
-  const size_t items_bytes = (size_t) max_pass_items * sizeof(SegItem);
-  const size_t order_bytes = (size_t) max_pass_items * 4;
+  uint64_t max_pass_slots = 0, max_pass_items = 0, max_pass_spans = 0;
+  for (const Pass& ps : passes) {
+    max_pass_slots = std::max(max_pass_slots, ps.slots);
+    max_pass_items = std::max<uint64_t>(max_pass_items, ps.nitems);
+    max_pass_spans = std::max<uint64_t>(max_pass_spans, ps.spans.size());
+  }
+  const size_t spans_bytes = (size_t) max_pass_spans * sizeof(PassSpan);
   const size_t cols_bytes = (size_t) ncols_total * sizeof(CombineCol);
-  D.items.ensure(items_bytes);
-  D.order.ensure(order_bytes);
+  D.spans.ensure(spans_bytes);
+  D.items.ensure((size_t) max_pass_items * sizeof(SegItem));
   D.cols.ensure(cols_bytes);
   D.tc.ensure(max_pass_slots * 4);
   D.tm.ensure(max_pass_slots * 4);
-  D.s_out.ensure(all.size() * 8);
   D.queue.ensure(npass * 4);
   PCG_HIP(hsl_launch_fill_zero(D.queue.p, npass * 4, s));
-  for (auto& hb : D.stage) hb.ensure(items_bytes + order_bytes + cols_bytes);
-  long long* s_item = (long long*) D.s_host.ensure(all.size() * 8);
+  for (auto& hb : D.stage) hb.ensure(spans_bytes + cols_bytes);
+  double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
 
   for (size_t pi = 0; pi < npass; pi++) {
     double tp = now();
-    const size_t pass_begin = pass_at[pi], pass_end = pass_at[pi + 1];
-    const uint32_t nitems = (uint32_t) (pass_end - pass_begin);
+    const Pass& ps = passes[pi];
     const int slot = (int) (pi & 1);
     if (pi >= 2) PCG_HIP(hipEventSynchronize(D.staged[slot]));
     char* stage = (char*) D.stage[slot].p;
-    SegItem* items = (SegItem*) stage;
-    uint32_t* order = (uint32_t*) (stage + items_bytes);
-    CombineCol* cols = (CombineCol*) (stage + items_bytes + order_bytes);
-
-    uint64_t off = 0;
-    for (uint32_t i = 0; i < nitems; i++) {
-      const ItemRef& r = all[pass_begin + i];
-      SegItem& it = items[i];
-      std::memset(&it, 0, sizeof(it));
-      it.query = r.slot;
-      it.seg = r.seg;
-      it.nbs = r.nbs;
-      it.t_off = off;
-      off += r.nbs;
-    }
-    const uint64_t slots = off;
-    // schedule: segments with the most slots first (one query: natural order)
-    std::iota(order, order + nitems, 0u);
-    if (qdesc.size() > 1)
-      std::stable_sort(order, order + nitems,
-                       [&](uint32_t a, uint32_t b) { return items[a].nbs > items[b].nbs; });
-    // combine columns of this pass
-    uint32_t ncols = 0;
-    uint32_t max_n = 1;
-    for (uint32_t i = 0; i < nitems;) {
-      uint32_t slot_q = items[i].query, j = i;
-      while (j < nitems && items[j].query == slot_q) j++;
-      // items [i, j) are consecutive segments of one query; nbs non-increasing
-      uint32_t n = j - i;
-      for (uint32_t br = 0; br < items[i].nbs; br++) {
-        while (n > 0 && items[i + n - 1].nbs <= br) n--;
-        if (n == 0) break;
-        cols[ncols++] = CombineCol{br, i, n, colbase[slot_q] + br};
+    PassSpan* spans = (PassSpan*) stage;
+    CombineCol* cols = (CombineCol*) (stage + spans_bytes);
+    std::memcpy(spans, ps.spans.data(), ps.spans.size() * sizeof(PassSpan));
+    // combine columns: per span, b with at least one segment in [s0, s1)
+    uint32_t ncols = 0, max_n = 1;
+    for (const PassSpan& sp : ps.spans) {
+      const QueryPlan& qp = plans[slot_query[sp.query]];
+      const uint32_t nbs0 = qp.nbs(sp.s0);
+      for (uint32_t br = 0; br < nbs0; br++) {
+        const uint32_t n = std::min(qp.reach[br], sp.s1) - sp.s0;   // reach[br] > s0
+        cols[ncols++] = CombineCol{br, sp.item_base, n, colbase[sp.query] + br};
         max_n = std::max(max_n, n);
       }
-      i = j;
     }
     uint32_t slice_len = std::max<uint32_t>(1, (max_n + 127) / 128);
     uint32_t nslices = (max_n + slice_len - 1) / slice_len;
 
-    PCG_HIP(hipMemcpyAsync(D.items.p, items, (size_t) nitems * sizeof(SegItem), hipMemcpyHostToDevice, s));
-    PCG_HIP(hipMemcpyAsync(D.order.p, order, (size_t) nitems * 4, hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(D.spans.p, spans, ps.spans.size() * sizeof(PassSpan), hipMemcpyHostToDevice, s));
     PCG_HIP(hipMemcpyAsync(D.cols.p, cols, (size_t) ncols * sizeof(CombineCol), hipMemcpyHostToDevice, s));
     PCG_HIP(hipEventRecord(D.staged[slot], s));
     if ((size_t) nslices * ncols * 16 > D.smp.cap)
@@ -590,12 +619,14 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     D.sc.ensure((size_t) nslices * ncols * 8);
     D.sm.ensure((size_t) nslices * ncols * 8);
     D.smp.ensure((size_t) nslices * ncols * 16);
+    PCG_HIP(hsl_launch_build_items((const QueryDesc*) D.queries.p, (const PassSpan*) D.spans.p,
+                                   (uint32_t) ps.spans.size(), (const uint32_t*) D.reach.p,
+                                   (const uint64_t*) D.sfx.p, (SegItem*) D.items.p, ps.nitems, s));
     t_plan += now() - tp;
 
     HslArgs a{};
     a.queries = (const QueryDesc*) D.queries.p;
     a.items = (const SegItem*) D.items.p;
-    a.order = (const uint32_t*) D.order.p;
     a.binfo = (const BInfo*) D.binfo.p;
     a.primes = (const uint32_t*) D.primes.p;
     a.factor = D.factor.p;
@@ -603,13 +634,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
     a.T_C = (uint32_t*) D.tc.p;
     a.T_M = (int32_t*) D.tm.p;
-    a.S_out = (long long*) D.s_out.p + pass_begin;
+    a.S_acc = (unsigned long long*) D.sacc.p;
     a.counters = (unsigned long long*) D.counters.p;
     a.queue = (unsigned int*) D.queue.p + pi;
-    a.nitems = nitems;
+    a.nitems = ps.nitems;
     a.W = W;
-    (void) slots;
-    uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, nitems);
+    uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, ps.nitems);
 
     PCG_HIP(hipEventRecord(D.event(3 * pi), s));
     PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
@@ -620,11 +650,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
                                (long long*) D.carry.p, D.out.p, s));
     PCG_HIP(hipEventRecord(D.event(3 * pi + 2), s));
   }
-  PCG_HIP(hipMemcpyAsync(s_item, D.s_out.p, all.size() * 8, hipMemcpyDeviceToHost, s));
 
   std::vector<i128> col_out(ncols_total);
+  std::vector<uint64_t> sacc(2 * nq);
   uint64_t counters[10] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(sacc.data(), D.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, D.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
   for (size_t pi = 0; pi < npass; pi++) {
@@ -635,9 +666,10 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     ms_combine += mc;
   }
 
-  std::vector<i128> per_slot(qdesc.size(), 0);
-  for (size_t i = 0; i < all.size(); i++) per_slot[all[i].slot] += s_item[i];
-  for (size_t q = 0; q < qdesc.size(); q++) {
+  std::vector<i128> per_slot(nq, 0);
+  for (size_t q = 0; q < nq; q++)
+    per_slot[q] = (i128) (((u128) sacc[2 * q + 1] << 64) | sacc[2 * q]);
+  for (size_t q = 0; q < nq; q++) {
     uint32_t ncol_q = (q + 1 < qdesc.size() ? colbase[q + 1] : ncols_total) - colbase[q];
     for (uint32_t c = 0; c < ncol_q; c++) per_slot[q] += col_out[colbase[q] + c];
   }
@@ -650,8 +682,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
     st->seconds_plan = t_plan;
     st->seconds_kernel = ms_kernel / 1000.0;
     st->seconds_combine = ms_combine / 1000.0;
-    st->segments = all.size();
-    st->chunks = all.size();
+    st->segments = nsegments;
+    st->chunks = nsegments;
     st->segment_words = W;
     st->sieve_limit = qdesc.size() == 1 ? qdesc[0].limit : 0;
     st->b_first = qdesc.size() == 1 ? qdesc[0].b_first : 0;

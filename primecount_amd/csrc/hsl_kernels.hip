@@ -351,10 +351,10 @@ __device__ __forceinline__ uint32_t wave_total_u32(uint32_t v)
 template <typename FT>
 __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
-    const uint32_t* __restrict__ order, const BInfo* __restrict__ binfo_all,
+    const BInfo* __restrict__ binfo_all,
     const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
     const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,
-    uint32_t* __restrict__ T_C, int32_t* __restrict__ T_M, long long* __restrict__ S_out,
+    uint32_t* __restrict__ T_C, int32_t* __restrict__ T_M, unsigned long long* __restrict__ S_acc,
     unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,
     uint32_t nitems, uint32_t W)
 {
@@ -388,8 +388,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     qi = __builtin_amdgcn_readfirstlane(qi);
     if (qi >= nitems)
       break;
-    const uint32_t item = order[qi];
-    const SegItem it = items[item];
+    const SegItem it = items[qi];
     const QueryDesc qd = queries[it.query];
     const BInfo* binfo = binfo_all + qd.binfo_off;
     uint32_t* TC = T_C + it.t_off;
@@ -675,9 +674,17 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       TC[bq - qd.b_first] = 0;
       TM[bq - qd.b_first] = 0;
     }
-    long long tot = wave_sum_i64(S);
-    if (lane == 0)
-      S_out[item] = tot;
+    const long long tot = wave_sum_i64(S);
+    if (lane == 0 && tot != 0) {
+      // 128-bit accumulation from two 64-bit atomics: the add whose low word
+      // wraps carries into the high word (exact under any interleaving)
+      unsigned long long* acc = S_acc + 2 * it.query;
+      const unsigned long long v = (unsigned long long) tot;
+      const unsigned long long old = atomicAdd(&acc[0], v);
+      const unsigned long long inc = (old + v < old ? 1ull : 0ull) - (tot < 0 ? 1ull : 0ull);
+      if (inc != 0)
+        atomicAdd(&acc[1], inc);
+    }
   }
 
   // the u_* counters are updated in wave-uniform control flow only
@@ -699,9 +706,9 @@ template <typename FT>
 static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st)
 {
   hipLaunchKernelGGL(hsl_sieve_kernel<FT>, dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
-                     a.queries, a.items, a.order, a.binfo, a.primes,
+                     a.queries, a.items, a.binfo, a.primes,
                      reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_C, a.T_M,
-                     a.S_out, a.counters, a.queue, a.nitems, a.W);
+                     a.S_acc, a.counters, a.queue, a.nitems, a.W);
   return hipGetLastError();
 }
 
@@ -712,6 +719,56 @@ __global__ void hsl_fill_zero(uint4* p, size_t n16)
   size_t stride = (size_t) gridDim.x * blockDim.x;
   for (; i < n16; i += stride)
     p[i] = make_uint4(0, 0, 0, 0);
+}
+
+/// Number of b with reach > s: reach does not increase, so this is the first
+/// index whose reach is <= s.
+__device__ __forceinline__ uint32_t reach_count(const uint32_t* reach, uint32_t nb, uint32_t s)
+{
+  uint32_t lo = 0, hi = nb;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (reach[mid] > s)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+/// Work items of one pass from its spans: item i is segment s0 + (i -
+/// item_base) of its span's query, with nbs and slot offset from the reach
+/// table (see QueryDesc).
+__global__ void hsl_build_items(const QueryDesc* __restrict__ queries,
+                                const PassSpan* __restrict__ spans, uint32_t nspans,
+                                const uint32_t* __restrict__ reach_all,
+                                const uint64_t* __restrict__ sfx_all, SegItem* __restrict__ items,
+                                uint32_t nitems)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nitems)
+    return;
+  uint32_t lo = 0, hi = nspans - 1;               // last span with item_base <= i
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (spans[mid].item_base <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const PassSpan sp = spans[lo];
+  const QueryDesc q = queries[sp.query];
+  const uint32_t nb = q.b_last - q.b_first + 1;
+  const uint32_t seg = sp.s0 + (i - sp.item_base);
+  const uint32_t nbs = reach_count(reach_all + q.reach_off, nb, seg);
+  SegItem it;
+  it.query = sp.query;
+  it.seg = seg;
+  it.nbs = nbs;
+  it.pad = 0;
+  it.t_off = sp.t_base + (uint64_t) seg * nbs + sfx_all[q.sfx_off + nbs];
+  it.pad1 = 0;
+  items[i] = it;
 }
 
 /// Column = one (query, b); slice = a range of the column's segments.  For
@@ -825,6 +882,16 @@ hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st)
   if (n16 == 0) return hipSuccess;
   uint32_t grid = (uint32_t) std::min<size_t>((n16 + 255) / 256, 4096);
   hipLaunchKernelGGL(hsl_fill_zero, dim3(grid), dim3(256), 0, st, (uint4*) p, n16);
+  return hipGetLastError();
+}
+
+hipError_t hsl_launch_build_items(const QueryDesc* queries, const PassSpan* spans, uint32_t nspans,
+                                  const uint32_t* reach, const uint64_t* sfx, SegItem* items,
+                                  uint32_t nitems, hipStream_t st)
+{
+  if (nitems == 0 || nspans == 0) return hipSuccess;
+  hipLaunchKernelGGL(hsl_build_items, dim3((nitems + 255) / 256), dim3(256), 0, st, queries,
+                     spans, nspans, reach, sfx, items, nitems);
   return hipGetLastError();
 }
 

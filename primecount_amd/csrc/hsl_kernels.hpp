@@ -40,12 +40,33 @@ struct alignas(16) BInfo {
 };
 
 /// One query of a launch.
+///
+/// reach[reach_off + b - b_first] = number of segments that reach b (segment
+/// s reaches b iff max(1, s * 240 W) <= x / p_b^2); it does not increase with
+/// b, so segment s holds the slots b - b_first < nbs(s) = #{b : reach > s},
+/// and the slots of segments [0, s) number
+///   F(s) = sum_b min(s, reach_b) = s * nbs(s) + sfx[nbs(s)]
+/// with the suffix sums sfx[sfx_off + k] = sum_{b - b_first >= k} reach_b.
 struct QueryDesc {
   uint64_t limit;        // sieve interval [0, limit)
   uint32_t b_first;      // first b (k+1 or c+1)
   uint32_t b_last;       // last b that can hold leaves
   uint32_t binfo_off;    // BInfo of b is binfo[binfo_off + b - b_first]
   uint32_t pre_last;     // primes p_4..p_pre_last are pre-sieved (pre_last = b_first-1)
+  uint32_t reach_off;    // per b: segments reaching b (b_last - b_first + 1 values)
+  uint32_t sfx_off;      // suffix sums of reach (b_last - b_first + 2 values)
+};
+
+/// Segments [s0, s1) of query `query` within one pass: items
+/// [item_base, item_base + s1 - s0); the slots of segment s start at
+/// t_base + F(s) (t_base = pass offset of s0 minus F(s0), modulo 2^64).
+struct alignas(16) PassSpan {
+  uint32_t query;
+  uint32_t s0;
+  uint32_t s1;
+  uint32_t item_base;
+  uint64_t t_base;
+  uint64_t pad;
 };
 
 /// One work-queue item: segment `seg` of query `query`.  Its slots
@@ -63,7 +84,6 @@ struct alignas(16) SegItem {
 struct HslArgs {
   const QueryDesc* queries;
   const SegItem* items;
-  const uint32_t* order;        // work-queue position -> item index
   const BInfo* binfo;
   const uint32_t* primes;       // primes[0] = 0, primes[1] = 2, ...
   const void* factor;           // uint16_t or uint32_t factor table
@@ -71,11 +91,11 @@ struct HslArgs {
   const uint32_t* pi_cnt;       // primes >= 7 below 240 * i
   uint32_t* T_C;                // per slot: unsieved count before p_b is crossed off
   int32_t* T_M;                 // per slot: sum of leaf signs
-  long long* S_out;             // per item: sum of sign * in-segment count
+  unsigned long long* S_acc;    // per query: 128-bit sum of sign * in-segment count (lo, hi)
   unsigned long long* counters; // leaves, hits, factor entries, slots
   unsigned int* queue;          // next work item
   uint32_t nitems;
-  uint32_t W;                   // words per segment (power of two, 256..4096)
+  uint32_t W;                   // words per segment (multiple of 128, 256..4096)
 };
 
 constexpr int kBlock = 64;     // one wave per segment
@@ -105,6 +125,9 @@ hipError_t hsl_upload_constants();
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
+hipError_t hsl_launch_build_items(const QueryDesc* queries, const PassSpan* spans, uint32_t nspans,
+                                  const uint32_t* reach, const uint64_t* sfx, SegItem* items,
+                                  uint32_t nitems, hipStream_t st);
 hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const SegItem* items,
                               const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
                               uint32_t nslices, long long* sC, long long* sM, void* sMP,

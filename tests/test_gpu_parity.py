@@ -67,7 +67,7 @@ def test_random_vs_oracle(pc, oracle):
         assert pc.S2_hard(x, yd, x // yd, 8) == oracle.S2_hard(x, yd, x // yd, 8), (x, yd)
 
 
-@pytest.mark.parametrize("words", [256, 512, 1024, 2048, 4096])
+@pytest.mark.parametrize("words", [256, 384, 512, 640, 1024, 2048, 4096])
 def test_segment_size_invariance(pc, words):
     log("segment words %d" % words)
     r = {int(v["x"]): v for v in golden("configs.json")["D"]}[10 ** 15]
