@@ -112,7 +112,7 @@ int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out);
  * results (replaces the segment sizing of src/LoadBalancerS2.cpp:65-110). */
 int pcg_set_segment_words(uint32_t words);
 
-/* (segment, prime) slots per kernel pass (0 = default, 2^30).  C and M take
+/* (segment, prime) slots per kernel pass (0 = default, 2^31).  C and M take
  * 8 bytes of HBM per slot; larger queries run in several passes whose carry is
  * kept on the device.  The pass size never changes results.  No counterpart in
  * the reference (its threads carry phi through their own segments). */

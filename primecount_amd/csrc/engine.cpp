@@ -192,7 +192,7 @@ struct QueryPlan {
 uint64_t imin(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
 /// Slot budget of one pass (C and M take 8 bytes per slot).
-constexpr uint64_t kMaxSlotsPerPass = (uint64_t) 1 << 30;
+constexpr uint64_t kMaxSlotsPerPass = (uint64_t) 1 << 31;
 
 /// Prime-table range a query needs.
 uint64_t prime_range(const HslQuery& q)
