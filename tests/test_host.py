@@ -2,6 +2,8 @@
 
 - the table builders against the arithmetic definitions (a C++ harness
   compiled here from primecount_amd/csrc/tables.cpp);
+- the error analysis of the kernel's leaf quotient (FP64 reciprocal estimate
+  plus one integer fix-up);
 - the shared library loads and exports every function that
   include/primecount_mi355x.h declares; the C++ overload header compiles;
 - without a device the engine refuses to run (no silent CPU fallback).
@@ -22,6 +24,16 @@ def test_table_builders_match_definitions(tmp_path):
                     os.path.join(REPO, "tests", "native", "tables_check.cpp"),
                     os.path.join(REPO, "primecount_amd", "csrc", "tables.cpp")], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+def test_leaf_quotient_estimate_is_exact(tmp_path):
+    # the kernel's FP64-reciprocal quotient with one integer fix-up, under a
+    # reciprocal that is up to 4096 ulps off
+    exe = tmp_path / "leaf_stop_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe),
+                    os.path.join(REPO, "tests", "native", "leaf_stop_check.cpp")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
 
 
