@@ -44,6 +44,21 @@ def expected_D(x):
     return None, None
 
 
+def measured_lds_peak():
+    """LDS read bandwidth of this device in GB/s, from the probe kernel built
+    next to the engine (bank-conflict-free 128-bit LDS loads on every CU)."""
+    import ctypes
+    path = os.path.join(REPO, "primecount_amd", "libpcg_lds_probe.so")
+    if not os.path.exists(path):
+        return None
+    lib = ctypes.CDLL(path)
+    lib.pcg_probe_lds_read_gbs.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    v = ctypes.c_double()
+    if lib.pcg_probe_lds_read_gbs(ctypes.byref(v)) != 0:
+        return None
+    return v.value
+
+
 def traffic_from_profile(x):
     """HBM bytes per sieve-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
@@ -173,12 +188,16 @@ def main():
         "frac": round(hbm_bytes / k_avg / 1e9 / HBM_PEAK_GBS, 6),
         "traffic": traffic,
     }
+    lds_peak = measured_lds_peak()
+    lds_peak_used = lds_peak if lds_peak else LDS_PEAK_GBS
     roofline_lds = {
         "bound": "lds",
         "achieved": round(lds_bytes / k_avg / 1e9, 3),
-        "peak": LDS_PEAK_GBS,
+        "peak": round(lds_peak_used, 1),
+        "peak_source": "measured (LDS read probe)" if lds_peak else "nominal",
+        "peak_nominal": LDS_PEAK_GBS,
         "unit": "GB/s",
-        "frac": round(lds_bytes / k_avg / 1e9 / LDS_PEAK_GBS, 6),
+        "frac": round(lds_bytes / k_avg / 1e9 / lds_peak_used, 6),
         "bytes_per_launch": lds_bytes,
     }
     out = {
