@@ -152,6 +152,62 @@ int pcg_last_stats(pcg_stats* out);
  * factor tables that stay resident in HBM between calls. */
 int pcg_release(void);
 
+/* ---- serving: continuous batching ---------------------------------------
+ *
+ * A context owns a worker thread.  pcg_submit queues queries from any thread
+ * and returns at once; whenever the device is free the worker runs everything
+ * queued (up to max_batch queries) as one batch -- one persistent launch per
+ * table set, as pcg_D_batch -- so queries that arrive during a launch join the
+ * next one.  pcg_poll hands out finished results in completion order.  No
+ * counterpart in the reference, whose entry points compute one x per call
+ * (D / S2_hard, include/gourdon.hpp:28,38, include/S.hpp:25,32). */
+
+#define PCG_KIND_D 0
+#define PCG_KIND_S2_HARD 1
+
+typedef struct pcg_query {
+  int32_t kind;        /* PCG_KIND_D or PCG_KIND_S2_HARD                       */
+  int32_t reserved;    /* 0                                                    */
+  pcg_i128 x;
+  int64_t y, z, k;     /* all 0: the reference's default parameters for x     */
+} pcg_query;
+
+typedef struct pcg_result {
+  uint64_t id;         /* as returned by pcg_submit                            */
+  int32_t status;      /* PCG_OK or a negative PCG_E_* code                    */
+  int32_t reserved;
+  pcg_i128 value;      /* D or S2_hard when status == PCG_OK                   */
+  double seconds;      /* submit to result available                           */
+} pcg_result;
+
+typedef struct pcg_ctx pcg_ctx;
+
+/* Creates a serving context.  y_max, z_max > 0 build the D tables for default
+ * queries up to that size now (else on first use); hbm_budget bounds the
+ * per-pass slot arrays in bytes (0 = default 16 GiB); max_batch bounds the
+ * queries of one launch (0 = 1024). */
+int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t max_batch,
+                   pcg_ctx** ctx);
+
+/* Queues n queries; ids[i] receives the id of q[i].  A query outside the
+ * engine's domain is answered at once with an error result. */
+int pcg_submit(pcg_ctx* ctx, const pcg_query* q, size_t n, uint64_t* ids);
+
+/* Moves up to max finished results into out; waits up to timeout_ms (< 0:
+ * without limit) for the first one.  *n receives the number written. */
+int pcg_poll(pcg_ctx* ctx, pcg_result* out, size_t max, int timeout_ms, size_t* n);
+
+typedef struct pcg_ctx_stats {
+  uint64_t submitted, completed, batches, max_batch;
+  double busy_seconds;  /* worker time inside launches                         */
+} pcg_ctx_stats;
+
+int pcg_ctx_get_stats(pcg_ctx* ctx, pcg_ctx_stats* out);
+
+/* Finishes every queued query, stops the worker and frees the context
+ * (results not yet polled are dropped). */
+int pcg_ctx_destroy(pcg_ctx* ctx);
+
 #ifdef __cplusplus
 }
 #endif

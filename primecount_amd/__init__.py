@@ -68,6 +68,26 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class _Query(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("reserved", ctypes.c_int32), ("x", _I128),
+                ("y", ctypes.c_int64), ("z", ctypes.c_int64), ("k", ctypes.c_int64)]
+
+
+class _Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("value", _I128), ("seconds", ctypes.c_double)]
+
+
+class _CtxStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("max_batch", ctypes.c_uint64),
+                ("busy_seconds", ctypes.c_double)]
+
+
+KIND_D = 0
+KIND_S2_HARD = 1
+
+
 def library_path():
     return _LIB_PATH
 
@@ -104,6 +124,14 @@ def load():
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
+        "pcg_ctx_create": ([I64, I64, ctypes.c_uint64, ctypes.c_uint32,
+                            ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
+        "pcg_submit": ([ctypes.c_void_p, ctypes.POINTER(_Query), ctypes.c_size_t,
+                        ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "pcg_poll": ([ctypes.c_void_p, ctypes.POINTER(_Result), ctypes.c_size_t, ctypes.c_int,
+                      ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+        "pcg_ctx_get_stats": ([ctypes.c_void_p, ctypes.POINTER(_CtxStats)], ctypes.c_int),
+        "pcg_ctx_destroy": ([ctypes.c_void_p], ctypes.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -240,3 +268,73 @@ def device_available():
 
 def release():
     _check(load().pcg_release())
+
+
+class Server:
+    """Continuous-batching serving context (pcg_ctx_*): submit queries from any
+    thread, poll finished results.  Queries arriving while a launch runs join
+    the next launch.
+
+        with Server(z_max=10**8) as sv:
+            ids = sv.submit([(KIND_D, 10**15), (KIND_D, 10**16)])
+            results = sv.poll_all(len(ids))
+    """
+
+    def __init__(self, y_max=0, z_max=0, hbm_budget=0, max_batch=0):
+        self._ctx = ctypes.c_void_p()
+        _check(load().pcg_ctx_create(int(y_max), int(z_max), int(hbm_budget), int(max_batch),
+                                     ctypes.byref(self._ctx)))
+
+    def submit(self, queries):
+        """queries: iterable of (kind, x) for default parameters or
+        (kind, x, y, z, k).  Returns the ids in order."""
+        qs = list(queries)
+        n = len(qs)
+        arr = (_Query * max(n, 1))()
+        for i, q in enumerate(qs):
+            arr[i].kind = int(q[0])
+            arr[i].x = _i128(q[1])
+            if len(q) > 2:
+                arr[i].y, arr[i].z, arr[i].k = int(q[2]), int(q[3]), int(q[4])
+        ids = (ctypes.c_uint64 * max(n, 1))()
+        _check(load().pcg_submit(self._ctx, arr, n, ids))
+        return [ids[i] for i in range(n)]
+
+    def poll(self, max_results=1024, timeout_ms=-1):
+        """Finished results: dicts with id, status, value, seconds, error."""
+        out = (_Result * max(max_results, 1))()
+        n = ctypes.c_size_t()
+        _check(load().pcg_poll(self._ctx, out, max_results, int(timeout_ms), ctypes.byref(n)))
+        return [{"id": out[i].id, "status": out[i].status, "value": _from128(out[i].value),
+                 "seconds": out[i].seconds} for i in range(n.value)]
+
+    def poll_all(self, count, timeout_ms=-1):
+        res = []
+        while len(res) < count:
+            got = self.poll(count - len(res), timeout_ms)
+            if not got and timeout_ms >= 0:
+                break
+            res.extend(got)
+        return res
+
+    def stats(self):
+        st = _CtxStats()
+        _check(load().pcg_ctx_get_stats(self._ctx, ctypes.byref(st)))
+        return {name: getattr(st, name) for name, _ in st._fields_}
+
+    def close(self):
+        if self._ctx:
+            _check(load().pcg_ctx_destroy(self._ctx))
+            self._ctx = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

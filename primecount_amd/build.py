@@ -34,6 +34,7 @@ SOURCES = [
     # exact IEEE double evaluation of the alpha formulas: no fused multiply-add
     ("params.cpp", ["-ffp-contract=off"], False),
     ("capi.cpp", [], False),
+    ("serve.cpp", [], False),
 ]
 
 

@@ -8,6 +8,7 @@
 
 #include "engine.hpp"
 #include "params.hpp"
+#include "serve.hpp"
 
 #include <exception>
 #include <new>
@@ -167,6 +168,96 @@ int pcg_dr_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* c)
     if (z) *z = p.z;
     if (c) *c = p.c;
   });
+}
+
+int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t max_batch,
+                   pcg_ctx** ctx)
+{
+  return guarded([&] {
+    if (!ctx) throw pcg_error("ctx must not be NULL");
+    *ctx = nullptr;
+    if (y_max < 0 || z_max < 0) throw pcg_error("y_max and z_max must be >= 0");
+    if (!device_available())
+      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU");
+    // default parameters have y == z: one shared D table covers them all
+    if (y_max > 0 || z_max > 0) {
+      const int64_t m = std::max(y_max, z_max);
+      hsl_reserve(LeafSum::D, m, m);
+    }
+    const uint64_t slots = hbm_budget ? std::max<uint64_t>(hbm_budget / 8, 64) : 0;
+    *ctx = reinterpret_cast<pcg_ctx*>(new Server(slots, max_batch));
+  });
+}
+
+int pcg_submit(pcg_ctx* ctx, const pcg_query* q, size_t n, uint64_t* ids)
+{
+  return guarded([&] {
+    if (!ctx) throw pcg_error("ctx must not be NULL");
+    if (n && (!q || !ids)) throw pcg_error("q and ids must not be NULL");
+    Server* sv = reinterpret_cast<Server*>(ctx);
+    for (size_t i = 0; i < n; i++) {
+      HslQuery hq;
+      int status = PCG_OK;
+      std::string err;
+      try {
+        if (q[i].kind != PCG_KIND_D && q[i].kind != PCG_KIND_S2_HARD)
+          throw pcg_error("unknown query kind");
+        const LeafSum kind = q[i].kind == PCG_KIND_D ? LeafSum::D : LeafSum::S2_hard;
+        const i128 x = from_abi(q[i].x);
+        if (q[i].y == 0 && q[i].z == 0 && q[i].k == 0) {
+          hq = default_query(kind, x);
+        } else {
+          hq.kind = kind;
+          hq.x = x;
+          hq.y = q[i].y;
+          hq.z = q[i].z;
+          hq.k = q[i].k;
+        }
+        hsl_validate(hq);
+      } catch (const std::exception& e) {
+        status = status_of(e);
+        err = e.what();
+      }
+      ids[i] = sv->submit(hq, status, err);
+    }
+  });
+}
+
+int pcg_poll(pcg_ctx* ctx, pcg_result* out, size_t max, int timeout_ms, size_t* n)
+{
+  return guarded([&] {
+    if (!ctx || !n) throw pcg_error("ctx and n must not be NULL");
+    if (max && !out) throw pcg_error("out must not be NULL");
+    *n = 0;
+    std::vector<ServeResult> res;
+    reinterpret_cast<Server*>(ctx)->poll(res, max, timeout_ms);
+    for (size_t i = 0; i < res.size(); i++) {
+      out[i].id = res[i].id;
+      out[i].status = res[i].status;
+      out[i].reserved = 0;
+      out[i].value = to_abi(res[i].value);
+      out[i].seconds = res[i].seconds;
+    }
+    *n = res.size();
+  });
+}
+
+int pcg_ctx_get_stats(pcg_ctx* ctx, pcg_ctx_stats* out)
+{
+  return guarded([&] {
+    if (!ctx || !out) throw pcg_error("ctx and out must not be NULL");
+    ServeStats st = reinterpret_cast<Server*>(ctx)->stats();
+    out->submitted = st.submitted;
+    out->completed = st.completed;
+    out->batches = st.batches;
+    out->max_batch = st.max_batch;
+    out->busy_seconds = st.busy_seconds;
+  });
+}
+
+int pcg_ctx_destroy(pcg_ctx* ctx)
+{
+  return guarded([&] { delete reinterpret_cast<Server*>(ctx); });
 }
 
 int pcg_set_alpha_y(double a) { return guarded([&] { set_alpha_y(a); }); }

@@ -441,7 +441,7 @@ const TableSet& resident_tables(const std::vector<HslQuery>& qs)
 /// written by the sieve kernel and folded by the combine kernels.  Items are
 /// processed in passes of at most kMaxSlotsPerPass slots so that the slot
 /// arrays stay bounded; the per-column carry survives between passes.
-std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
+std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint64_t slots_override = 0)
 {
   Device& D = g_dev;
   D.init();
@@ -524,7 +524,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st)
   PCG_HIP(hsl_launch_fill_zero(D.sacc.p, nq * 16, s));
   PCG_HIP(hsl_launch_fill_zero(D.counters.p, 128, s));
 
-  uint64_t max_slots = engine_config().max_slots_per_pass;
+  uint64_t max_slots = slots_override ? slots_override : engine_config().max_slots_per_pass;
   if (max_slots == 0) max_slots = kMaxSlotsPerPass;
 
   // Passes: runs of (query, segment) items with at most max_slots slots (a
@@ -728,7 +728,8 @@ i128 hsl_compute(const HslQuery& q, pcg_stats* stats)
   return r[0];
 }
 
-std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats)
+std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats,
+                                    uint64_t max_slots)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
   std::vector<i128> out(qs.size(), 0);
@@ -738,7 +739,7 @@ std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* 
     std::vector<HslQuery> sub;
     for (size_t i : g) sub.push_back(qs[i]);
     pcg_stats st{};
-    auto r = run_group(sub, &st);
+    auto r = run_group(sub, &st, max_slots);
     for (size_t j = 0; j < g.size(); j++) out[g[j]] = r[j];
     acc.seconds_tables += st.seconds_tables;
     acc.seconds_plan += st.seconds_plan;
@@ -751,10 +752,34 @@ std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* 
     acc.hits += st.hits;
     acc.factor_entries += st.factor_entries;
     acc.pairs += st.pairs;
+    acc.windows += st.windows;
+    acc.leaf_primes += st.leaf_primes;
+    acc.leaf_steps += st.leaf_steps;
+    acc.dense_primes += st.dense_primes;
+    acc.dense_steps += st.dense_steps;
+    acc.group_steps += st.group_steps;
   }
   acc.seconds_total = now() - t0;
   if (stats) *stats = acc;
   return out;
+}
+
+void hsl_validate(const HslQuery& q)
+{
+  validate(q);
+}
+
+void hsl_reserve(LeafSum kind, int64_t y, int64_t z)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_dev.init();
+  HslQuery q;
+  q.kind = kind;
+  q.x = 1;
+  q.y = y;
+  q.z = z;
+  validate(q);
+  (void) resident_tables({q});
 }
 
 bool device_available()

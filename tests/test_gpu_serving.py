@@ -1,0 +1,101 @@
+"""Serving context (pcg_ctx_*, continuous batching) on the device.
+
+Every answer is checked against the reference's own value for that x (the
+256-query fixture printed by the reference build, tests/golden/batch_D.json,
+and the default-parameter configs), whether the query ran alone or inside a
+mixed launch, submitted at once, in waves or from several threads.
+"""
+import threading
+import time
+
+import pytest
+
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def fixture_rows(limit):
+    return [(int(r["x"]), int(r["D"])) for r in golden("batch_D.json")["rows"]
+            if int(r["x"]) <= limit]
+
+
+def test_serving_batch_matches_reference(pc):
+    rows = fixture_rows(10 ** 16)[:64]
+    log("serving: %d queries in one submit" % len(rows))
+    with pc.Server(z_max=10 ** 7) as sv:
+        ids = sv.submit([(pc.KIND_D, x) for x, _ in rows])
+        res = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=120000)}
+        assert len(res) == len(ids)
+        for i, (x, want) in zip(ids, rows):
+            assert res[i]["status"] == 0, x
+            assert res[i]["value"] == want, x
+            assert res[i]["seconds"] > 0
+        st = sv.stats()
+        assert st["completed"] == len(ids)
+        assert st["batches"] < len(ids)          # queries shared launches
+
+
+def test_serving_waves_alone_and_mixed(pc):
+    rows = fixture_rows(10 ** 15)[:40]
+    with pc.Server() as sv:
+        # one query alone
+        x0, want0 = rows[0]
+        log("serving: one query alone")
+        i0 = sv.submit([(pc.KIND_D, x0)])[0]
+        r0 = sv.poll_all(1, timeout_ms=60000)
+        assert r0[0]["id"] == i0 and r0[0]["value"] == want0
+        # the same query again inside waves of others, arriving while
+        # launches run
+        log("serving: 4 waves")
+        ids = []
+        for w in range(4):
+            part = rows[w * 10:(w + 1) * 10]
+            ids += sv.submit([(pc.KIND_D, x) for x, _ in part])
+            time.sleep(0.005)
+        got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=120000)}
+        for i, (x, want) in zip(ids, rows):
+            assert got[i]["status"] == 0 and got[i]["value"] == want, x
+
+
+def test_serving_threads_and_errors(pc):
+    rows = fixture_rows(10 ** 15)[:48]
+    results = {}
+    lock = threading.Lock()
+    with pc.Server() as sv:
+        def client(part):
+            ids = sv.submit([(pc.KIND_D, x) for x, _ in part])
+            with lock:
+                for i, (x, want) in zip(ids, part):
+                    results[i] = want
+        log("serving: 4 client threads")
+        ts = [threading.Thread(target=client, args=(rows[t::4],)) for t in range(4)]
+        for t in ts:
+            t.start()
+        for t in ts:
+            t.join()
+        # a query outside the domain (z < y) is answered with an error and
+        # does not disturb the others
+        bad = sv.submit([(pc.KIND_D, 10 ** 12, 100, 50, 8)])[0]
+        got = {r["id"]: r for r in sv.poll_all(len(rows) + 1, timeout_ms=120000)}
+        assert got[bad]["status"] < 0
+        for i, want in results.items():
+            assert got[i]["status"] == 0 and got[i]["value"] == want
+
+
+def test_serving_S2_hard_and_explicit_parameters(pc):
+    cfg = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}
+    kat = [[int(v) for v in r] for r in golden("kat_D.json")["rows"]][:10]
+    with pc.Server() as sv:
+        log("serving: S2_hard defaults and explicit D parameters")
+        ids = sv.submit([(pc.KIND_S2_HARD, 10 ** 13), (pc.KIND_S2_HARD, 10 ** 15)])
+        ids += sv.submit([(pc.KIND_D, x, y, z, k) for x, y, z, k, _ in kat])
+        got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=120000)}
+        assert got[ids[0]]["value"] == int(cfg[10 ** 13]["value"])
+        assert got[ids[1]]["value"] == int(cfg[10 ** 15]["value"])
+        for i, row in zip(ids[2:], kat):
+            assert got[i]["status"] == 0 and got[i]["value"] == row[4], row

@@ -65,11 +65,14 @@ def test_version_and_no_silent_fallback(pc):
         pytest.skip("a GPU is present; the no-device path is covered on CPU hosts")
     with pytest.raises(pc.PrimecountError, match="HIP device|gfx950"):
         pc.D(10 ** 12, 76500, 76500, 8)
+    # a serving context cannot be created without the device either
+    with pytest.raises(pc.PrimecountError, match="HIP device|gfx950"):
+        pc.Server()
 
 
 def test_argument_errors(pc):
     with pytest.raises(pc.PrimecountError):
-        pc.set_segment_words(1000)            # not a power of two
+        pc.set_segment_words(1000)            # not a multiple of 128
     with pytest.raises(pc.PrimecountError):
         pc.set_host_threads(-1)
     pc.set_segment_words(0)
