@@ -152,6 +152,37 @@ int pcg_last_stats(pcg_stats* out);
  * factor tables that stay resident in HBM between calls. */
 int pcg_release(void);
 
+/* ---- the prime counting function ---------------------------------------
+ *
+ * pi(x) by Xavier Gourdon's algorithm, pi(x) = A - B + C + D + Phi0 + Sigma
+ * (src/gourdon/pi_gourdon.cpp:82-87), with the reference's default
+ * parameters; every term runs on the device (D on the hard-special-leaf
+ * engine, the others over a prime-count table resident in HBM).  Replaces
+ * pi(x) / primecount_pi (include/primecount.hpp:57,74, include/primecount.h:
+ * 51, 69 -> src/gourdon/pi_gourdon.cpp:22-155).  x <= 10^8 is answered from
+ * the prime-count table directly.  Domain: x / y < 2^50. */
+int pcg_pi(pcg_i128 x, pcg_i128* out);
+
+/* The other Gourdon terms with explicit parameters, as in
+ * include/gourdon.hpp:26-29 (64-bit) and 36-39 (128-bit): A + C
+ * (src/gourdon/AC.cpp), B (src/gourdon/B.cpp), Phi0 (src/gourdon/Phi0.cpp,
+ * device path for 5 <= k <= 8) and Sigma (src/gourdon/Sigma.cpp). */
+int pcg_AC(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);
+int pcg_B(pcg_i128 x, int64_t y, pcg_i128* out);
+int pcg_Phi0(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);
+int pcg_Sigma(pcg_i128 x, int64_t y, pcg_i128* out);
+
+typedef struct pcg_pi_stats {
+  double seconds_total;      /* wall time of the last pcg_pi call             */
+  double seconds_table;      /* tables, including the prime-count table       */
+  double seconds_sigma, seconds_phi0, seconds_ac, seconds_b, seconds_d;
+  uint64_t ac_leaves;        /* A and C leaves summed                         */
+  uint64_t table_limit;      /* the prime-count table covers [0, table_limit] */
+} pcg_pi_stats;
+
+/* Statistics of the last pcg_pi call on this thread. */
+int pcg_last_pi_stats(pcg_pi_stats* out);
+
 /* ---- serving: continuous batching ---------------------------------------
  *
  * A context owns a worker thread.  pcg_submit queues queries from any thread

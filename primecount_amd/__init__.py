@@ -84,6 +84,14 @@ class _CtxStats(ctypes.Structure):
                 ("busy_seconds", ctypes.c_double)]
 
 
+class PiStats(ctypes.Structure):
+    _fields_ = [("seconds_total", ctypes.c_double), ("seconds_table", ctypes.c_double),
+                ("seconds_sigma", ctypes.c_double), ("seconds_phi0", ctypes.c_double),
+                ("seconds_ac", ctypes.c_double), ("seconds_b", ctypes.c_double),
+                ("seconds_d", ctypes.c_double), ("ac_leaves", ctypes.c_uint64),
+                ("table_limit", ctypes.c_uint64)]
+
+
 KIND_D = 0
 KIND_S2_HARD = 1
 
@@ -124,6 +132,12 @@ def load():
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
+        "pcg_pi": ([_I128, PI128], ctypes.c_int),
+        "pcg_last_pi_stats": ([ctypes.POINTER(PiStats)], ctypes.c_int),
+        "pcg_AC": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
+        "pcg_Phi0": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_Sigma": ([_I128, I64, PI128], ctypes.c_int),
         "pcg_ctx_create": ([I64, I64, ctypes.c_uint64, ctypes.c_uint32,
                             ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "pcg_submit": ([ctypes.c_void_p, ctypes.POINTER(_Query), ctypes.c_size_t,
@@ -216,6 +230,43 @@ def D_batch(xs):
     out = (_I128 * max(n, 1))()
     _check(load().pcg_D_batch(arr, n, out))
     return [_from128(out[i]) for i in range(n)]
+
+
+def pi(x):
+    """pi(x): the number of primes <= x (Gourdon's algorithm on the device)."""
+    out = _I128()
+    _check(load().pcg_pi(_i128(x), ctypes.byref(out)))
+    return _from128(out)
+
+
+def last_pi_stats():
+    st = PiStats()
+    _check(load().pcg_last_pi_stats(ctypes.byref(st)))
+    return {name: getattr(st, name) for name, _ in st._fields_}
+
+
+def AC(x, y, z, k):
+    out = _I128()
+    _check(load().pcg_AC(_i128(x), int(y), int(z), int(k), ctypes.byref(out)))
+    return _from128(out)
+
+
+def B(x, y):
+    out = _I128()
+    _check(load().pcg_B(_i128(x), int(y), ctypes.byref(out)))
+    return _from128(out)
+
+
+def Phi0(x, y, z, k):
+    out = _I128()
+    _check(load().pcg_Phi0(_i128(x), int(y), int(z), int(k), ctypes.byref(out)))
+    return _from128(out)
+
+
+def Sigma(x, y):
+    out = _I128()
+    _check(load().pcg_Sigma(_i128(x), int(y), ctypes.byref(out)))
+    return _from128(out)
 
 
 def gourdon_params(x):

@@ -29,6 +29,8 @@ HOST = ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "includ
 SOURCES = [
     # (file, extra flags, has device code)
     ("hsl_kernels.hip", [], True),
+    ("pix_kernels.hip", [], True),
+    ("pix.cpp", [], False),
     ("engine.cpp", [], False),
     ("tables.cpp", [], False),
     # exact IEEE double evaluation of the alpha formulas: no fused multiply-add

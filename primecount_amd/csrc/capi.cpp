@@ -8,6 +8,7 @@
 
 #include "engine.hpp"
 #include "params.hpp"
+#include "pix.hpp"
 #include "serve.hpp"
 
 #include <exception>
@@ -20,6 +21,7 @@ namespace {
 
 thread_local std::string t_error;
 thread_local pcg_stats t_stats{};
+thread_local pcg_pi_stats t_pi_stats{};
 
 i128 from_abi(pcg_i128 v)
 {
@@ -167,6 +169,64 @@ int pcg_dr_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* c)
     if (y) *y = p.y;
     if (z) *z = p.z;
     if (c) *c = p.c;
+  });
+}
+
+int pcg_pi(pcg_i128 x, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    PixStats st;
+    *out = to_abi(pi_gourdon(from_abi(x), &st));
+    t_pi_stats.seconds_total = st.seconds_total;
+    t_pi_stats.seconds_table = st.seconds_table;
+    t_pi_stats.seconds_sigma = st.seconds_sigma;
+    t_pi_stats.seconds_phi0 = st.seconds_phi0;
+    t_pi_stats.seconds_ac = st.seconds_ac;
+    t_pi_stats.seconds_b = st.seconds_b;
+    t_pi_stats.seconds_d = st.seconds_d;
+    t_pi_stats.ac_leaves = st.ac_leaves;
+    t_pi_stats.table_limit = st.table_limit;
+  });
+}
+
+int pcg_last_pi_stats(pcg_pi_stats* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = t_pi_stats;
+  });
+}
+
+int pcg_AC(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(gourdon_term(GourdonTerm::AC, from_abi(x), y, z, k, nullptr));
+  });
+}
+
+int pcg_B(pcg_i128 x, int64_t y, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(gourdon_term(GourdonTerm::B, from_abi(x), y, y, 0, nullptr));
+  });
+}
+
+int pcg_Phi0(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(gourdon_term(GourdonTerm::Phi0, from_abi(x), y, z, k, nullptr));
+  });
+}
+
+int pcg_Sigma(pcg_i128 x, int64_t y, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(gourdon_term(GourdonTerm::Sigma, from_abi(x), y, y, 0, nullptr));
   });
 }
 

@@ -11,6 +11,7 @@
 
 #include "engine.hpp"
 #include "hsl_kernels.hpp"
+#include "pix_kernels.hpp"
 #include "params.hpp"
 #include "tables.hpp"
 
@@ -109,6 +110,9 @@ struct Device {
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
   DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
   DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans;
+  DevBuf pix_bits, pix_cnt, pix_seg;  // prime-count table over [0, pix_n]
+  uint64_t pix_n = 0;
+  DevBuf extra[8];                    // scratch of the other pi(x) terms
   HostBuf stage[2];                   // per pass: spans, columns
 
   hipEvent_t event(size_t i)
@@ -149,6 +153,11 @@ struct Device {
       b->release();
     for (HostBuf* b : {&stage[0], &stage[1]})
       b->release();
+    for (DevBuf* b : {&pix_bits, &pix_cnt, &pix_seg})
+      b->release();
+    for (DevBuf& b : extra)
+      b.release();
+    pix_n = 0;
   }
 };
 
@@ -793,9 +802,86 @@ bool device_available()
   }
 }
 
+std::unique_lock<std::mutex> engine_lock()
+{
+  return std::unique_lock<std::mutex>(g_mutex);
+}
+
+void* engine_buffer(int slot, size_t bytes)
+{
+  if (slot < 0 || slot >= 8) throw pcg_error("internal: engine buffer slot");
+  return g_dev.extra[slot].ensure(bytes);
+}
+
+/// Prime-count table over [0, n] built on the device: segmented sieve of
+/// the numbers coprime to 30 with the primes 7 .. isqrt(n) (from the resident
+/// prime table), then the per-word counts.  Prefix valid, so a larger table
+/// serves every smaller query.
+static void ensure_pix_table(uint64_t n)
+{
+  Device& D = g_dev;
+  if (D.pix_n >= n) return;
+  n = std::max<uint64_t>(n, 1u << 20);
+  const TableSet& T = g_tables;
+  const uint64_t r = isqrt(n);
+  if (r > T.P.max) throw pcg_error("internal: prime table too small for the prime-count table");
+  const uint64_t nwords = n / 240 + 1;
+  const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
+  D.pix_n = 0;
+  D.pix_bits.ensure(nwords * 8);
+  D.pix_cnt.ensure(nwords * 8);
+  D.pix_seg.ensure(nseg * 8);
+  const uint32_t j_end = (uint32_t) T.P.pi(r) + 1;               // primes[4 .. pi(r)]
+  const uint32_t j_small = (uint32_t) std::min<uint64_t>(T.P.pi(std::min<uint64_t>(r, 240ull * kPixSegWords / 64)) + 1, j_end);
+  hipStream_t s = D.stream;
+  PCG_HIP(pix_launch_sieve((uint64_t*) D.pix_bits.p, (uint64_t*) D.pix_seg.p,
+                           (const uint32_t*) D.primes.p, std::max<uint32_t>(j_small, 4),
+                           std::max<uint32_t>(j_end, 4), nwords, n, s));
+  PCG_HIP(pix_launch_scan((uint64_t*) D.pix_seg.p, (uint32_t) nseg, s));
+  PCG_HIP(pix_launch_counts((const uint64_t*) D.pix_bits.p, (const uint64_t*) D.pix_seg.p,
+                            (uint64_t*) D.pix_cnt.p, nwords, s));
+  PCG_HIP(hipStreamSynchronize(s));
+  D.pix_n = nwords * 240 - 1 < n ? nwords * 240 - 1 : n;
+}
+
+EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n)
+{
+  Device& D = g_dev;
+  D.init();
+  HslQuery q;
+  q.kind = LeafSum::D;
+  q.x = 1;
+  q.y = y;
+  q.z = z;
+  validate(q);
+  const TableSet& T = resident_tables({q});
+  if (pix_n) ensure_pix_table(pix_n);
+  EngineView v;
+  v.stream = D.stream;
+  v.cus = D.cus;
+  v.primes = (const uint32_t*) D.primes.p;
+  v.nprimes = T.P.primes.size();
+  v.host = &T.P;
+  v.factor = D.factor.p;
+  v.factor32 = T.factor32;
+  v.factor_n = T.factor32 ? T.f32.size() : T.f16.size();
+  v.pix_bits = (const uint64_t*) D.pix_bits.p;
+  v.pix_cnt = (const uint64_t*) D.pix_cnt.p;
+  v.pix_n = D.pix_n;
+  return v;
+}
+
+static uint64_t g_generation = 1;
+
+uint64_t engine_generation()
+{
+  return g_generation;
+}
+
 void release_device()
 {
   std::lock_guard<std::mutex> lock(g_mutex);
+  g_generation++;
   g_dev.release();
   g_tables = TableSet();
   g_tables_valid = false;

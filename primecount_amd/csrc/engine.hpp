@@ -4,6 +4,9 @@
 #include "pc_math.hpp"
 #include "../../include/primecount_mi355x.h"
 
+#include <hip/hip_runtime_api.h>
+
+#include <mutex>
 #include <vector>
 
 namespace pcg {
@@ -46,5 +49,38 @@ void hsl_reserve(LeafSum kind, int64_t y, int64_t z);
 
 bool device_available();
 void release_device();
+
+// ---- shared with the other pi(x) terms (pix.cpp) --------------------------
+
+struct PrimeTables;
+
+/// Device-resident data of the engine.  Valid while the caller holds the
+/// lock returned by engine_lock().
+struct EngineView {
+  hipStream_t stream = nullptr;
+  int cus = 0;
+  const uint32_t* primes = nullptr;     // primes[0] = 0, primes[1] = 2, ...
+  uint64_t nprimes = 0;                 // entries of primes[]
+  const PrimeTables* host = nullptr;    // host copy of the prime tables
+  const void* factor = nullptr;         // FactorTableD entries (D tables)
+  bool factor32 = false;
+  uint64_t factor_n = 0;                // entries of factor[]
+  const uint64_t* pix_bits = nullptr;   // prime-count table (see pix_kernels.hpp)
+  const uint64_t* pix_cnt = nullptr;
+  uint64_t pix_n = 0;                   // it covers [0, pix_n]
+};
+
+std::unique_lock<std::mutex> engine_lock();
+
+/// Makes the D tables for (y, z) (y >= z shares one prefix-valid table) and
+/// a prime-count table covering [0, pix_n] resident; caller holds the lock.
+EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n);
+
+/// A device buffer of at least `bytes`, kept between calls (slot 0 .. 7);
+/// caller holds the lock.
+void* engine_buffer(int slot, size_t bytes);
+
+/// Changes whenever release_device() frees the device buffers.
+uint64_t engine_generation();
 
 } // namespace pcg

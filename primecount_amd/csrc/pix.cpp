@@ -1,0 +1,401 @@
+// pix.cpp -- host orchestration of Gourdon's pi(x) on the device.
+//
+// The easy-leaf sums of A and C (src/gourdon/AC.cpp) are cut into one
+// descriptor per sieving prime (or per pair of primes for the m = q r leaves
+// of C1) whose leaves are consecutive primes; the flat leaf-sum kernel then
+// reads pi(x / (p m)) from the resident prime-count table.  The reference
+// walks the same leaves segment by segment with a small pi table
+// (SegmentedPiTable, x^(1/4) memory) and clusters equal pi values; here the
+// whole table up to max(x / y, sqrt(x)) lives in HBM (7.5 GB at x = 1e19),
+// which turns every leaf into two loads.
+
+#include "pix.hpp"
+
+#include "hsl_kernels.hpp"
+#include "params.hpp"
+#include "pix_kernels.hpp"
+#include "tables.hpp"
+
+#include <hip/hip_runtime_api.h>
+
+#include <chrono>
+#include <cstring>
+#include <map>
+
+namespace pcg {
+
+#define PIX_HIP(expr)                                                              \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      throw pcg_error(std::string("HIP error '") + hipGetErrorString(e_) +         \
+                      "' in " #expr);                                              \
+  } while (0)
+
+namespace {
+
+double now_s()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+uint64_t umin(uint64_t a, uint64_t b) { return a < b ? a : b; }
+uint64_t umax(uint64_t a, uint64_t b) { return a > b ? a : b; }
+
+/// Largest n the terms of x read pi(n) for.
+uint64_t table_need(i128 x, int64_t y, int64_t x_star)
+{
+  u128 n = (u128) x / (u128) y;
+  n = std::max<u128>(n, isqrt((u128) x));
+  n = std::max<u128>(n, (u128) x / ((u128) x_star * (u128) y));
+  n = std::max<u128>(n, isqrt((u128) x / (u128) x_star));
+  if (n >= ((u128) 1 << 50))
+    throw pcg_error("pi(x): x / y must be < 2^50 (prime-count table limit)");
+  return (uint64_t) n + 1;
+}
+
+i128 acc_value(const uint64_t* a)
+{
+  return (i128) (((u128) a[1] << 64) | a[0]);
+}
+
+/// Accumulators of one run: 8 signed 128-bit sums in device memory.
+struct Acc {
+  unsigned long long* d = nullptr;
+  uint64_t h[16] = {};
+};
+
+void acc_zero(Acc& A, hipStream_t s)
+{
+  PIX_HIP(hsl_launch_fill_zero(A.d, 128, s));
+}
+
+void acc_read(Acc& A, hipStream_t s)
+{
+  PIX_HIP(hipMemcpyAsync(A.h, A.d, 128, hipMemcpyDeviceToHost, s));
+  PIX_HIP(hipStreamSynchronize(s));
+}
+
+/// pi(n) for a few n, on the device.
+std::vector<uint64_t> pi_points(const EngineView& v, const std::vector<uint64_t>& ns)
+{
+  for (uint64_t n : ns)
+    if (n > v.pix_n) throw pcg_error("internal: prime-count table too small");
+  uint64_t* dn = (uint64_t*) engine_buffer(1, ns.size() * 8);
+  uint64_t* dout = (uint64_t*) engine_buffer(2, ns.size() * 8);
+  PIX_HIP(hipMemcpyAsync(dn, ns.data(), ns.size() * 8, hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_pi_points(dn, dout, (uint32_t) ns.size(), v.pix_bits, v.pix_cnt, v.stream));
+  std::vector<uint64_t> out(ns.size());
+  PIX_HIP(hipMemcpyAsync(out.data(), dout, ns.size() * 8, hipMemcpyDeviceToHost, v.stream));
+  PIX_HIP(hipStreamSynchronize(v.stream));
+  return out;
+}
+
+/// phi(r, k) for r < pp (pp = product of the first k primes) and phi(pp, k):
+/// the tiny phi of src/PhiTiny.hpp, tabulated once per k and kept on the
+/// device.
+struct PhiTiny {
+  uint32_t k = 0, pp = 1;
+  uint64_t phi_pp = 1;
+  std::vector<uint32_t> tbl;
+};
+
+const PhiTiny& phi_tiny_table(uint32_t k)
+{
+  static std::map<uint32_t, PhiTiny> cache;
+  auto it = cache.find(k);
+  if (it != cache.end()) return it->second;
+  static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
+  PhiTiny t;
+  t.k = k;
+  for (uint32_t i = 0; i < k; i++) {
+    t.pp *= small[i];
+    t.phi_pp *= small[i] - 1;
+  }
+  t.tbl.resize(t.pp);
+  uint32_t c = 0;
+  for (uint32_t r = 0; r < t.pp; r++) {
+    bool ok = r > 0;
+    for (uint32_t i = 0; i < k && ok; i++)
+      ok = r % small[i] != 0;
+    c += ok;
+    t.tbl[r] = c;                      // phi(r, k): 1 <= m <= r coprime to pp
+  }
+  return cache.emplace(k, std::move(t)).first->second;
+}
+
+i128 phi_tiny_host(i128 x, const PhiTiny& t)
+{
+  const u128 u = (u128) x;
+  return (i128) ((u / t.pp) * t.phi_pp + t.tbl[(uint32_t) (u % t.pp)]);
+}
+
+i128 run_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v)
+{
+  const uint64_t x13 = iroot<3>((u128) x);
+  const uint64_t sqrt_xy = isqrt((u128) x / (u128) y);
+  const std::vector<uint64_t> p = pi_points(v, {(uint64_t) y, x13, sqrt_xy, (uint64_t) x_star,
+                                                isqrt((u128) x)});
+  const i128 a = p[0], b = p[1], c = p[2], d = p[3], pi_sqrtx = p[4];
+  // Sigma0 .. Sigma3 (src/gourdon/Sigma.cpp:30-52)
+  const i128 s0 = a - 1 + (pi_sqrtx * (pi_sqrtx - 1)) / 2 - (a * (a - 1)) / 2;
+  const i128 s1 = (a - b) * (a - b - 1) / 2;
+  const i128 s2 = a * (b - c - (c * (c - 3)) / 2 + (d * (d - 3)) / 2);
+  const i128 s3 = (b * (b - 1) * (2 * b - 1)) / 6 - b - (d * (d - 1) * (2 * d - 1)) / 6 + d;
+  // Sigma4 .. Sigma6 over the primes x_star < p <= x^(1/3)
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  const uint32_t b_lo = (uint32_t) v.host->pi((uint64_t) x_star) + 1;
+  const uint32_t b_hi = (uint32_t) v.host->pi(x13) + 1;
+  if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma");
+  PIX_HIP(pix_launch_sigma((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, sqrt_xy, v.primes,
+                           b_lo, b_hi, v.pix_bits, v.pix_cnt, A.d, v.stream));
+  acc_read(A, v.stream);
+  return s0 + s1 + s2 + s3 + acc_value(A.h) * a + acc_value(A.h + 2) - acc_value(A.h + 4);
+}
+
+i128 run_phi0(i128 x, int64_t y, int64_t z, int64_t k, const EngineView& v)
+{
+  if (k < 5 || k > 8)
+    throw pcg_error("Phi0(x, y, z, k): the device path needs 5 <= k <= 8");
+  static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
+  const PhiTiny& t = phi_tiny_table((uint32_t) k);
+  uint32_t* dtbl = (uint32_t*) engine_buffer(3, (size_t) t.pp * 4);
+  static const void* uploaded = nullptr;     // the device copy stays until the buffer moves
+  static uint32_t uploaded_k = 0;
+  static uint64_t uploaded_gen = 0;
+  if (dtbl != uploaded || uploaded_k != (uint32_t) k || uploaded_gen != engine_generation()) {
+    PIX_HIP(hipMemcpyAsync(dtbl, t.tbl.data(), (size_t) t.pp * 4, hipMemcpyHostToDevice, v.stream));
+    PIX_HIP(hipStreamSynchronize(v.stream));
+    uploaded = dtbl;
+    uploaded_k = (uint32_t) k;
+    uploaded_gen = engine_generation();
+  }
+  const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) z) + 1;
+  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for Phi0");
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, nidx,
+                          (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, A.d, v.stream));
+  acc_read(A, v.stream);
+  return phi_tiny_host(x, t) + acc_value(A.h);
+}
+
+i128 run_B(i128 x, int64_t y, const EngineView& v)
+{
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  PIX_HIP(pix_launch_B((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, isqrt((u128) x),
+                       v.pix_bits, v.pix_cnt, A.d, v.stream));
+  acc_read(A, v.stream);
+  return acc_value(A.h);
+}
+
+/// Descriptor lists of the A and C leaves (AC.cpp, as one segment [0, isqrt(x))).
+struct LeafLists {
+  std::vector<PixLeafDesc> plus, minus;
+  uint64_t n_plus = 0, n_minus = 0;
+  i128 correction = 0;       // the (2 - b) parts of C, summed on the host
+};
+
+void add_desc(std::vector<PixLeafDesc>& list, uint64_t& total, uint64_t xp, uint64_t q1,
+              uint64_t i_lo, uint64_t i_hi)
+{
+  if (i_hi < i_lo) return;
+  PixLeafDesc d;
+  d.xp = xp;
+  d.q1 = q1;
+  d.off = total;
+  d.i_lo = (uint32_t) i_lo;
+  d.pad = 0;
+  list.push_back(d);
+  total += i_hi - i_lo + 1;
+}
+
+LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P)
+{
+  LeafLists L;
+  const u128 ux = (u128) x;
+  const uint64_t sqrtx = isqrt(ux);
+  const uint64_t x13 = iroot<3>(ux);
+  const u128 xhigh = ux / sqrtx;                 // one segment [0, isqrt(x))
+  const uint64_t uy = (uint64_t) y, uz = (uint64_t) z;
+  auto pi = [&](uint64_t n) -> uint64_t {
+    if (n > P.max) throw pcg_error("internal: prime table too small for A/C");
+    return P.pi(n);
+  };
+  auto xp_of = [&](uint64_t p) -> uint64_t {
+    const u128 xp = ux / p;
+    if (xp >> 64) throw pcg_error("A/C: x / p does not fit in 64 bits");
+    return (uint64_t) xp;
+  };
+
+  // A: pi[x_star] < b <= pi[x^(1/3)] (AC.cpp:57-113, 456-465)
+  for (uint64_t b = pi((uint64_t) x_star) + 1; b <= pi(x13); b++) {
+    const uint64_t p = P.primes[b];
+    const uint64_t xp = xp_of(p);
+    const uint64_t sqrt_xp = isqrt(xp);
+    const uint64_t min2 = umin((uint64_t) (xhigh / p), sqrt_xp);
+    add_desc(L.plus, L.n_plus, xp, xp / uy, pi(umax(p, min2)) + 1, pi(sqrt_xp));
+  }
+  // C1: max(k, pi[(x/z)^(1/3)]) < b <= pi[sqrt(z)] (AC.cpp:115-214, 402-418)
+  const uint64_t sqrtz = isqrt((u128) uz);
+  const uint64_t xz = (uint64_t) (ux / uz);
+  for (uint64_t b = umax((uint64_t) k, pi(iroot<3>((u128) xz))) + 1; b <= pi(sqrtz); b++) {
+    const uint64_t p = P.primes[b];
+    const uint64_t xp = xp_of(p);
+    const uint64_t max_m = umin(xp, uz);
+    const uint64_t min_m = umax(umax((uint64_t) (xhigh / p), xp / (p * p)), uz / p);
+    if (min_m >= max_m) continue;
+    const uint64_t max_prime = umin(uy, max_m);
+    if (min_m < max_prime) {
+      const uint64_t lo = pi(min_m) + 1, hi = pi(max_prime);
+      add_desc(L.plus, L.n_plus, xp, ~0ull, lo, hi);
+      if (hi >= lo) L.correction += (i128) (hi - lo + 1) * (2 - (i128) b);
+    }
+    const uint64_t max_q = umin(uy, isqrt((u128) max_m));
+    const uint64_t min_q = umax(p, min_m / uy);
+    if (min_q < max_q) {
+      for (uint64_t i = pi(min_q) + 1; i <= pi(max_q); i++) {
+        const uint64_t q = P.primes[i];
+        const uint64_t min_r = umax(q, min_m / q);
+        const uint64_t max_r = umin(uy, max_m / q);
+        if (min_r >= max_r) continue;
+        const uint64_t lo = pi(min_r) + 1, hi = pi(max_r);
+        add_desc(L.minus, L.n_minus, xp / q, ~0ull, lo, hi);
+        if (hi >= lo) L.correction -= (i128) (hi - lo + 1) * (2 - (i128) b);
+      }
+    }
+  }
+  // C2: max(k, pi[(x/y)^(1/3)], pi[sqrt(z)]) < b <= pi[x_star] (AC.cpp:216-346, 420-454)
+  const uint64_t xy = (uint64_t) (ux / uy);
+  const uint64_t min_c2 = umax(umax((uint64_t) k, pi(iroot<3>((u128) xy))), pi(sqrtz));
+  for (uint64_t b = min_c2 + 1; b <= pi((uint64_t) x_star); b++) {
+    const uint64_t p = P.primes[b];
+    const uint64_t xp = xp_of(p);
+    const uint64_t max_m = umin(xp / p, uy);
+    const uint64_t min_m = umax(umax((uint64_t) (xhigh / p), xp / (p * p)), p);
+    if (min_m >= max_m) continue;
+    const uint64_t lo = pi(min_m) + 1, hi = pi(max_m);
+    add_desc(L.plus, L.n_plus, xp, ~0ull, lo, hi);
+    if (hi >= lo) L.correction += (i128) (hi - lo + 1) * (2 - (i128) b);
+  }
+  return L;
+}
+
+i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const EngineView& v,
+            uint64_t* leaves)
+{
+  LeafLists L = ac_lists(x, y, z, k, x_star, *v.host);
+  *leaves = L.n_plus + L.n_minus;
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  const size_t bp = L.plus.size() * sizeof(PixLeafDesc);
+  const size_t bm = L.minus.size() * sizeof(PixLeafDesc);
+  char* dd = (char*) engine_buffer(4, bp + bm + 64);
+  PixLeafDesc* dplus = (PixLeafDesc*) dd;
+  PixLeafDesc* dminus = (PixLeafDesc*) (dd + bp);
+  if (bp) PIX_HIP(hipMemcpyAsync(dplus, L.plus.data(), bp, hipMemcpyHostToDevice, v.stream));
+  if (bm) PIX_HIP(hipMemcpyAsync(dminus, L.minus.data(), bm, hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_bits,
+                             v.pix_cnt, A.d, (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_bits,
+                             v.pix_cnt, A.d + 2, (uint32_t) v.cus, v.stream));
+  acc_read(A, v.stream);
+  return acc_value(A.h) - acc_value(A.h + 2) + L.correction;
+}
+
+void check_params(i128 x, int64_t y, int64_t z, int64_t k)
+{
+  if (x < 1 || y < 1 || z < y)
+    throw pcg_error("Gourdon terms need x >= 1 and 1 <= y <= z");
+  if (k < 0 || k > 8)
+    throw pcg_error("k must be in [0, 8]");
+  if (x > ((i128) y << 62))
+    throw pcg_error("x is too large (x > y * 2^62)");
+  if ((u128) z * (u128) z > (u128) x && z > 1)
+    throw pcg_error("Gourdon terms need z < sqrt(x)");
+}
+
+} // namespace
+
+i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st)
+{
+  check_params(x, y, z, k);
+  const int64_t x_star = get_x_star_gourdon(x, y);
+  const double t0 = now_s();
+  auto lock = engine_lock();
+  const EngineView v = engine_view(y, z, table_need(x, y, x_star));
+  const double t1 = now_s();
+  i128 r = 0;
+  uint64_t leaves = 0;
+  switch (term) {
+    case GourdonTerm::AC: r = run_ac(x, y, z, k, x_star, v, &leaves); break;
+    case GourdonTerm::B: r = run_B(x, y, v); break;
+    case GourdonTerm::Phi0: r = run_phi0(x, y, z, k, v); break;
+    case GourdonTerm::Sigma: r = run_sigma(x, y, x_star, v); break;
+  }
+  if (st) {
+    *st = PixStats();
+    st->seconds_table = t1 - t0;
+    st->seconds_total = now_s() - t0;
+    st->ac_leaves = leaves;
+    st->table_limit = v.pix_n;
+  }
+  return r;
+}
+
+i128 pi_gourdon(i128 x, PixStats* st)
+{
+  PixStats s;
+  const double t0 = now_s();
+  i128 result = 0;
+  if (x < 2) {
+    result = 0;
+  } else if (x <= (i128) 100000000) {
+    // small x: the prime-count table answers directly
+    auto lock = engine_lock();
+    // (the table is built for at least 2^20 numbers: primes to sqrt of that)
+    const int64_t r = (int64_t) isqrt((u128) std::max<i128>(x, (i128) 1 << 20)) + 1;
+    const EngineView v = engine_view(r, r, (uint64_t) x);
+    s.seconds_table = now_s() - t0;
+    result = (i128) pi_points(v, {(uint64_t) x})[0];
+    s.table_limit = v.pix_n;
+  } else {
+    const GourdonParams gp = gourdon_params(x);
+    check_params(x, gp.y, gp.z, gp.k);
+    {
+      auto lock = engine_lock();
+      const EngineView v = engine_view(gp.y, gp.z, table_need(x, gp.y, gp.x_star));
+      double t = now_s();
+      s.seconds_table = t - t0;
+      s.table_limit = v.pix_n;
+      result += run_sigma(x, gp.y, gp.x_star, v);
+      double t2 = now_s(); s.seconds_sigma = t2 - t; t = t2;
+      result += run_phi0(x, gp.y, gp.z, gp.k, v);
+      t2 = now_s(); s.seconds_phi0 = t2 - t; t = t2;
+      result += run_ac(x, gp.y, gp.z, gp.k, gp.x_star, v, &s.ac_leaves);
+      t2 = now_s(); s.seconds_ac = t2 - t; t = t2;
+      result -= run_B(x, gp.y, v);
+      s.seconds_b = now_s() - t;
+    }
+    const double t = now_s();
+    HslQuery q;
+    q.kind = LeafSum::D;
+    q.x = x;
+    q.y = gp.y;
+    q.z = gp.z;
+    q.k = gp.k;
+    result += hsl_compute(q, nullptr);
+    s.seconds_d = now_s() - t;
+  }
+  s.seconds_total = now_s() - t0;
+  if (st) *st = s;
+  return result;
+}
+
+} // namespace pcg

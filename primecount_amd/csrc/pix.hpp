@@ -1,0 +1,27 @@
+// pix.hpp -- Gourdon's prime counting function pi(x) on the device:
+//   pi(x) = A - B + C + D + Phi0 + Sigma   (src/gourdon/pi_gourdon.cpp:82-87)
+// with D from the hard-special-leaf engine and the other terms from the
+// kernels of pix_kernels.hip over a prime-count table resident in HBM.
+#pragma once
+
+#include "engine.hpp"
+
+namespace pcg {
+
+enum class GourdonTerm { AC, B, Phi0, Sigma };
+
+struct PixStats {
+  double seconds_total = 0, seconds_table = 0, seconds_sigma = 0, seconds_phi0 = 0,
+         seconds_ac = 0, seconds_b = 0, seconds_d = 0;
+  uint64_t ac_leaves = 0;     // A and C terms summed
+  uint64_t table_limit = 0;   // prime-count table covers [0, table_limit]
+};
+
+/// One term with explicit parameters (src/gourdon/AC.cpp, B.cpp, Phi0.cpp,
+/// Sigma.cpp signatures of include/gourdon.hpp:26-29,36-39).  Throws pcg_error.
+i128 gourdon_term(GourdonTerm t, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st);
+
+/// pi(x) with the reference's default parameters.  Throws pcg_error.
+i128 pi_gourdon(i128 x, PixStats* st);
+
+} // namespace pcg

@@ -1,0 +1,491 @@
+// pix_kernels.hip -- CDNA4 (gfx950) kernels for the terms of Gourdon's pi(x)
+// other than D: the prime-count table, the easy leaves of A and C, B, Phi0
+// and the Sigma sums.  See pix_kernels.hpp for the table layout.
+
+#include <hip/hip_runtime.h>
+
+#include "pix_kernels.hpp"
+#include "wheel_dev.hpp"
+
+#include <algorithm>
+
+namespace pcg {
+
+namespace {
+
+constexpr uint32_t kThreads = 256;
+typedef unsigned __int128 u128;
+typedef __int128 i128;
+
+/// pi(n) from the resident table.
+__device__ __forceinline__ uint64_t pix_pi(const uint64_t* __restrict__ bits,
+                                           const uint64_t* __restrict__ cnt, uint64_t n)
+{
+  if (n < 7)
+    return n < 2 ? 0 : (n < 3 ? 1 : (n < 5 ? 2 : 3));
+  const uint64_t w = n / 240;
+  const uint32_t r = (uint32_t) (n - w * 240);
+  return 3 + cnt[w] + (uint64_t) __popcll(bits[w] & le_mask(r));
+}
+
+/// Adds a signed 128-bit value to a (lo, hi) pair with two 64-bit atomics; the
+/// add whose low word wraps carries into the high word.
+__device__ __forceinline__ void atomic_add_i128(unsigned long long* acc, i128 v)
+{
+  const unsigned long long lo = (unsigned long long) v;
+  const unsigned long long hi = (unsigned long long) (v >> 64);
+  const unsigned long long old = atomicAdd(&acc[0], lo);
+  const unsigned long long h = hi + (old + lo < old ? 1ull : 0ull);
+  if (h != 0)
+    atomicAdd(&acc[1], h);
+}
+
+/// Block-wide sum of a signed 128-bit value, added to acc by thread 0.
+__device__ __forceinline__ void block_add_i128(unsigned long long* acc, i128 v)
+{
+  __shared__ i128 red[kThreads];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (uint32_t s = kThreads / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && red[0] != 0)
+    atomic_add_i128(acc, red[0]);
+}
+
+/// floor(v / d) and v mod d for v < 2^128, 0 < d < 2^32 (32-bit limbs).
+__device__ __forceinline__ u128 divmod128_32(u128 v, uint32_t d, uint32_t* rem)
+{
+  u128 q = 0;
+  uint64_t r = 0;
+  for (int s = 96; s >= 0; s -= 32) {
+    const uint64_t cur = (r << 32) | (uint32_t) (v >> s);
+    const uint64_t ql = cur / d;
+    r = cur - ql * d;
+    q |= (u128) ql << s;
+  }
+  *rem = (uint32_t) r;
+  return q;
+}
+
+/// floor(a / d) for a < 2^64, d < 2^32, from a double estimate and an exact
+/// integer correction.
+__device__ __forceinline__ uint64_t div64_32(uint64_t a, uint32_t d)
+{
+  uint64_t q = (uint64_t) ((double) a / (double) d);
+  // the estimate is within a few units; step until a - q d is in [0, d)
+  while ((u128) q * d > a)
+    q--;
+  while ((u128) (q + 1) * d <= a)
+    q++;
+  return q;
+}
+
+/// floor(sqrt(n)) for n < 2^64.
+__device__ __forceinline__ uint64_t isqrt64(uint64_t n)
+{
+  uint64_t r = (uint64_t) sqrt((double) n);
+  while ((u128) r * r > n)
+    r--;
+  while ((u128) (r + 1) * (r + 1) <= n)
+    r++;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// prime-count table
+
+/// One workgroup sieves one segment of kPixSegWords words in LDS: the numbers
+/// coprime to 30 in [low, high) (high <= N + 1), without 1, then the multiples
+/// p * q (q >= p, q coprime to 30) of the primes p_4 = 7 .. p_{j_end - 1}.
+/// Small primes are crossed off by all threads together (8 residue classes
+/// of q, 32 threads per class), the others by one thread each.  Writes the
+/// bits and the segment's prime count.
+__global__ __launch_bounds__(kThreads) void pix_sieve_kernel(
+    uint64_t* __restrict__ bits, uint64_t* __restrict__ seg_tot,
+    const uint32_t* __restrict__ primes, uint32_t j_small_end, uint32_t j_end,
+    uint64_t nwords, uint64_t N)
+{
+  __shared__ uint32_t bm[2 * kPixSegWords];
+  __shared__ unsigned long long red[kThreads];
+  const uint32_t t = threadIdx.x;
+  const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
+  const uint32_t nw = (uint32_t) min<uint64_t>(kPixSegWords, nwords - w0);
+  const uint64_t low = w0 * 240;
+  const uint64_t high = min(low + 240ull * nw, N + 1);
+  for (uint32_t w = t; w < nw; w += kThreads) {
+    const uint64_t base = low + 240ull * w;
+    uint64_t v = base + 240 <= high ? ~0ull : (base >= high ? 0ull : le_mask((uint32_t) (high - base - 1)));
+    if (base == 0)
+      v &= ~1ull;                                    // 1 is not prime
+    bm[2 * w] = (uint32_t) v;
+    bm[2 * w + 1] = (uint32_t) (v >> 32);
+  }
+  __syncthreads();
+  if (high > low) {
+    const uint32_t cls = t & 7;
+    const uint32_t t0 = t >> 3;
+    for (uint32_t j = 4; j < j_small_end; j++) {
+      const uint64_t p = primes[j];
+      const uint64_t q_lo = max(p, div_exact((double) (low + p - 1), low + p - 1, p));
+      const uint64_t q_hi = div_exact((double) (high - 1), high - 1, p);
+      if (q_lo > q_hi)
+        continue;
+      uint64_t q = q_lo - q_lo % 30 + ((kWheelBytes >> (8 * cls)) & 0xff);
+      if (q < q_lo)
+        q += 30;
+      if (q > q_hi)
+        continue;
+      const uint32_t n = (uint32_t) ((q_hi - q) / 30) + 1;
+      const uint32_t off = (uint32_t) (p * q - low);
+      const uint32_t byte0 = off / 30;
+      const uint32_t bib = wheel_bit(off - byte0 * 30);
+      for (uint32_t k = t0; k < n; k += kThreads / 8) {
+        const uint32_t B = byte0 + k * (uint32_t) p;
+        atomicAnd(&bm[B >> 2], ~(1u << (((B & 3) << 3) | bib)));
+      }
+    }
+    for (uint32_t j = j_small_end + t; j < j_end; j += kThreads) {
+      const uint64_t p = primes[j];
+      const uint64_t q_lo = max(p, div_exact((double) (low + p - 1), low + p - 1, p));
+      const uint64_t q_hi = div_exact((double) (high - 1), high - 1, p);
+      if (q_lo > q_hi)
+        continue;
+      // walk the cofactors coprime to 30 in [q_lo, q_hi] by wheel index
+      const uint32_t qm = (uint32_t) (q_lo % 30);
+      const uint64_t qbase = q_lo - qm;
+      for (uint32_t idx = qm ? le_count(qm - 1) : 0u;; idx++) {
+        const uint64_t q = qbase + 30ull * (idx >> 3) + ((kWheelBytes >> (8 * (idx & 7))) & 0xff);
+        if (q > q_hi)
+          break;
+        const uint32_t off = (uint32_t) (p * q - low);
+        const uint32_t B = off / 30;
+        atomicAnd(&bm[B >> 2], ~(1u << (((B & 3) << 3) | wheel_bit(off - B * 30))));
+      }
+    }
+  }
+  __syncthreads();
+  unsigned long long tot = 0;
+  for (uint32_t w = t; w < nw; w += kThreads) {
+    const uint64_t v = ((uint64_t) bm[2 * w + 1] << 32) | bm[2 * w];
+    bits[w0 + w] = v;
+    tot += (unsigned long long) __popcll(v);
+  }
+  red[t] = tot;
+  __syncthreads();
+  for (uint32_t s = kThreads / 2; s > 0; s >>= 1) {
+    if (t < s)
+      red[t] += red[t + s];
+    __syncthreads();
+  }
+  if (t == 0)
+    seg_tot[blockIdx.x] = red[0];
+}
+
+/// Exclusive prefix sums of the segment counts, in place (one workgroup).
+__global__ __launch_bounds__(1024) void pix_scan_kernel(uint64_t* __restrict__ seg, uint32_t nseg)
+{
+  __shared__ unsigned long long part[1024];
+  const uint32_t t = threadIdx.x;
+  const uint32_t per = (nseg + 1023) / 1024;
+  const uint32_t b0 = min(nseg, t * per), b1 = min(nseg, b0 + per);
+  unsigned long long s = 0;
+  for (uint32_t i = b0; i < b1; i++)
+    s += seg[i];
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t d = 1; d < 1024; d <<= 1) {
+    const unsigned long long v = t >= d ? part[t - d] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned long long run = part[t] - s;
+  for (uint32_t i = b0; i < b1; i++) {
+    const unsigned long long v = seg[i];
+    seg[i] = run;
+    run += v;
+  }
+}
+
+/// cnt[w] = primes >= 7 below 240 w: segment base plus the prefix of the
+/// word popcounts inside the segment (16 words per thread, one block scan).
+__global__ __launch_bounds__(kThreads) void pix_count_kernel(const uint64_t* __restrict__ bits,
+                                                             const uint64_t* __restrict__ seg_base,
+                                                             uint64_t* __restrict__ cnt,
+                                                             uint64_t nwords)
+{
+  __shared__ unsigned long long part[kThreads];
+  const uint32_t t = threadIdx.x;
+  const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
+  const uint32_t nw = (uint32_t) min<uint64_t>(kPixSegWords, nwords - w0);
+  constexpr uint32_t per = kPixSegWords / kThreads;     // 16
+  const uint32_t a = t * per;
+  unsigned long long s = 0;
+  for (uint32_t i = 0; i < per; i++)
+    if (a + i < nw)
+      s += (unsigned long long) __popcll(bits[w0 + a + i]);
+  part[t] = s;
+  __syncthreads();
+  for (uint32_t d = 1; d < kThreads; d <<= 1) {
+    const unsigned long long v = t >= d ? part[t - d] : 0ull;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned long long run = seg_base[blockIdx.x] + part[t] - s;
+  for (uint32_t i = 0; i < per; i++) {
+    if (a + i < nw) {
+      cnt[w0 + a + i] = run;
+      run += (unsigned long long) __popcll(bits[w0 + a + i]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// leaf sums
+
+/// Sum over the flat terms of all descriptors of w * pi(xp / primes[i]).
+/// Each wave takes chunks of 1,024 consecutive terms; a lane finds the
+/// descriptor of its first term by binary search and then steps forward.
+__global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
+    const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total,
+    const uint32_t* __restrict__ primes, const uint64_t* __restrict__ bits,
+    const uint64_t* __restrict__ cnt, unsigned long long* __restrict__ acc)
+{
+  constexpr uint32_t kSteps = 16;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t wave = ((uint64_t) blockIdx.x * kThreads + threadIdx.x) >> 6;
+  const uint64_t nwaves = ((uint64_t) gridDim.x * kThreads) >> 6;
+  const uint64_t chunk = 64ull * kSteps;
+  u128 sum = 0;
+  for (uint64_t c0 = wave * chunk; c0 < total; c0 += nwaves * chunk) {
+    uint64_t t = c0 + lane;
+    // last descriptor with off <= t
+    uint32_t lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (desc[mid].off <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    uint32_t d = lo;
+    PixLeafDesc ds = desc[d];
+    uint64_t next = d + 1 < ndesc ? desc[d + 1].off : total;
+    for (uint32_t s = 0; s < kSteps && t < total; s++, t += 64) {
+      while (t >= next) {
+        d++;
+        ds = desc[d];
+        next = d + 1 < ndesc ? desc[d + 1].off : total;
+      }
+      const uint32_t q = primes[ds.i_lo + (uint32_t) (t - ds.off)];
+      const uint64_t v = div64_32(ds.xp, q);
+      const uint64_t pv = pix_pi(bits, cnt, v);
+      sum += q <= ds.q1 ? pv : 2 * pv;
+    }
+  }
+  block_add_i128(acc, (i128) sum);
+}
+
+/// B: sum of pi(x / p) over the primes y < p <= isqrt(x), read from the
+/// table's bits (one thread per word).
+__global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t x_hi, uint64_t y,
+                                                         uint64_t sqrtx,
+                                                         const uint64_t* __restrict__ bits,
+                                                         const uint64_t* __restrict__ cnt,
+                                                         unsigned long long* __restrict__ acc)
+{
+  const u128 x = ((u128) x_hi << 64) | x_lo;
+  const double xd = (double) x;
+  const uint64_t w_lo = (y + 1) / 240, w_hi = sqrtx / 240;
+  u128 sum = 0;
+  for (uint64_t w = w_lo + (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
+       w += (uint64_t) gridDim.x * kThreads) {
+    uint64_t v = bits[w];
+    while (v) {
+      const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
+      v &= v - 1;
+      const uint64_t p = 240 * w + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
+      if (p > y && p <= sqrtx)
+        sum += pix_pi(bits, cnt, div_exact(xd, x_lo, p));
+    }
+  }
+  block_add_i128(acc, (i128) sum);
+}
+
+/// Sigma4..6 (src/gourdon/Sigma.cpp:58-90) over the primes b_lo <= b < b_hi:
+/// acc[0..1] += pi(x / (p y)) (p <= sqrt(x / y)), acc[2..3] += pi(x / p^2)
+/// (p > sqrt(x / y)), acc[4..5] += pi(isqrt(x / p))^2.
+__global__ __launch_bounds__(kThreads) void pix_sigma_kernel(
+    uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy, const uint32_t* __restrict__ primes,
+    uint32_t b_lo, uint32_t b_hi, const uint64_t* __restrict__ bits,
+    const uint64_t* __restrict__ cnt, unsigned long long* __restrict__ acc)
+{
+  const u128 x = ((u128) x_hi << 64) | x_lo;
+  i128 s4 = 0, s5 = 0, s6 = 0;
+  for (uint32_t b = b_lo + blockIdx.x * kThreads + threadIdx.x; b < b_hi; b += gridDim.x * kThreads) {
+    const uint64_t p = primes[b];
+    if (p <= sqrt_xy)
+      s4 += pix_pi(bits, cnt, (uint64_t) (x / ((u128) p * y)));
+    else
+      s5 += pix_pi(bits, cnt, (uint64_t) (x / ((u128) p * p)));
+    const uint64_t t = pix_pi(bits, cnt, isqrt64((uint64_t) (x / p)));
+    s6 += (i128) ((u128) t * t);
+  }
+  block_add_i128(acc, s4);
+  block_add_i128(acc + 2, s5);
+  block_add_i128(acc + 4, s6);
+}
+
+/// Phi0 (src/gourdon/Phi0.cpp:43-92): sum of mu(n) phi(x / n, k) over the
+/// square-free n > 1 whose prime factors lie in (p_k, y] and that are primes
+/// or <= z, read from the FactorTableD entries of n (coprime to 2310):
+/// T_MAX = prime <= y (mu = -1), 0 = not square-free or a factor > y,
+/// otherwise 2 pi(lpf) + [mu = -1].  phi(v, k) = (v / pp) phi(pp) +
+/// phi_tbl[v mod pp] with pp the product of the first k primes.
+template <typename FT>
+__global__ __launch_bounds__(kThreads) void pix_phi0_kernel(
+    uint64_t x_lo, uint64_t x_hi, const FT* __restrict__ factor, uint64_t nidx, uint32_t k,
+    uint32_t pk, const uint32_t* __restrict__ phi_tbl, uint32_t pp, uint64_t phi_pp,
+    unsigned long long* __restrict__ acc)
+{
+  __shared__ uint16_t cop[480];
+  if (threadIdx.x == 0) {
+    uint32_t c = 0;
+    for (uint32_t n = 0; n < 2310; n++)
+      if (n % 2 && n % 3 && n % 5 && n % 7 && n % 11)
+        cop[c++] = (uint16_t) n;
+  }
+  __syncthreads();
+  const u128 x = ((u128) x_hi << 64) | x_lo;
+  const FT tmax = (FT) ~(FT) 0;
+  i128 sum = 0;
+  for (uint64_t i = 1 + (uint64_t) blockIdx.x * kThreads + threadIdx.x; i < nidx;
+       i += (uint64_t) gridDim.x * kThreads) {
+    const FT f = factor[i];
+    if (f == 0 || f == (FT) (tmax - 1))
+      continue;
+    const uint64_t n = 2310ull * (i / 480) + cop[i % 480];
+    int mu;
+    if (f == tmax) {
+      if (n <= pk)
+        continue;
+      mu = -1;
+    } else {
+      if ((uint64_t) f <= 2ull * k + 1)
+        continue;
+      mu = (f & 1) ? -1 : 1;
+    }
+    uint32_t r;
+    const u128 v = divmod128_32(x, (uint32_t) n, &r);
+    const u128 q = divmod128_32(v, pp, &r);
+    const i128 ph = (i128) (q * phi_pp) + phi_tbl[r];
+    sum += mu > 0 ? ph : -ph;
+  }
+  block_add_i128(acc, sum);
+}
+
+__global__ void pix_pi_points_kernel(const uint64_t* __restrict__ n, uint64_t* __restrict__ out,
+                                     uint32_t count, const uint64_t* __restrict__ bits,
+                                     const uint64_t* __restrict__ cnt)
+{
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count)
+    out[i] = pix_pi(bits, cnt, n[i]);
+}
+
+uint32_t grid_for(uint64_t work, uint32_t cap)
+{
+  const uint64_t g = (work + kThreads - 1) / kThreads;
+  return (uint32_t) (g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+} // namespace
+
+hipError_t pix_launch_sieve(uint64_t* bits, uint64_t* seg_tot, const uint32_t* primes,
+                            uint32_t j_small_end, uint32_t j_end, uint64_t nwords, uint64_t N,
+                            hipStream_t st)
+{
+  const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
+  hipLaunchKernelGGL(pix_sieve_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, bits, seg_tot,
+                     primes, j_small_end, j_end, nwords, N);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_scan(uint64_t* seg_tot, uint32_t nseg, hipStream_t st)
+{
+  hipLaunchKernelGGL(pix_scan_kernel, dim3(1), dim3(1024), 0, st, seg_tot, nseg);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_counts(const uint64_t* bits, const uint64_t* seg_base, uint64_t* cnt,
+                             uint64_t nwords, hipStream_t st)
+{
+  const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
+  hipLaunchKernelGGL(pix_count_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, bits, seg_base,
+                     cnt, nwords);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
+                              const uint32_t* primes, const uint64_t* bits, const uint64_t* cnt,
+                              unsigned long long* acc, uint32_t cus, hipStream_t st)
+{
+  if (total == 0 || ndesc == 0) return hipSuccess;
+  const uint64_t waves = (total + 1023) / 1024;
+  const uint32_t grid = (uint32_t) std::min<uint64_t>((waves + 3) / 4, (uint64_t) cus * 32);
+  hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
+                     primes, bits, cnt, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
+                        const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
+                        hipStream_t st)
+{
+  if (sqrtx <= y) return hipSuccess;
+  const uint64_t words = sqrtx / 240 - (y + 1) / 240 + 1;
+  hipLaunchKernelGGL(pix_B_kernel, dim3(grid_for(words, 65536)), dim3(kThreads), 0, st, x_lo, x_hi,
+                     y, sqrtx, bits, cnt, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
+                            const uint32_t* primes, uint32_t b_lo, uint32_t b_hi,
+                            const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
+                            hipStream_t st)
+{
+  if (b_hi <= b_lo) return hipSuccess;
+  hipLaunchKernelGGL(pix_sigma_kernel, dim3(grid_for(b_hi - b_lo, 4096)), dim3(kThreads), 0, st,
+                     x_lo, x_hi, y, sqrt_xy, primes, b_lo, b_hi, bits, cnt, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, bool factor32,
+                           uint64_t nidx, uint32_t k, uint32_t pk, const uint32_t* phi_tbl,
+                           uint32_t pp, uint64_t phi_pp, unsigned long long* acc, hipStream_t st)
+{
+  if (nidx <= 1) return hipSuccess;
+  const uint32_t grid = grid_for(nidx, 16384);
+  if (factor32)
+    hipLaunchKernelGGL(pix_phi0_kernel<uint32_t>, dim3(grid), dim3(kThreads), 0, st, x_lo, x_hi,
+                       (const uint32_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  else
+    hipLaunchKernelGGL(pix_phi0_kernel<uint16_t>, dim3(grid), dim3(kThreads), 0, st, x_lo, x_hi,
+                       (const uint16_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_pi_points(const uint64_t* n, uint64_t* out, uint32_t count,
+                                const uint64_t* bits, const uint64_t* cnt, hipStream_t st)
+{
+  if (count == 0) return hipSuccess;
+  hipLaunchKernelGGL(pix_pi_points_kernel, dim3((count + 255) / 256), dim3(256), 0, st, n, out,
+                     count, bits, cnt);
+  return hipGetLastError();
+}
+
+} // namespace pcg

@@ -20,6 +20,9 @@ configs.json        D(x) and S2_hard(x) with default parameters at the
 batch_D.json        the 256-query batch: x_i = floor(10^(12 + 6 u_i)),
                     u_i = (splitmix64 >> 11) * 2^-53, and D(x_i)
 alpha_D.json        D(x) at non-default alpha_y / alpha_z (y < z cases)
+gourdon_terms.json  the other Gourdon terms (A + C, B, Phi0, Sigma) and D at
+                    default parameters, pi(x) assembled from them and (to
+                    1e16) printed by the reference's pi(x)
 
 Usage:  python tests/golden/gen_fixtures.py [--big] [--only NAME]
 """
@@ -191,6 +194,47 @@ def gen_alpha():
                "rows": rows}, open(os.path.join(HERE, "alpha_D.json"), "w"), indent=0)
 
 
+def gourdon_terms(x, extra=()):
+    """The reference's Gourdon terms and parameters for x (`--AC/--B/--Phi0/
+    --Sigma/--D -s`, each printing the parameters it used)."""
+    row = {"x": str(x)}
+    for opt, key in (("--AC", "A + C"), ("--B", "B"), ("--Phi0", "Phi0"),
+                     ("--Sigma", "Sigma"), ("--D", "D")):
+        out = subprocess.run([CLI, str(x), opt, "-s"] + list(extra), capture_output=True,
+                             text=True, timeout=3600, check=True).stdout
+        for line in out.splitlines():
+            line = line.replace("\x1b[K", "").split("\r")[-1]
+            m = re.match(r"^\s*([A-Za-z_0-9 +]+?)\s*=\s*(-?[0-9.]+)\s*$", line)
+            if m and m.group(1) in (key, "y", "z", "k", "x_star"):
+                row[{"A + C": "AC"}.get(m.group(1), m.group(1))] = m.group(2)
+    return row
+
+
+def gen_gourdon():
+    """pi(x) = A - B + C + D + Phi0 + Sigma (src/gourdon/pi_gourdon.cpp:82-87)
+    term by term, at powers of ten, at seeded x and at alpha_z = 2."""
+    xs = [10 ** e for e in range(10, 20)] + [3 * 10 ** e for e in range(10, 19, 2)]
+    xs += batch_xs(0x6A0DC0DE, n=16, lo_exp=10.0, span=8.0)
+    rows = []
+    for x in xs:
+        r = gourdon_terms(x)
+        r["pi"] = str(int(r["AC"]) - int(r["B"]) + int(r["D"]) + int(r["Phi0"]) + int(r["Sigma"]))
+        if x <= 10 ** 16:
+            r["pi_cli"] = run_cli([x, "-t", "8"]).get("pi", None) or subprocess.run(
+                [CLI, str(x), "-t", "8"], capture_output=True, text=True, check=True).stdout.strip()
+        rows.append(r)
+        print(x, r, flush=True)
+    alpha = []
+    for x in (10 ** 12, 10 ** 14, 7 * 10 ** 15, 10 ** 17):
+        r = gourdon_terms(x, ("--alpha-z=2",))
+        alpha.append(r)
+        print(x, r, flush=True)
+    json.dump({"source": "oracle/_ref primecount X --AC/--B/--Phi0/--Sigma/--D -s",
+               "rule": "pi = AC - B + D + Phi0 + Sigma; pi_cli = primecount X (x <= 1e16)",
+               "rows": rows, "alpha_z2": alpha},
+              open(os.path.join(HERE, "gourdon_terms.json"), "w"), indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also run x = 1e21 in `configs`")
@@ -199,7 +243,7 @@ def main():
     if not os.path.exists(CLI):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     steps = {"kat": gen_kat, "params": gen_params, "configs": lambda: gen_configs(a.big),
-             "batch": gen_batch, "alpha": gen_alpha, "big": gen_big}
+             "batch": gen_batch, "alpha": gen_alpha, "big": gen_big, "gourdon": gen_gourdon}
     for name, fn in steps.items():
         if (a.only and a.only != name) or (not a.only and name == "big"):
             continue

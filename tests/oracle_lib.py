@@ -1,7 +1,7 @@
-"""ctypes access to the CPU oracle (oracle/hsl_oracle.c).
+"""ctypes access to the CPU oracle (oracle/hsl_oracle.c, oracle/gourdon_oracle.c).
 
 Test infrastructure: the oracle is the checker the HIP engine is compared
-against; it is built into oracle/liboracle.so by oracle/build_oracle.sh (or
+against; it is built into oracle/liboracle.so by build() here (or
 __graft_entry__.build()).
 """
 import ctypes
@@ -10,7 +10,9 @@ import subprocess
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "oracle", "liboracle.so")
-SRC = os.path.join(REPO, "oracle", "hsl_oracle.c")
+SRCS = [os.path.join(REPO, "oracle", "hsl_oracle.c"),
+        os.path.join(REPO, "oracle", "gourdon_oracle.c")]
+SRC = SRCS[0]
 
 
 class Oracle:
@@ -24,6 +26,13 @@ class Oracle:
             fn.restype = None
         lib.oracle_phi_direct.argtypes = [ctypes.c_int64, ctypes.c_int64]
         lib.oracle_phi_direct.restype = ctypes.c_int64
+        lib.oracle_gourdon_term.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                            ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64),
+                                            ctypes.POINTER(ctypes.c_int64)]
+        lib.oracle_gourdon_term.restype = None
+        lib.oracle_pi_sieve.argtypes = [ctypes.c_int64]
+        lib.oracle_pi_sieve.restype = ctypes.c_int64
 
     def _call(self, fn, x, y, z, k, seg):
         x = int(x)
@@ -42,10 +51,24 @@ class Oracle:
     def phi_direct(self, n, a):
         return self.lib.oracle_phi_direct(int(n), int(a))
 
+    def gourdon_term(self, which, x, y, z, k, x_star):
+        """which: "AC", "B", "Phi0" or "Sigma" (plain sums over the definitions)."""
+        idx = {"AC": 0, "B": 1, "Phi0": 2, "Sigma": 3}[which]
+        u = int(x) & ((1 << 128) - 1)
+        lo, hi = ctypes.c_uint64(), ctypes.c_int64()
+        self.lib.oracle_gourdon_term(idx, u & ((1 << 64) - 1), ctypes.c_int64(u >> 64).value,
+                                     int(y), int(z), int(k), int(x_star),
+                                     ctypes.byref(lo), ctypes.byref(hi))
+        return (hi.value << 64) | lo.value
 
-def build():
-    if (not os.path.exists(LIB)) or os.path.getmtime(LIB) < os.path.getmtime(SRC):
-        subprocess.run(["gcc", "-O2", "-shared", "-fPIC", "-o", LIB, SRC, "-lm"], check=True)
+    def pi_sieve(self, n):
+        return self.lib.oracle_pi_sieve(int(n))
+
+
+def build(force=False):
+    if force or (not os.path.exists(LIB)) or \
+            os.path.getmtime(LIB) < max(os.path.getmtime(s) for s in SRCS):
+        subprocess.run(["gcc", "-O2", "-shared", "-fPIC", "-o", LIB] + SRCS + ["-lm"], check=True)
     return LIB
 
 

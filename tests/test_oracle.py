@@ -84,3 +84,38 @@ def test_phi_direct_small(oracle):
     for n in (1, 2, 29, 30, 31, 97, 1000):
         for a in range(0, 9):
             assert oracle.phi_direct(n, a) == phi(n, a)
+
+
+# ---- the other Gourdon terms (oracle/gourdon_oracle.c) ----------------------
+
+def test_oracle_gourdon_terms_match_reference(oracle):
+    """Plain sums over the definitions of A + C, B, Phi0 and Sigma reproduce
+    the values the reference build printed (default and alpha_z = 2
+    parameters), and with D they assemble pi(x) = the reference's pi(x)."""
+    fx = golden("gourdon_terms.json")
+    rows = [r for r in fx["rows"] if int(r["x"]) <= 10 ** 12] + \
+           [r for r in fx["alpha_z2"] if int(r["x"]) <= 10 ** 12]
+    assert len(rows) >= 8
+    for r in rows:
+        x, y, z, k, xs = (int(r[t]) for t in ("x", "y", "z", "k", "x_star"))
+        got = {t: oracle.gourdon_term(t, x, y, z, k, xs) for t in ("AC", "B", "Phi0", "Sigma")}
+        for t in got:
+            assert got[t] == int(r[t]), (x, t)
+        if "pi_cli" in r:
+            d = oracle.D(x, y, z, k)
+            assert got["AC"] - got["B"] + d + got["Phi0"] + got["Sigma"] == int(r["pi_cli"])
+
+
+def test_oracle_pi_known_values(oracle):
+    # pi(10^n) of the reference's README table, by sieving (n <= 8 here) and
+    # by the assembled terms for 10^10 .. 10^11
+    assert [oracle.pi_sieve(10 ** n) for n in range(1, 9)] == \
+        [4, 25, 168, 1229, 9592, 78498, 664579, 5761455]
+    known = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    fx = {int(r["x"]): r for r in golden("gourdon_terms.json")["rows"]}
+    for x in (10 ** 10, 10 ** 11):
+        r = fx[x]
+        y, z, k, xs = (int(r[t]) for t in ("y", "z", "k", "x_star"))
+        total = sum(s * oracle.gourdon_term(t, x, y, z, k, xs)
+                    for s, t in ((1, "AC"), (-1, "B"), (1, "Phi0"), (1, "Sigma")))
+        assert total + oracle.D(x, y, z, k) == known[x]
