@@ -10,7 +10,8 @@ path: planning, the sieve kernel, the carry combine and the read-back.
 
 Prints ONE JSON line.  `roofline` is measured on the sieve kernel with HIP
 events on the engine's stream; `cpu_baseline` times the reference build
-(oracle/_ref/primecount, when present) on this host.
+(oracle/_ref/primecount, when present) on this host; `pi` reports the full
+pi(x) (all Gourdon terms on the device) outside the timed D steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--x 1e19] [--no-cpu]
 """
@@ -57,6 +58,28 @@ def measured_lds_peak():
     if lib.pcg_probe_lds_read_gbs(ctypes.byref(v)) != 0:
         return None
     return v.value
+
+
+def measure_pi(pc, x):
+    """pi(x) by Gourdon's algorithm on the device (all six terms), outside
+    the timed D steps: one untimed call builds the prime-count table, then
+    the median of 3 calls; checked against the reference's value."""
+    known = {}
+    with open(os.path.join(REPO, "tests", "golden", "pi_known.json")) as f:
+        for r in json.load(f)["rows"]:
+            known[int(r["x"])] = int(r["pi"])
+    got = pc.pi(x)
+    times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        got = pc.pi(x)
+        times.append(time.perf_counter() - t)
+    st = pc.last_pi_stats()
+    times.sort()
+    return {"x": str(x), "value": str(got), "bit_exact": known.get(x) == got,
+            "seconds": round(times[1], 6),
+            "terms_s": {k: round(st["seconds_" + k], 6) for k in ("sigma", "phi0", "ac", "b", "d")},
+            "ac_leaves": st["ac_leaves"], "table_limit": st["table_limit"]}
 
 
 def traffic_from_profile(x):
@@ -126,6 +149,7 @@ def main():
     ap.add_argument("--cpu-timeout", type=int, default=120)
     ap.add_argument("--segment-words", type=int, default=0,
                     help="64-bit words per sieve segment (0 = automatic)")
+    ap.add_argument("--no-pi", action="store_true", help="skip the pi(x) measurement")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -231,6 +255,8 @@ def main():
         "roofline_lds": roofline_lds,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),
     }
+    if not a.no_pi:
+        out["pi"] = measure_pi(pc, x)
     if a.calibrate_copy:
         out["calibration_copy_bytes"] = device_copy(1 << 30)
     print(json.dumps(out), flush=True)
