@@ -172,6 +172,13 @@ int pcg_B(pcg_i128 x, int64_t y, pcg_i128* out);
 int pcg_Phi0(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);
 int pcg_Sigma(pcg_i128 x, int64_t y, pcg_i128* out);
 
+/* The n-th prime, 1 <= n <= 216289611853439384 (the primes below 2^63):
+ * pi(x0) on the device at x0 = R^-1(n) (Riemann's R), then a device sieve of
+ * the interval between x0 and the answer.  Replaces nth_prime /
+ * primecount_nth_prime (include/primecount.hpp:84, include/primecount.h:81
+ * -> src/nth_prime.cpp:95-145). */
+int pcg_nth_prime(int64_t n, int64_t* out);
+
 typedef struct pcg_pi_stats {
   double seconds_total;      /* wall time of the last pcg_pi call             */
   double seconds_table;      /* tables, including the prime-count table       */

@@ -134,6 +134,7 @@ def load():
         "pcg_release": ([], ctypes.c_int),
         "pcg_pi": ([_I128, PI128], ctypes.c_int),
         "pcg_last_pi_stats": ([ctypes.POINTER(PiStats)], ctypes.c_int),
+        "pcg_nth_prime": ([I64, P64], ctypes.c_int),
         "pcg_AC": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
         "pcg_Phi0": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
@@ -237,6 +238,13 @@ def pi(x):
     out = _I128()
     _check(load().pcg_pi(_i128(x), ctypes.byref(out)))
     return _from128(out)
+
+
+def nth_prime(n):
+    """The n-th prime (n <= 216289611853439384), on the device."""
+    out = ctypes.c_int64()
+    _check(load().pcg_nth_prime(int(n), ctypes.byref(out)))
+    return out.value
 
 
 def last_pi_stats():

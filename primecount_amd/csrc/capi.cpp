@@ -190,6 +190,15 @@ int pcg_pi(pcg_i128 x, pcg_i128* out)
   });
 }
 
+int pcg_nth_prime(int64_t n, int64_t* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
+    *out = (int64_t) nth_prime((uint64_t) n, nullptr);
+  });
+}
+
 int pcg_last_pi_stats(pcg_pi_stats* out)
 {
   return guarded([&] {

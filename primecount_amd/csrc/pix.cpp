@@ -19,6 +19,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <map>
 
@@ -321,7 +322,156 @@ void check_params(i128 x, int64_t y, int64_t z, int64_t k)
     throw pcg_error("Gourdon terms need z < sqrt(x)");
 }
 
+/// zeta(s) for integers s >= 2: the first 64 terms plus the Euler-Maclaurin
+/// tail (relative error far below what the R^-1 starting point needs).
+long double zeta_int(int s)
+{
+  static long double cache[1024] = {};
+  if (s < 1024 && cache[s] != 0) return cache[s];
+  const int N = 64;
+  long double z = 0;
+  for (int n = 1; n <= N; n++) z += powl((long double) n, (long double) -s);
+  z += powl((long double) N, (long double) (1 - s)) / (s - 1) - 0.5L * powl((long double) N, (long double) -s);
+  if (s < 1024) cache[s] = z;
+  return z;
+}
+
+/// Riemann's R(x) = 1 + sum_{k>=1} (ln x)^k / (k k! zeta(k + 1)) (Gram).
+long double riemann_r(long double x)
+{
+  if (x < 2) return 0;
+  const long double lx = logl(x);
+  long double term = 1, sum = 1;
+  for (int k = 1; k < 1000; k++) {
+    term *= lx / k;
+    const long double add = term / (k * zeta_int(k + 1));
+    sum += add;
+    if (k > lx && add < 1e-16L * sum) break;
+  }
+  return sum;
+}
+
+/// x with R(x) ~ n (Newton steps): the starting point of the nth prime
+/// search.  The answer does not depend on it, only the sieve interval does
+/// (the reference uses Li^-1, R^-1 or R(psi)^-1, src/nth_prime.cpp:78-88).
+uint64_t riemann_r_inverse(uint64_t n)
+{
+  long double x = (long double) n * logl((long double) n);
+  for (int it = 0; it < 100; it++) {
+    const long double nx = x - (riemann_r(x) - (long double) n) * logl(x);
+    if (nx < 2) { x = 2; break; }
+    if (fabsl(nx - x) < 0.5L) { x = nx; break; }
+    x = nx;
+  }
+  return (uint64_t) llroundl(x);
+}
+
 } // namespace
+
+uint64_t nth_prime(uint64_t n, PixStats* st)
+{
+  // primes below 2^63, as the reference's 64-bit nth_prime (src/nth_prime.cpp:95-100)
+  const uint64_t max_n = 216289611853439384ull;
+  if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
+  if (n > max_n) throw pcg_error("nth_prime(n): n must be <= 216289611853439384");
+  const double t0 = now_s();
+  PixStats s;
+  uint64_t result = 0;
+  if (n <= 3) {
+    result = n == 1 ? 2 : (n == 2 ? 3 : 5);
+  } else if (n <= 5761455) {                       // pi(10^8): the table answers
+    auto lock = engine_lock();
+    const EngineView v = engine_view(10001, 10001, 100000000);
+    uint64_t* dout = (uint64_t*) engine_buffer(2, 8);
+    PIX_HIP(pix_launch_select(v.pix_bits, v.pix_cnt, v.pix_n / 240 + 1, n, dout, v.stream));
+    PIX_HIP(hipMemcpyAsync(&result, dout, 8, hipMemcpyDeviceToHost, v.stream));
+    PIX_HIP(hipStreamSynchronize(v.stream));
+  } else {
+    const uint64_t x0 = riemann_r_inverse(n);
+    if (x0 >= ((uint64_t) 1 << 62)) throw pcg_error("nth_prime(n): n is too large");
+    PixStats ps;
+    const uint64_t c = (uint64_t) pi_gourdon((i128) x0, &ps);
+    s = ps;
+    const GourdonParams gp = gourdon_params((i128) x0);
+    const bool down = c >= n;
+    const uint64_t k = down ? c - n + 1 : n - c;   // rank from x0 downward / upward
+    const double lx = std::log((double) x0);
+    uint64_t delta = (uint64_t) ((double) (k + 16) * lx * 2.0) + 100000;
+    auto lock = engine_lock();
+    for (;;) {
+      uint64_t lo, hi;
+      if (down) {
+        hi = x0;
+        lo = x0 > delta + 2 ? x0 - delta : 2;
+      } else {
+        lo = x0 + 1;
+        hi = x0 + delta;
+      }
+      const uint64_t r = isqrt((u128) hi);
+      const EngineView v = engine_view(gp.y, gp.z, r + 1);
+      const std::vector<uint64_t> pr = pi_points(v, {r, 15359});
+      const uint64_t n_all = pr[0] >= 3 ? pr[0] - 3 : 0;       // sieving primes >= 7
+      const uint64_t n_small = std::min<uint64_t>(pr[1] - 3, n_all);
+      uint32_t* sp = (uint32_t*) engine_buffer(5, (n_all + 1) * 4);
+      if (n_all) PIX_HIP(pix_launch_extract(v.pix_bits, v.pix_cnt, r, sp, v.stream));
+      const uint64_t nwords = hi / 240 - lo / 240 + 1;
+      const uint64_t nblk = (nwords + kPixSegWords - 1) / kPixSegWords;
+      uint64_t* bits = (uint64_t*) engine_buffer(6, nwords * 8);
+      uint64_t* cnts = (uint64_t*) engine_buffer(7, nblk * 8);
+      PIX_HIP(pix_launch_interval(bits, lo, hi, sp, (uint32_t) n_small, n_all, v.stream));
+      PIX_HIP(pix_launch_block_counts(bits, nwords, cnts, v.stream));
+      std::vector<uint64_t> bc(nblk);
+      PIX_HIP(hipMemcpyAsync(bc.data(), cnts, nblk * 8, hipMemcpyDeviceToHost, v.stream));
+      PIX_HIP(hipStreamSynchronize(v.stream));
+      uint64_t total = 0;
+      for (uint64_t b : bc) total += b;
+      if (total < k) {
+        if (down && lo == 2) throw pcg_error("internal: nth_prime interval");
+        delta *= 2;
+        continue;
+      }
+      // block holding the k-th prime from the interval's top (down) or bottom
+      uint64_t need = k, blk = 0;
+      if (down) {
+        for (blk = nblk; blk-- > 0;) {
+          if (bc[blk] >= need) break;
+          need -= bc[blk];
+        }
+      } else {
+        for (blk = 0; blk < nblk; blk++) {
+          if (bc[blk] >= need) break;
+          need -= bc[blk];
+        }
+      }
+      const uint64_t w0 = blk * kPixSegWords;
+      const uint64_t nw = std::min<uint64_t>(kPixSegWords, nwords - w0);
+      std::vector<uint64_t> words(nw);
+      PIX_HIP(hipMemcpyAsync(words.data(), bits + w0, nw * 8, hipMemcpyDeviceToHost, v.stream));
+      PIX_HIP(hipStreamSynchronize(v.stream));
+      static const uint8_t wheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+      const uint64_t base = (lo / 240 + w0) * 240;
+      for (uint64_t i = 0; i < nw && !result; i++) {
+        const uint64_t wi = down ? nw - 1 - i : i;
+        const uint64_t wv = words[wi];
+        const uint64_t c1 = (uint64_t) __builtin_popcountll(wv);
+        if (c1 < need) { need -= c1; continue; }
+        for (int j = 0; j < 64; j++) {
+          const int bit = down ? 63 - j : j;
+          if (!((wv >> bit) & 1)) continue;
+          if (--need == 0) {
+            result = base + 240 * wi + 30 * (bit >> 3) + wheel[bit & 7];
+            break;
+          }
+        }
+      }
+      if (!result) throw pcg_error("internal: nth_prime selection");
+      break;
+    }
+  }
+  s.seconds_total = now_s() - t0;
+  if (st) *st = s;
+  return result;
+}
 
 i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st)
 {

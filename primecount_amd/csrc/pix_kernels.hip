@@ -397,6 +397,166 @@ __global__ void pix_pi_points_kernel(const uint64_t* __restrict__ n, uint64_t* _
     out[i] = pix_pi(bits, cnt, n[i]);
 }
 
+// ---------------------------------------------------------------------------
+// nth prime: sieving an interval and selecting a prime
+
+/// The primes 7 <= p <= p_max in order, read from the table's bits: the
+/// prime of bit j of word w goes to out[cnt[w] + rank of j in the word].
+__global__ __launch_bounds__(kThreads) void pix_extract_kernel(const uint64_t* __restrict__ bits,
+                                                               const uint64_t* __restrict__ cnt,
+                                                               uint64_t p_max, uint32_t* __restrict__ out)
+{
+  const uint64_t w_hi = p_max / 240;
+  for (uint64_t w = (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
+       w += (uint64_t) gridDim.x * kThreads) {
+    uint64_t v = bits[w];
+    uint64_t pos = cnt[w];
+    while (v) {
+      const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
+      v &= v - 1;
+      const uint64_t p = 240 * w + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
+      if (p <= p_max)
+        out[pos] = (uint32_t) p;
+      pos++;
+    }
+  }
+}
+
+/// Sieve of the interval [lo, hi] (words lo / 240 .. hi / 240, one
+/// workgroup per kPixSegWords words in LDS): the numbers coprime to 30 in
+/// the interval minus the multiples p * q (q >= p) of the sieving primes
+/// sp[0 .. n_small) (all threads per prime); the larger sieving primes are
+/// applied afterwards by pix_interval_large_kernel.
+__global__ __launch_bounds__(kThreads) void pix_interval_kernel(
+    uint64_t* __restrict__ out, uint64_t lo, uint64_t hi, uint64_t nwords,
+    const uint32_t* __restrict__ sp, uint32_t n_small)
+{
+  __shared__ uint32_t bm[2 * kPixSegWords];
+  const uint32_t t = threadIdx.x;
+  const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
+  const uint32_t nw = (uint32_t) min<uint64_t>(kPixSegWords, nwords - w0);
+  const uint64_t low = (lo / 240 + w0) * 240;            // first number of the block
+  for (uint32_t w = t; w < nw; w += kThreads) {
+    const uint64_t base = low + 240ull * w;
+    uint64_t v = ~0ull;
+    if (base + 239 > hi)
+      v = base > hi ? 0ull : le_mask((uint32_t) (hi - base));
+    if (base < lo)
+      v &= lo - base >= 240 ? 0ull : ~(lo - base == 0 ? 0ull : le_mask((uint32_t) (lo - base - 1)));
+    if (base == 0)
+      v &= ~1ull;                                          // 1 is not prime
+    bm[2 * w] = (uint32_t) v;
+    bm[2 * w + 1] = (uint32_t) (v >> 32);
+  }
+  __syncthreads();
+  const uint64_t high = low + 240ull * nw;               // exclusive
+  const uint32_t cls = t & 7;
+  const uint32_t t0 = t >> 3;
+  for (uint32_t j = 0; j < n_small; j++) {
+    const uint64_t p = sp[j];
+    // exact 64-bit divisions: the interval may lie anywhere below 2^63
+    const uint64_t q_lo = max(p, (low + p - 1) / p);
+    const uint64_t q_hi = (high - 1) / p;
+    if (q_lo > q_hi)
+      continue;
+    uint64_t q = q_lo - q_lo % 30 + ((kWheelBytes >> (8 * cls)) & 0xff);
+    if (q < q_lo)
+      q += 30;
+    if (q > q_hi)
+      continue;
+    const uint32_t n = (uint32_t) ((q_hi - q) / 30) + 1;
+    const uint32_t off = (uint32_t) (p * q - low);
+    const uint32_t byte0 = off / 30;
+    const uint32_t bib = wheel_bit(off - byte0 * 30);
+    for (uint32_t k = t0; k < n; k += kThreads / 8) {
+      const uint32_t B = byte0 + k * (uint32_t) p;
+      atomicAnd(&bm[B >> 2], ~(1u << (((B & 3) << 3) | bib)));
+    }
+  }
+  __syncthreads();
+  for (uint32_t w = t; w < nw; w += kThreads)
+    out[w0 + w] = ((uint64_t) bm[2 * w + 1] << 32) | bm[2 * w];
+}
+
+/// The sieving primes sp[j_lo .. j_hi) on the interval, one thread per prime,
+/// with 64-bit atomics on the interval's words in HBM.
+__global__ __launch_bounds__(kThreads) void pix_interval_large_kernel(
+    uint64_t* __restrict__ out, uint64_t lo, uint64_t hi, const uint32_t* __restrict__ sp,
+    uint64_t j_lo, uint64_t j_hi)
+{
+  const uint64_t wbase = lo / 240;
+  for (uint64_t j = j_lo + (uint64_t) blockIdx.x * kThreads + threadIdx.x; j < j_hi;
+       j += (uint64_t) gridDim.x * kThreads) {
+    const uint64_t p = sp[j];
+    const uint64_t q_lo = max(p, (lo + p - 1) / p);
+    const uint64_t q_hi = hi / p;
+    if (q_lo > q_hi)
+      continue;
+    const uint32_t qm = (uint32_t) (q_lo % 30);
+    const uint64_t qbase = q_lo - qm;
+    for (uint32_t idx = qm ? le_count(qm - 1) : 0u;; idx++) {
+      const uint64_t q = qbase + 30ull * (idx >> 3) + ((kWheelBytes >> (8 * (idx & 7))) & 0xff);
+      if (q > q_hi)
+        break;
+      const uint64_t m = p * q;
+      const uint64_t w = m / 240;
+      const uint32_t r = (uint32_t) (m - w * 240);
+      const uint32_t b = 8 * (r / 30) + wheel_bit(r % 30);
+      atomicAnd((unsigned long long*) &out[w - wbase], ~(1ull << b));
+    }
+  }
+}
+
+/// Primes per block of kPixSegWords words of the interval bitmap.
+__global__ __launch_bounds__(kThreads) void pix_block_count_kernel(const uint64_t* __restrict__ bits,
+                                                                   uint64_t nwords,
+                                                                   uint64_t* __restrict__ counts)
+{
+  __shared__ unsigned long long red[kThreads];
+  const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
+  unsigned long long s = 0;
+  for (uint64_t w = w0 + threadIdx.x; w < min<uint64_t>(w0 + kPixSegWords, nwords); w += kThreads)
+    s += (unsigned long long) __popcll(bits[w]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (uint32_t d = kThreads / 2; d > 0; d >>= 1) {
+    if (threadIdx.x < d)
+      red[threadIdx.x] += red[threadIdx.x + d];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    counts[blockIdx.x] = red[0];
+}
+
+/// The n-th prime (n >= 4) from the table: the last word w with
+/// 3 + cnt[w] < n holds it (binary search), then its bit of rank
+/// n - 3 - cnt[w].  One thread.
+__global__ void pix_select_kernel(const uint64_t* __restrict__ bits,
+                                  const uint64_t* __restrict__ cnt, uint64_t nwords, uint64_t n,
+                                  uint64_t* __restrict__ out)
+{
+  uint64_t lo = 0, hi = nwords - 1;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi + 1) / 2;
+    if (3 + cnt[mid] < n)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  uint64_t v = bits[lo];
+  uint64_t k = n - 3 - cnt[lo];                 // 1-based rank inside the word
+  uint64_t res = 0;
+  while (v) {
+    const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
+    v &= v - 1;
+    if (--k == 0) {
+      res = 240 * lo + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
+      break;
+    }
+  }
+  *out = res;
+}
+
 uint32_t grid_for(uint64_t work, uint32_t cap)
 {
   const uint64_t g = (work + kThreads - 1) / kThreads;
@@ -476,6 +636,44 @@ hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, boo
   else
     hipLaunchKernelGGL(pix_phi0_kernel<uint16_t>, dim3(grid), dim3(kThreads), 0, st, x_lo, x_hi,
                        (const uint16_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_extract(const uint64_t* bits, const uint64_t* cnt, uint64_t p_max,
+                              uint32_t* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(pix_extract_kernel, dim3(grid_for(p_max / 240 + 1, 65536)), dim3(kThreads), 0,
+                     st, bits, cnt, p_max, out);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const uint32_t* sp,
+                               uint32_t n_small, uint64_t n_all, hipStream_t st)
+{
+  const uint64_t nwords = hi / 240 - lo / 240 + 1;
+  const uint64_t nblk = (nwords + kPixSegWords - 1) / kPixSegWords;
+  hipLaunchKernelGGL(pix_interval_kernel, dim3((uint32_t) nblk), dim3(kThreads), 0, st, out, lo, hi,
+                     nwords, sp, n_small);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || n_all <= n_small) return e;
+  hipLaunchKernelGGL(pix_interval_large_kernel, dim3(grid_for(n_all - n_small, 65536)),
+                     dim3(kThreads), 0, st, out, lo, hi, sp, (uint64_t) n_small, n_all);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_block_counts(const uint64_t* bits, uint64_t nwords, uint64_t* counts,
+                                   hipStream_t st)
+{
+  const uint64_t nblk = (nwords + kPixSegWords - 1) / kPixSegWords;
+  hipLaunchKernelGGL(pix_block_count_kernel, dim3((uint32_t) nblk), dim3(kThreads), 0, st, bits,
+                     nwords, counts);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_select(const uint64_t* bits, const uint64_t* cnt, uint64_t nwords, uint64_t n,
+                             uint64_t* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(pix_select_kernel, dim3(1), dim3(1), 0, st, bits, cnt, nwords, n, out);
   return hipGetLastError();
 }
 

@@ -46,6 +46,14 @@ hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t s
 hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, bool factor32,
                            uint64_t nidx, uint32_t k, uint32_t pk, const uint32_t* phi_tbl,
                            uint32_t pp, uint64_t phi_pp, unsigned long long* acc, hipStream_t st);
+hipError_t pix_launch_extract(const uint64_t* bits, const uint64_t* cnt, uint64_t p_max,
+                              uint32_t* out, hipStream_t st);
+hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const uint32_t* sp,
+                               uint32_t n_small, uint64_t n_all, hipStream_t st);
+hipError_t pix_launch_block_counts(const uint64_t* bits, uint64_t nwords, uint64_t* counts,
+                                   hipStream_t st);
+hipError_t pix_launch_select(const uint64_t* bits, const uint64_t* cnt, uint64_t nwords, uint64_t n,
+                             uint64_t* out, hipStream_t st);
 hipError_t pix_launch_pi_points(const uint64_t* n, uint64_t* out, uint32_t count,
                                 const uint64_t* bits, const uint64_t* cnt, hipStream_t st);
 

@@ -20,6 +20,7 @@ configs.json        D(x) and S2_hard(x) with default parameters at the
 batch_D.json        the 256-query batch: x_i = floor(10^(12 + 6 u_i)),
                     u_i = (splitmix64 >> 11) * 2^-53, and D(x_i)
 alpha_D.json        D(x) at non-default alpha_y / alpha_z (y < z cases)
+nth_prime.json      nth_prime(n) at small, power-of-ten and seeded n
 gourdon_terms.json  the other Gourdon terms (A + C, B, Phi0, Sigma) and D at
                     default parameters, pi(x) assembled from them and (to
                     1e16) printed by the reference's pi(x)
@@ -235,6 +236,22 @@ def gen_gourdon():
               open(os.path.join(HERE, "gourdon_terms.json"), "w"), indent=1)
 
 
+def gen_nth():
+    """nth_prime(n) printed by the reference (src/nth_prime.cpp): the tiny
+    and small-table ranges, powers of ten and seeded n = 10^(8 + 9 u)."""
+    ns = list(range(1, 200, 7)) + [169, 170, 3314, 3315, 10 ** 4, 10 ** 5, 99999, 10 ** 6]
+    ns += [10 ** e for e in range(7, 17)] + [3 * 10 ** e for e in range(7, 16, 2)]
+    ns += [int(v) for v in batch_xs(0x17A5E, n=24, lo_exp=8.0, span=9.0)]
+    rows = []
+    for n in ns:
+        out = subprocess.run([CLI, str(n), "--nth-prime", "-t", "8"], capture_output=True,
+                             text=True, timeout=3600, check=True).stdout.strip()
+        rows.append({"n": str(n), "p": out.splitlines()[-1].strip()})
+        print(n, rows[-1]["p"], flush=True)
+    json.dump({"source": "oracle/_ref primecount N --nth-prime", "rows": rows},
+              open(os.path.join(HERE, "nth_prime.json"), "w"), indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also run x = 1e21 in `configs`")
@@ -243,7 +260,8 @@ def main():
     if not os.path.exists(CLI):
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     steps = {"kat": gen_kat, "params": gen_params, "configs": lambda: gen_configs(a.big),
-             "batch": gen_batch, "alpha": gen_alpha, "big": gen_big, "gourdon": gen_gourdon}
+             "batch": gen_batch, "alpha": gen_alpha, "big": gen_big, "gourdon": gen_gourdon,
+             "nth": gen_nth}
     for name, fn in steps.items():
         if (a.only and a.only != name) or (not a.only and name == "big"):
             continue

@@ -106,3 +106,20 @@ def test_term_errors(pc):
     with pytest.raises(pc.PrimecountError):
         pc.AC(10 ** 12, 76500, 1000, 8)          # z < y
     assert pc.pi(10 ** 12) == 37607912018        # still fine afterwards
+
+
+def test_nth_prime_matches_reference(pc, oracle):
+    # nth_prime(n) printed by the reference (tests/golden/nth_prime.json):
+    # the tiny range, the small-table range, powers of ten to 1e16 and seeded
+    # n = 10^(8 + 9u); below pi(1e8) also against a plain sieve
+    rows = golden("nth_prime.json")["rows"]
+    for r in rows:
+        n, p = int(r["n"]), int(r["p"])
+        log("nth_prime(%d)" % n)
+        assert pc.nth_prime(n) == p, n
+    # consistency: pi(nth_prime(n)) == n and nth_prime(pi(p)) == p for primes p
+    for n in (5761456, 10 ** 9 + 7, 123456789012):
+        p = pc.nth_prime(n)
+        assert pc.pi(p) == n and pc.pi(p - 1) == n - 1, n
+    with pytest.raises(pc.PrimecountError):
+        pc.nth_prime(0)
