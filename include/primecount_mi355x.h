@@ -194,27 +194,31 @@ int pcg_last_pi_stats(pcg_pi_stats* out);
  *
  * A context owns a worker thread.  pcg_submit queues queries from any thread
  * and returns at once; whenever the device is free the worker runs everything
- * queued (up to max_batch queries) as one batch -- one persistent launch per
- * table set, as pcg_D_batch -- so queries that arrive during a launch join the
- * next one.  pcg_poll hands out finished results in completion order.  No
+ * queued (up to max_batch queries) as one batch -- D / S2_hard in one
+ * persistent launch per table set, as pcg_D_batch; pi(x) and nth_prime with
+ * the D parts of all of them in one launch -- so queries that arrive during a
+ * launch join the next one.  pcg_poll hands out finished results in completion order.  No
  * counterpart in the reference, whose entry points compute one x per call
  * (D / S2_hard, include/gourdon.hpp:28,38, include/S.hpp:25,32). */
 
 #define PCG_KIND_D 0
 #define PCG_KIND_S2_HARD 1
+#define PCG_KIND_PI 2          /* pi(x)                                       */
+#define PCG_KIND_NTH_PRIME 3   /* the x-th prime (x < 2^63)                   */
 
 typedef struct pcg_query {
-  int32_t kind;        /* PCG_KIND_D or PCG_KIND_S2_HARD                       */
+  int32_t kind;        /* PCG_KIND_*                                           */
   int32_t reserved;    /* 0                                                    */
-  pcg_i128 x;
-  int64_t y, z, k;     /* all 0: the reference's default parameters for x     */
+  pcg_i128 x;          /* x, or n for PCG_KIND_NTH_PRIME                      */
+  int64_t y, z, k;     /* D / S2_hard: all 0 = the reference's default
+                          parameters for x; must be 0 for pi and nth_prime    */
 } pcg_query;
 
 typedef struct pcg_result {
   uint64_t id;         /* as returned by pcg_submit                            */
   int32_t status;      /* PCG_OK or a negative PCG_E_* code                    */
   int32_t reserved;
-  pcg_i128 value;      /* D or S2_hard when status == PCG_OK                   */
+  pcg_i128 value;      /* the answer when status == PCG_OK                     */
   double seconds;      /* submit to result available                           */
 } pcg_result;
 

@@ -94,6 +94,8 @@ class PiStats(ctypes.Structure):
 
 KIND_D = 0
 KIND_S2_HARD = 1
+KIND_PI = 2
+KIND_NTH_PRIME = 3
 
 
 def library_path():

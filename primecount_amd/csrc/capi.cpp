@@ -265,29 +265,40 @@ int pcg_submit(pcg_ctx* ctx, const pcg_query* q, size_t n, uint64_t* ids)
     if (n && (!q || !ids)) throw pcg_error("q and ids must not be NULL");
     Server* sv = reinterpret_cast<Server*>(ctx);
     for (size_t i = 0; i < n; i++) {
-      HslQuery hq;
+      ServeQuery sq;
       int status = PCG_OK;
       std::string err;
       try {
-        if (q[i].kind != PCG_KIND_D && q[i].kind != PCG_KIND_S2_HARD)
-          throw pcg_error("unknown query kind");
-        const LeafSum kind = q[i].kind == PCG_KIND_D ? LeafSum::D : LeafSum::S2_hard;
         const i128 x = from_abi(q[i].x);
-        if (q[i].y == 0 && q[i].z == 0 && q[i].k == 0) {
-          hq = default_query(kind, x);
+        const bool dflt = q[i].y == 0 && q[i].z == 0 && q[i].k == 0;
+        if (q[i].kind == PCG_KIND_PI || q[i].kind == PCG_KIND_NTH_PRIME) {
+          if (!dflt) throw pcg_error("pi and nth_prime queries take no parameters");
+          sq.kind = q[i].kind == PCG_KIND_PI ? ServeQuery::PI : ServeQuery::NTH_PRIME;
+          sq.arg = x;
+          if (sq.kind == ServeQuery::NTH_PRIME && (x < 1 || x > (i128) 216289611853439384LL))
+            throw pcg_error("nth_prime(n): n must be in [1, 216289611853439384]");
+          if (sq.kind == ServeQuery::PI && x < 0)
+            throw pcg_error("pi(x): x must be >= 0");
+        } else if (q[i].kind == PCG_KIND_D || q[i].kind == PCG_KIND_S2_HARD) {
+          const LeafSum kind = q[i].kind == PCG_KIND_D ? LeafSum::D : LeafSum::S2_hard;
+          if (dflt) {
+            sq.q = default_query(kind, x);
+          } else {
+            sq.q.kind = kind;
+            sq.q.x = x;
+            sq.q.y = q[i].y;
+            sq.q.z = q[i].z;
+            sq.q.k = q[i].k;
+          }
+          hsl_validate(sq.q);
         } else {
-          hq.kind = kind;
-          hq.x = x;
-          hq.y = q[i].y;
-          hq.z = q[i].z;
-          hq.k = q[i].k;
+          throw pcg_error("unknown query kind");
         }
-        hsl_validate(hq);
       } catch (const std::exception& e) {
         status = status_of(e);
         err = e.what();
       }
-      ids[i] = sv->submit(hq, status, err);
+      ids[i] = sv->submit(sq, status, err);
     }
   });
 }

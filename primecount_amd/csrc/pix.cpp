@@ -368,36 +368,176 @@ uint64_t riemann_r_inverse(uint64_t n)
 
 } // namespace
 
-uint64_t nth_prime(uint64_t n, PixStats* st)
+i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st)
+{
+  check_params(x, y, z, k);
+  const int64_t x_star = get_x_star_gourdon(x, y);
+  const double t0 = now_s();
+  auto lock = engine_lock();
+  const EngineView v = engine_view(y, z, table_need(x, y, x_star));
+  const double t1 = now_s();
+  i128 r = 0;
+  uint64_t leaves = 0;
+  switch (term) {
+    case GourdonTerm::AC: r = run_ac(x, y, z, k, x_star, v, &leaves); break;
+    case GourdonTerm::B: r = run_B(x, y, v); break;
+    case GourdonTerm::Phi0: r = run_phi0(x, y, z, k, v); break;
+    case GourdonTerm::Sigma: r = run_sigma(x, y, x_star, v); break;
+  }
+  if (st) {
+    *st = PixStats();
+    st->seconds_table = t1 - t0;
+    st->seconds_total = now_s() - t0;
+    st->ac_leaves = leaves;
+    st->table_limit = v.pix_n;
+  }
+  return r;
+}
+
+std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st)
+{
+  PixStats s;
+  const double t0 = now_s();
+  std::vector<i128> result(xs.size(), 0);
+  // the tables of the largest query serve all (prefix valid)
+  int64_t ymax = 0;
+  uint64_t need = 0;
+  bool any = false;
+  std::vector<GourdonParams> gps(xs.size());
+  for (size_t i = 0; i < xs.size(); i++) {
+    const i128 x = xs[i];
+    if (x < 2) continue;
+    any = true;
+    if (x <= (i128) 100000000) {
+      // small x: the prime-count table answers directly (it is built for at
+      // least 2^20 numbers, with the primes up to the square root of that)
+      ymax = std::max<int64_t>(ymax, (int64_t) isqrt((u128) std::max<i128>(x, (i128) 1 << 20)) + 1);
+      need = std::max<uint64_t>(need, (uint64_t) x);
+    } else {
+      gps[i] = gourdon_params(x);
+      check_params(x, gps[i].y, gps[i].z, gps[i].k);
+      ymax = std::max<int64_t>(ymax, gps[i].z);
+      need = std::max<uint64_t>(need, table_need(x, gps[i].y, gps[i].x_star));
+    }
+  }
+  if (!any) {
+    if (st) *st = s;
+    return result;
+  }
+  std::vector<HslQuery> dq;
+  std::vector<size_t> di;
+  {
+    auto lock = engine_lock();
+    const EngineView v = engine_view(ymax, ymax, need);
+    double t = now_s();
+    s.seconds_table = t - t0;
+    s.table_limit = v.pix_n;
+    std::vector<uint64_t> small;
+    std::vector<size_t> si;
+    for (size_t i = 0; i < xs.size(); i++)
+      if (xs[i] >= 2 && xs[i] <= (i128) 100000000) {
+        small.push_back((uint64_t) xs[i]);
+        si.push_back(i);
+      }
+    if (!small.empty()) {
+      const std::vector<uint64_t> p = pi_points(v, small);
+      for (size_t j = 0; j < si.size(); j++) result[si[j]] = (i128) p[j];
+    }
+    for (size_t i = 0; i < xs.size(); i++) {
+      const i128 x = xs[i];
+      if (x <= (i128) 100000000) continue;
+      const GourdonParams& gp = gps[i];
+      double t1 = now_s();
+      result[i] += run_sigma(x, gp.y, gp.x_star, v);
+      double t2 = now_s(); s.seconds_sigma += t2 - t1; t1 = t2;
+      result[i] += run_phi0(x, gp.y, gp.z, gp.k, v);
+      t2 = now_s(); s.seconds_phi0 += t2 - t1; t1 = t2;
+      uint64_t leaves = 0;
+      result[i] += run_ac(x, gp.y, gp.z, gp.k, gp.x_star, v, &leaves);
+      s.ac_leaves += leaves;
+      t2 = now_s(); s.seconds_ac += t2 - t1; t1 = t2;
+      result[i] -= run_B(x, gp.y, v);
+      s.seconds_b += now_s() - t1;
+      HslQuery q;
+      q.kind = LeafSum::D;
+      q.x = x;
+      q.y = gp.y;
+      q.z = gp.z;
+      q.k = gp.k;
+      dq.push_back(q);
+      di.push_back(i);
+    }
+  }
+  if (!dq.empty()) {
+    // the hard special leaves of all queries in one persistent launch
+    const double t = now_s();
+    const std::vector<i128> d = hsl_compute_batch(dq, nullptr);
+    for (size_t j = 0; j < di.size(); j++) result[di[j]] += d[j];
+    s.seconds_d = now_s() - t;
+  }
+  s.seconds_total = now_s() - t0;
+  if (st) *st = s;
+  return result;
+}
+
+i128 pi_gourdon(i128 x, PixStats* st)
+{
+  return pi_gourdon_batch({x}, st)[0];
+}
+
+std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st)
 {
   // primes below 2^63, as the reference's 64-bit nth_prime (src/nth_prime.cpp:95-100)
   const uint64_t max_n = 216289611853439384ull;
-  if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
-  if (n > max_n) throw pcg_error("nth_prime(n): n must be <= 216289611853439384");
+  for (uint64_t n : ns) {
+    if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
+    if (n > max_n) throw pcg_error("nth_prime(n): n must be <= 216289611853439384");
+  }
   const double t0 = now_s();
   PixStats s;
-  uint64_t result = 0;
-  if (n <= 3) {
-    result = n == 1 ? 2 : (n == 2 ? 3 : 5);
-  } else if (n <= 5761455) {                       // pi(10^8): the table answers
+  std::vector<uint64_t> result(ns.size(), 0);
+  std::vector<size_t> table_i, big_i;
+  for (size_t i = 0; i < ns.size(); i++) {
+    const uint64_t n = ns[i];
+    if (n <= 3) result[i] = n == 1 ? 2 : (n == 2 ? 3 : 5);
+    else if (n <= 5761455) table_i.push_back(i);           // pi(10^8): the table answers
+    else big_i.push_back(i);
+  }
+  if (!table_i.empty()) {
     auto lock = engine_lock();
     const EngineView v = engine_view(10001, 10001, 100000000);
     uint64_t* dout = (uint64_t*) engine_buffer(2, 8);
-    PIX_HIP(pix_launch_select(v.pix_bits, v.pix_cnt, v.pix_n / 240 + 1, n, dout, v.stream));
-    PIX_HIP(hipMemcpyAsync(&result, dout, 8, hipMemcpyDeviceToHost, v.stream));
-    PIX_HIP(hipStreamSynchronize(v.stream));
-  } else {
-    const uint64_t x0 = riemann_r_inverse(n);
+    for (size_t i : table_i) {
+      PIX_HIP(pix_launch_select(v.pix_bits, v.pix_cnt, v.pix_n / 240 + 1, ns[i], dout, v.stream));
+      PIX_HIP(hipMemcpyAsync(&result[i], dout, 8, hipMemcpyDeviceToHost, v.stream));
+      PIX_HIP(hipStreamSynchronize(v.stream));
+    }
+  }
+  if (big_i.empty()) {
+    s.seconds_total = now_s() - t0;
+    if (st) *st = s;
+    return result;
+  }
+  // pi at the starting points of all queries in one batch
+  std::vector<i128> x0s;
+  for (size_t i : big_i) {
+    const uint64_t x0 = riemann_r_inverse(ns[i]);
     if (x0 >= ((uint64_t) 1 << 62)) throw pcg_error("nth_prime(n): n is too large");
-    PixStats ps;
-    const uint64_t c = (uint64_t) pi_gourdon((i128) x0, &ps);
-    s = ps;
-    const GourdonParams gp = gourdon_params((i128) x0);
+    x0s.push_back((i128) x0);
+  }
+  const std::vector<i128> cs = pi_gourdon_batch(x0s, &s);
+  int64_t ymax = 0;
+  for (i128 x0 : x0s) ymax = std::max<int64_t>(ymax, gourdon_params(x0).z);
+  auto lock = engine_lock();
+  for (size_t j = 0; j < big_i.size(); j++) {
+    const uint64_t n = ns[big_i[j]];
+    const uint64_t x0 = (uint64_t) x0s[j];
+    const uint64_t c = (uint64_t) cs[j];
+    uint64_t res = 0;
     const bool down = c >= n;
     const uint64_t k = down ? c - n + 1 : n - c;   // rank from x0 downward / upward
     const double lx = std::log((double) x0);
     uint64_t delta = (uint64_t) ((double) (k + 16) * lx * 2.0) + 100000;
-    auto lock = engine_lock();
     for (;;) {
       uint64_t lo, hi;
       if (down) {
@@ -408,7 +548,7 @@ uint64_t nth_prime(uint64_t n, PixStats* st)
         hi = x0 + delta;
       }
       const uint64_t r = isqrt((u128) hi);
-      const EngineView v = engine_view(gp.y, gp.z, r + 1);
+      const EngineView v = engine_view(ymax, ymax, r + 1);
       const std::vector<uint64_t> pr = pi_points(v, {r, 15359});
       const uint64_t n_all = pr[0] >= 3 ? pr[0] - 3 : 0;       // sieving primes >= 7
       const uint64_t n_small = std::min<uint64_t>(pr[1] - 3, n_all);
@@ -450,102 +590,33 @@ uint64_t nth_prime(uint64_t n, PixStats* st)
       PIX_HIP(hipStreamSynchronize(v.stream));
       static const uint8_t wheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
       const uint64_t base = (lo / 240 + w0) * 240;
-      for (uint64_t i = 0; i < nw && !result; i++) {
+      for (uint64_t i = 0; i < nw && !res; i++) {
         const uint64_t wi = down ? nw - 1 - i : i;
         const uint64_t wv = words[wi];
         const uint64_t c1 = (uint64_t) __builtin_popcountll(wv);
         if (c1 < need) { need -= c1; continue; }
-        for (int j = 0; j < 64; j++) {
-          const int bit = down ? 63 - j : j;
+        for (int jb = 0; jb < 64; jb++) {
+          const int bit = down ? 63 - jb : jb;
           if (!((wv >> bit) & 1)) continue;
           if (--need == 0) {
-            result = base + 240 * wi + 30 * (bit >> 3) + wheel[bit & 7];
+            res = base + 240 * wi + 30 * (bit >> 3) + wheel[bit & 7];
             break;
           }
         }
       }
-      if (!result) throw pcg_error("internal: nth_prime selection");
+      if (!res) throw pcg_error("internal: nth_prime selection");
       break;
     }
+    result[big_i[j]] = res;
   }
   s.seconds_total = now_s() - t0;
   if (st) *st = s;
   return result;
 }
 
-i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st)
+uint64_t nth_prime(uint64_t n, PixStats* st)
 {
-  check_params(x, y, z, k);
-  const int64_t x_star = get_x_star_gourdon(x, y);
-  const double t0 = now_s();
-  auto lock = engine_lock();
-  const EngineView v = engine_view(y, z, table_need(x, y, x_star));
-  const double t1 = now_s();
-  i128 r = 0;
-  uint64_t leaves = 0;
-  switch (term) {
-    case GourdonTerm::AC: r = run_ac(x, y, z, k, x_star, v, &leaves); break;
-    case GourdonTerm::B: r = run_B(x, y, v); break;
-    case GourdonTerm::Phi0: r = run_phi0(x, y, z, k, v); break;
-    case GourdonTerm::Sigma: r = run_sigma(x, y, x_star, v); break;
-  }
-  if (st) {
-    *st = PixStats();
-    st->seconds_table = t1 - t0;
-    st->seconds_total = now_s() - t0;
-    st->ac_leaves = leaves;
-    st->table_limit = v.pix_n;
-  }
-  return r;
-}
-
-i128 pi_gourdon(i128 x, PixStats* st)
-{
-  PixStats s;
-  const double t0 = now_s();
-  i128 result = 0;
-  if (x < 2) {
-    result = 0;
-  } else if (x <= (i128) 100000000) {
-    // small x: the prime-count table answers directly
-    auto lock = engine_lock();
-    // (the table is built for at least 2^20 numbers: primes to sqrt of that)
-    const int64_t r = (int64_t) isqrt((u128) std::max<i128>(x, (i128) 1 << 20)) + 1;
-    const EngineView v = engine_view(r, r, (uint64_t) x);
-    s.seconds_table = now_s() - t0;
-    result = (i128) pi_points(v, {(uint64_t) x})[0];
-    s.table_limit = v.pix_n;
-  } else {
-    const GourdonParams gp = gourdon_params(x);
-    check_params(x, gp.y, gp.z, gp.k);
-    {
-      auto lock = engine_lock();
-      const EngineView v = engine_view(gp.y, gp.z, table_need(x, gp.y, gp.x_star));
-      double t = now_s();
-      s.seconds_table = t - t0;
-      s.table_limit = v.pix_n;
-      result += run_sigma(x, gp.y, gp.x_star, v);
-      double t2 = now_s(); s.seconds_sigma = t2 - t; t = t2;
-      result += run_phi0(x, gp.y, gp.z, gp.k, v);
-      t2 = now_s(); s.seconds_phi0 = t2 - t; t = t2;
-      result += run_ac(x, gp.y, gp.z, gp.k, gp.x_star, v, &s.ac_leaves);
-      t2 = now_s(); s.seconds_ac = t2 - t; t = t2;
-      result -= run_B(x, gp.y, v);
-      s.seconds_b = now_s() - t;
-    }
-    const double t = now_s();
-    HslQuery q;
-    q.kind = LeafSum::D;
-    q.x = x;
-    q.y = gp.y;
-    q.z = gp.z;
-    q.k = gp.k;
-    result += hsl_compute(q, nullptr);
-    s.seconds_d = now_s() - t;
-  }
-  s.seconds_total = now_s() - t0;
-  if (st) *st = s;
-  return result;
+  return nth_prime_batch({n}, st)[0];
 }
 
 } // namespace pcg

@@ -24,10 +24,17 @@ i128 gourdon_term(GourdonTerm t, i128 x, int64_t y, int64_t z, int64_t k, PixSta
 /// pi(x) with the reference's default parameters.  Throws pcg_error.
 i128 pi_gourdon(i128 x, PixStats* st);
 
+/// pi(x) for many x: the terms other than D query by query over shared
+/// resident tables, D for all queries in one persistent launch.
+std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st);
+
 /// The n-th prime, n <= 216289611853439384 (primes below 2^63): pi(x0) at
 /// x0 = R^-1(n), then a sieve of the interval between x0 and the answer on
 /// the device (src/nth_prime.cpp:95-145, nth_prime_sieve.cpp).  Throws
 /// pcg_error.
 uint64_t nth_prime(uint64_t n, PixStats* st);
+
+/// nth_prime for many n (pi at the starting points as one pi_gourdon_batch).
+std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st);
 
 } // namespace pcg

@@ -2,6 +2,8 @@
 
 #include "serve.hpp"
 
+#include "pix.hpp"
+
 #include <chrono>
 #include <new>
 
@@ -44,7 +46,7 @@ Server::~Server()
   worker_.join();
 }
 
-uint64_t Server::submit(const HslQuery& q, int status, const std::string& error)
+uint64_t Server::submit(const ServeQuery& q, int status, const std::string& error)
 {
   const double t = now_s();
   uint64_t id;
@@ -106,23 +108,46 @@ void Server::run()
       }
     }
     const double t0 = now_s();
-    std::vector<HslQuery> qs;
-    qs.reserve(batch.size());
-    for (const auto& p : batch) qs.push_back(p.q);
     std::vector<ServeResult> res(batch.size());
-    try {
-      std::vector<i128> v = hsl_compute_batch(qs, nullptr, max_slots_);
-      for (size_t i = 0; i < batch.size(); i++) res[i].value = v[i];
-    } catch (const std::exception&) {
-      for (size_t i = 0; i < batch.size(); i++) {
-        try {
-          res[i].value = hsl_compute_batch({qs[i]}, nullptr, max_slots_)[0];
-        } catch (const std::exception& e) {
-          res[i].status = status_of(e);
-          res[i].error = e.what();
+    // one batch per kind of query
+    auto run_kind = [&](ServeQuery::Kind kind) {
+      std::vector<size_t> idx;
+      for (size_t i = 0; i < batch.size(); i++)
+        if (batch[i].q.kind == kind) idx.push_back(i);
+      if (idx.empty()) return;
+      auto compute = [&](const std::vector<size_t>& part) {
+        std::vector<i128> v;
+        if (kind == ServeQuery::HSL) {
+          std::vector<HslQuery> qs;
+          for (size_t i : part) qs.push_back(batch[i].q.q);
+          v = hsl_compute_batch(qs, nullptr, max_slots_);
+        } else if (kind == ServeQuery::PI) {
+          std::vector<i128> xs;
+          for (size_t i : part) xs.push_back(batch[i].q.arg);
+          v = pi_gourdon_batch(xs, nullptr);
+        } else {
+          std::vector<uint64_t> ns;
+          for (size_t i : part) ns.push_back((uint64_t) batch[i].q.arg);
+          for (uint64_t p : nth_prime_batch(ns, nullptr)) v.push_back((i128) p);
+        }
+        for (size_t j = 0; j < part.size(); j++) res[part[j]].value = v[j];
+      };
+      try {
+        compute(idx);
+      } catch (const std::exception&) {
+        for (size_t i : idx) {
+          try {
+            compute({i});
+          } catch (const std::exception& e) {
+            res[i].status = status_of(e);
+            res[i].error = e.what();
+          }
         }
       }
-    }
+    };
+    run_kind(ServeQuery::HSL);
+    run_kind(ServeQuery::PI);
+    run_kind(ServeQuery::NTH_PRIME);
     const double t1 = now_s();
     {
       std::lock_guard<std::mutex> lock(mu_);

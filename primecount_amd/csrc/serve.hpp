@@ -1,9 +1,11 @@
 // serve.hpp -- continuous-batching query server over the engine.
 //
 // Queries arrive from any thread (submit); one worker thread takes everything
-// that is pending when the device becomes free and runs it as one batch
-// (hsl_compute_batch: one persistent launch per table set), so arrivals during
-// a launch join the next one.  Results are collected with poll.
+// that is pending when the device becomes free and runs it as one batch, so
+// arrivals during a launch join the next one: D / S2_hard queries in one
+// persistent launch per table set (hsl_compute_batch), pi(x) and nth_prime
+// queries with their D parts in one launch (pi_gourdon_batch,
+// nth_prime_batch).  Results are collected with poll.
 #pragma once
 
 #include "engine.hpp"
@@ -25,6 +27,14 @@ struct ServeResult {
   std::string error;
 };
 
+/// One query: kind D / S2_hard with parameters in `q`, or pi(x) / nth_prime
+/// with the argument in `arg`.
+struct ServeQuery {
+  enum Kind { HSL = 0, PI = 1, NTH_PRIME = 2 } kind = HSL;
+  HslQuery q;
+  i128 arg = 0;
+};
+
 struct ServeStats {
   uint64_t submitted = 0, completed = 0, batches = 0, max_batch = 0;
   double busy_seconds = 0;  // worker time inside batches
@@ -41,7 +51,7 @@ public:
 
   /// Queues a validated query (status == PCG_OK) or records an immediate
   /// error result for it.  Returns the query id.
-  uint64_t submit(const HslQuery& q, int status, const std::string& error);
+  uint64_t submit(const ServeQuery& q, int status, const std::string& error);
   /// Moves up to `max` finished results into `out`, waiting up to
   /// timeout_ms (< 0: forever) for the first one.
   size_t poll(std::vector<ServeResult>& out, size_t max, int timeout_ms);
@@ -50,7 +60,7 @@ public:
 private:
   struct Pending {
     uint64_t id;
-    HslQuery q;
+    ServeQuery q;
     double t_submit;
   };
   void run();

@@ -252,6 +252,56 @@ def gen_nth():
               open(os.path.join(HERE, "nth_prime.json"), "w"), indent=1)
 
 
+SERVE_SEED = 0xC5C5
+
+
+def serve_queries(seed=SERVE_SEED, n=4096):
+    """The serving mix: per query two SplitMix64 draws u1, u2 in [0, 1);
+    u1 < 0.8 gives pi(x) with x = floor(10^(9 + 10 u2)), otherwise
+    nth_prime(n) with n = floor(10^(8 + 9 u2))."""
+    qs, st = [], seed
+    for _ in range(n):
+        st, r1 = splitmix64(st)
+        st, r2 = splitmix64(st)
+        u1 = (r1 >> 11) * (2.0 ** -53)
+        u2 = (r2 >> 11) * (2.0 ** -53)
+        if u1 < 0.8:
+            qs.append(("pi", int(10.0 ** (9.0 + 10.0 * u2))))
+        else:
+            qs.append(("nth", int(10.0 ** (8.0 + 9.0 * u2))))
+    return qs
+
+
+def gen_serve(threads=7):
+    """Answers of the 4096-query serving mix from the reference CLI
+    (`primecount X` and `primecount N --nth-prime`).  Progress goes to a
+    JSON-lines file under _work/ so that an interrupted run resumes."""
+    qs = serve_queries()
+    part = os.path.join(REPO, "_work", "serve_c5.partial")
+    os.makedirs(os.path.dirname(part), exist_ok=True)
+    done = {}
+    if os.path.exists(part):
+        for line in open(part):
+            r = json.loads(line)
+            done[(r["kind"], r["arg"])] = r["value"]
+    with open(part, "a") as f:
+        for i, (kind, arg) in enumerate(qs):
+            if (kind, str(arg)) in done:
+                continue
+            args = [CLI, str(arg)] + (["--nth-prime"] if kind == "nth" else []) + ["-t", str(threads)]
+            out = subprocess.run(args, capture_output=True, text=True, timeout=3600,
+                                 check=True).stdout.strip()
+            done[(kind, str(arg))] = out.splitlines()[-1].strip()
+            f.write(json.dumps({"kind": kind, "arg": str(arg), "value": done[(kind, str(arg))]}) + "\n")
+            f.flush()
+            if i % 64 == 0:
+                print(i, kind, arg, done[(kind, str(arg))], flush=True)
+    rows = [{"kind": k, "arg": str(a), "value": done[(k, str(a))]} for k, a in qs]
+    json.dump({"source": "oracle/_ref primecount X / primecount N --nth-prime",
+               "seed": SERVE_SEED, "rows": rows},
+              open(os.path.join(HERE, "serve_c5.json"), "w"), indent=0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also run x = 1e21 in `configs`")
@@ -261,9 +311,9 @@ def main():
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     steps = {"kat": gen_kat, "params": gen_params, "configs": lambda: gen_configs(a.big),
              "batch": gen_batch, "alpha": gen_alpha, "big": gen_big, "gourdon": gen_gourdon,
-             "nth": gen_nth}
+             "nth": gen_nth, "serve": gen_serve}
     for name, fn in steps.items():
-        if (a.only and a.only != name) or (not a.only and name == "big"):
+        if (a.only and a.only != name) or (not a.only and name in ("big", "serve")):
             continue
         print("==", name, flush=True)
         fn()

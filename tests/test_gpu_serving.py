@@ -99,3 +99,36 @@ def test_serving_S2_hard_and_explicit_parameters(pc):
         assert got[ids[1]]["value"] == int(cfg[10 ** 15]["value"])
         for i, row in zip(ids[2:], kat):
             assert got[i]["status"] == 0 and got[i]["value"] == row[4], row
+
+
+def test_serving_pi_and_nth_prime(pc, oracle):
+    """pi(x) and nth_prime queries in a serving context, mixed with D queries
+    in the same submit: the README pi(10^n) values, the reference's
+    nth_prime values, and the prime-count sieve for small x."""
+    known = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
+             if int(r["x"]) <= 10 ** 17]
+    nth = [(int(r["n"]), int(r["p"])) for r in golden("nth_prime.json")["rows"]
+           if int(r["n"]) <= 10 ** 15]
+    small = [0, 1, 2, 100, 7919, 10 ** 6, 10 ** 8 + 7]
+    drow = fixture_rows(10 ** 14)[:4]
+    with pc.Server() as sv:
+        log("serving: %d pi, %d nth_prime, %d D queries in one submit"
+            % (len(known) + len(small), len(nth), len(drow)))
+        qs = [(pc.KIND_PI, x) for x, _ in known] + [(pc.KIND_PI, x) for x in small]
+        qs += [(pc.KIND_NTH_PRIME, n) for n, _ in nth] + [(pc.KIND_D, x) for x, _ in drow]
+        want = [v for _, v in known] + [oracle.pi_sieve(x) for x in small]
+        want += [p for _, p in nth] + [v for _, v in drow]
+        ids = sv.submit(qs)
+        # parameters on a pi query, and n = 0, are answered with an error
+        bad = sv.submit([(pc.KIND_PI, 10 ** 12, 1, 2, 3), (pc.KIND_NTH_PRIME, 0)])
+        got = {r["id"]: r for r in sv.poll_all(len(ids) + 2, timeout_ms=180000)}
+        assert len(got) == len(ids) + 2
+        for i, q, w in zip(ids, qs, want):
+            assert got[i]["status"] == 0 and got[i]["value"] == w, q
+        assert all(got[i]["status"] < 0 for i in bad)
+        # the same queries again, one per submit, arriving while launches run
+        log("serving: the same queries one by one")
+        ids2 = [sv.submit([q])[0] for q in qs]
+        got = {r["id"]: r for r in sv.poll_all(len(ids2), timeout_ms=180000)}
+        for i, q, w in zip(ids2, qs, want):
+            assert got[i]["status"] == 0 and got[i]["value"] == w, q
