@@ -208,10 +208,10 @@ void add_desc(std::vector<PixLeafDesc>& list, uint64_t& total, uint64_t xp, uint
   if (i_hi < i_lo) return;
   PixLeafDesc d;
   d.xp = xp;
-  d.q1 = q1;
+  d.xd = (double) xp;
+  d.q1 = (uint32_t) std::min<uint64_t>(q1, 0xffffffffull);
   d.off = total;
   d.i_lo = (uint32_t) i_lo;
-  d.pad = 0;
   list.push_back(d);
   total += i_hi - i_lo + 1;
 }

@@ -247,27 +247,55 @@ __global__ __launch_bounds__(kThreads) void pix_count_kernel(const uint64_t* __r
 // ---------------------------------------------------------------------------
 // leaf sums
 
+/// floor(a / d) for 1 <= d < 2^32 and a quotient below 2^49; ad is a rounded
+/// to double.  The reciprocal estimate gets two Newton steps (relative error
+/// then near 2^-52, so the truncated product is within one of the quotient);
+/// the residual a - q d, exact modulo 2^64 because |a - q d| < 2 d, fixes it.
+/// qd receives the quotient as a double.
+__device__ __forceinline__ uint64_t div49(uint64_t a, double ad, uint32_t d, double* qd)
+{
+  const double dd = (double) d;
+  double r = __builtin_amdgcn_rcp(dd);
+  r = __builtin_fma(r, __builtin_fma(-dd, r, 1.0), r);
+  r = __builtin_fma(r, __builtin_fma(-dd, r, 1.0), r);
+  const double qf = __builtin_trunc(ad * r);
+  // the integer in the low 52 bits of qf + 2^52
+  const uint64_t q = __builtin_bit_cast(uint64_t, qf + 4503599627370496.0) & ((1ull << 52) - 1);
+  const int64_t rem = (int64_t) (a - q * d);
+  const int32_t fix = rem < 0 ? -1 : ((uint64_t) rem >= d ? 1 : 0);
+  *qd = qf + (double) fix;
+  return q + (int64_t) fix;
+}
+
 /// Sum over the flat terms of all descriptors of w * pi(xp / primes[i]).
-/// Each wave takes chunks of 1,024 consecutive terms; a lane finds the
-/// descriptor of its first term by binary search and then steps forward.
+/// Each wave takes one contiguous range of terms: a lane finds the
+/// descriptor of its first term by binary search and then steps forward by
+/// 64 terms.  A lane works on kGroup terms at a time so that their prime
+/// loads, and then their table loads, are in flight together; the masks of
+/// the table words sit in LDS.
 __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
-    const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total,
+    const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total, uint64_t per_wave,
     const uint32_t* __restrict__ primes, const uint64_t* __restrict__ bits,
     const uint64_t* __restrict__ cnt, unsigned long long* __restrict__ acc)
 {
-  constexpr uint32_t kSteps = 16;
+  constexpr uint32_t kGroup = 8;
+  __shared__ uint64_t masks[240];
+  for (uint32_t r = threadIdx.x; r < 240; r += kThreads)
+    masks[r] = le_mask(r);
+  __syncthreads();
+
   const uint32_t lane = threadIdx.x & 63;
   const uint64_t wave = ((uint64_t) blockIdx.x * kThreads + threadIdx.x) >> 6;
-  const uint64_t nwaves = ((uint64_t) gridDim.x * kThreads) >> 6;
-  const uint64_t chunk = 64ull * kSteps;
+  const uint64_t t0 = wave * per_wave;
+  const uint64_t t1 = t0 + per_wave < total ? t0 + per_wave : total;
   u128 sum = 0;
-  for (uint64_t c0 = wave * chunk; c0 < total; c0 += nwaves * chunk) {
-    uint64_t t = c0 + lane;
-    // last descriptor with off <= t
+  if (t0 < total) {
+    // last descriptor with off <= t (clamped to the last term)
+    const uint64_t tf = t0 + lane < t1 ? t0 + lane : t1 - 1;
     uint32_t lo = 0, hi = ndesc - 1;
     while (lo < hi) {
       const uint32_t mid = (lo + hi + 1) >> 1;
-      if (desc[mid].off <= t)
+      if (desc[mid].off <= tf)
         lo = mid;
       else
         hi = mid - 1;
@@ -275,16 +303,58 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
     uint32_t d = lo;
     PixLeafDesc ds = desc[d];
     uint64_t next = d + 1 < ndesc ? desc[d + 1].off : total;
-    for (uint32_t s = 0; s < kSteps && t < total; s++, t += 64) {
-      while (t >= next) {
-        d++;
-        ds = desc[d];
-        next = d + 1 < ndesc ? desc[d + 1].off : total;
+    for (uint64_t tb = t0 + lane; tb < t1 + lane; tb += 64 * kGroup) {
+      uint64_t xp[kGroup];
+      double xd[kGroup];
+      uint32_t idx[kGroup], q1[kGroup], q[kGroup];
+#pragma unroll
+      for (uint32_t s = 0; s < kGroup; s++) {
+        const uint64_t t = tb + 64 * s;
+        const bool ok = t < t1;
+        if (ok) {
+          while (t >= next) {
+            d++;
+            ds = desc[d];
+            next = d + 1 < ndesc ? desc[d + 1].off : total;
+          }
+        }
+        xp[s] = ok ? ds.xp : 0;
+        xd[s] = ok ? ds.xd : 0.0;
+        q1[s] = ds.q1;
+        idx[s] = ok ? ds.i_lo + (uint32_t) (t - ds.off) : ds.i_lo;
       }
-      const uint32_t q = primes[ds.i_lo + (uint32_t) (t - ds.off)];
-      const uint64_t v = div64_32(ds.xp, q);
-      const uint64_t pv = pix_pi(bits, cnt, v);
-      sum += q <= ds.q1 ? pv : 2 * pv;
+#pragma unroll
+      for (uint32_t s = 0; s < kGroup; s++)
+        q[s] = primes[idx[s]];
+      uint64_t w[kGroup], bw[kGroup], cw[kGroup];
+      uint32_t rr[kGroup];
+#pragma unroll
+      for (uint32_t s = 0; s < kGroup; s++) {
+        const uint32_t qs = xp[s] ? q[s] : 1u;
+        double vd;
+        const uint64_t v = div49(xp[s], xd[s], qs, &vd);
+        // w = floor(v / 240): (v + 1/2) / 240 is at least 1/480 from an
+        // integer, far more than the rounding error for v < 2^49
+        const double wf = __builtin_trunc(__builtin_fma(vd, 1.0 / 240.0, 0.5 / 240.0));
+        w[s] = __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
+        rr[s] = (uint32_t) v - 240u * (uint32_t) w[s];
+        xp[s] = v;
+      }
+#pragma unroll
+      for (uint32_t s = 0; s < kGroup; s++) {
+        bw[s] = bits[w[s]];
+        cw[s] = cnt[w[s]];
+      }
+      uint64_t part = 0;
+#pragma unroll
+      for (uint32_t s = 0; s < kGroup; s++) {
+        const uint64_t v = xp[s];
+        // pi(v) for v < 7 from a nibble table: 0, 0, 1, 2, 2, 3, 3
+        const uint64_t pv = v < 7 ? (uint64_t) ((0x3322100u >> (4 * (uint32_t) v)) & 15)
+                                  : 3 + cw[s] + (uint64_t) __popcll(bw[s] & masks[rr[s]]);
+        part += q[s] <= q1[s] ? pv : 2 * pv;
+      }
+      sum += part;
     }
   }
   block_add_i128(acc, (i128) sum);
@@ -595,10 +665,13 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
                               unsigned long long* acc, uint32_t cus, hipStream_t st)
 {
   if (total == 0 || ndesc == 0) return hipSuccess;
-  const uint64_t waves = (total + 1023) / 1024;
-  const uint32_t grid = (uint32_t) std::min<uint64_t>((waves + 3) / 4, (uint64_t) cus * 32);
+  // contiguous ranges of at least 4,096 terms, up to 64 waves per CU
+  const uint64_t max_waves = (uint64_t) cus * 64;
+  const uint64_t waves = std::min<uint64_t>((total + 4095) / 4096, max_waves);
+  const uint64_t per_wave = ((total + waves - 1) / waves + 63) / 64 * 64;
+  const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
   hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
-                     primes, bits, cnt, acc);
+                     per_wave, primes, bits, cnt, acc);
   return hipGetLastError();
 }
 

@@ -16,13 +16,14 @@ namespace pcg {
 
 /// Leaves of one sieving prime for the flat leaf-sum kernel: terms
 /// w * pi(xp / primes[i]) for i = i_lo .. i_lo + (next off - off) - 1, with
-/// w = 1 if primes[i] <= q1 and 2 otherwise.
+/// w = 1 if primes[i] <= q1 and 2 otherwise.  Every quotient xp / primes[i]
+/// is below 2^49 (it is covered by the prime-count table).
 struct alignas(16) PixLeafDesc {
   uint64_t xp;
-  uint64_t q1;
+  double xd;         // xp rounded to double
   uint64_t off;      // first flat term index of this descriptor
   uint32_t i_lo;
-  uint32_t pad;
+  uint32_t q1;       // primes are below 2^32: q1 = min(bound, 2^32 - 1)
 };
 
 constexpr uint32_t kPixSegWords = 4096;       // words per sieve segment (983,040 numbers)
