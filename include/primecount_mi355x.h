@@ -183,6 +183,9 @@ typedef struct pcg_pi_stats {
   double seconds_total;      /* wall time of the last pcg_pi call             */
   double seconds_table;      /* tables, including the prime-count table       */
   double seconds_sigma, seconds_phi0, seconds_ac, seconds_b, seconds_d;
+                             /* per term: device time between stream events
+                                (the terms run back to back; A/C includes the
+                                descriptor upload, D its host planning)       */
   uint64_t ac_leaves;        /* A and C leaves summed                         */
   uint64_t table_limit;      /* the prime-count table covers [0, table_limit] */
 } pcg_pi_stats;

@@ -741,6 +741,12 @@ std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* 
                                     uint64_t max_slots)
 {
   std::lock_guard<std::mutex> lock(g_mutex);
+  return hsl_compute_batch_locked(qs, stats, max_slots);
+}
+
+std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_stats* stats,
+                                           uint64_t max_slots)
+{
   std::vector<i128> out(qs.size(), 0);
   double t0 = now();
   pcg_stats acc{};

@@ -39,6 +39,11 @@ i128 hsl_compute(const HslQuery& q, pcg_stats* stats);
 std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats,
                                     uint64_t max_slots = 0);
 
+/// hsl_compute_batch for a caller that holds engine_lock(); work already
+/// queued on the engine stream runs first.
+std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_stats* stats,
+                                           uint64_t max_slots = 0);
+
 /// Throws pcg_error if the engine cannot take the query (domain checks only;
 /// no device work).
 void hsl_validate(const HslQuery& q);

@@ -21,7 +21,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
 #include <map>
+#include <thread>
 
 namespace pcg {
 
@@ -131,57 +133,97 @@ i128 phi_tiny_host(i128 x, const PhiTiny& t)
   return (i128) ((u / t.pp) * t.phi_pp + t.tbl[(uint32_t) (u % t.pp)]);
 }
 
+/// Sigma4 .. Sigma6 over the primes x_star < p <= x^(1/3), into acc[0..5].
+void launch_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v, unsigned long long* acc)
+{
+  const uint64_t x13 = iroot<3>((u128) x);
+  const uint64_t sqrt_xy = isqrt((u128) x / (u128) y);
+  const uint32_t b_lo = (uint32_t) v.host->pi((uint64_t) x_star) + 1;
+  const uint32_t b_hi = (uint32_t) v.host->pi(x13) + 1;
+  if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma");
+  PIX_HIP(pix_launch_sigma((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, sqrt_xy, v.primes,
+                           b_lo, b_hi, v.pix_bits, v.pix_cnt, acc, v.stream));
+}
+
+/// Sigma from a = pi(y), b = pi(x^(1/3)), c = pi(sqrt(x / y)), d = pi(x_star),
+/// pi(isqrt(x)) and the Sigma4 .. Sigma6 sums h[0..5].
+i128 sigma_value(i128 a, i128 b, i128 c, i128 d, i128 pi_sqrtx, const uint64_t* h)
+{
+  // Sigma0 .. Sigma3 (src/gourdon/Sigma.cpp:30-52)
+  const i128 s0 = a - 1 + (pi_sqrtx * (pi_sqrtx - 1)) / 2 - (a * (a - 1)) / 2;
+  const i128 s1 = (a - b) * (a - b - 1) / 2;
+  const i128 s2 = a * (b - c - (c * (c - 3)) / 2 + (d * (d - 3)) / 2);
+  const i128 s3 = (b * (b - 1) * (2 * b - 1)) / 6 - b - (d * (d - 1) * (2 * d - 1)) / 6 + d;
+  return s0 + s1 + s2 + s3 + acc_value(h) * a + acc_value(h + 2) - acc_value(h + 4);
+}
+
 i128 run_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v)
 {
   const uint64_t x13 = iroot<3>((u128) x);
   const uint64_t sqrt_xy = isqrt((u128) x / (u128) y);
   const std::vector<uint64_t> p = pi_points(v, {(uint64_t) y, x13, sqrt_xy, (uint64_t) x_star,
                                                 isqrt((u128) x)});
-  const i128 a = p[0], b = p[1], c = p[2], d = p[3], pi_sqrtx = p[4];
-  // Sigma0 .. Sigma3 (src/gourdon/Sigma.cpp:30-52)
-  const i128 s0 = a - 1 + (pi_sqrtx * (pi_sqrtx - 1)) / 2 - (a * (a - 1)) / 2;
-  const i128 s1 = (a - b) * (a - b - 1) / 2;
-  const i128 s2 = a * (b - c - (c * (c - 3)) / 2 + (d * (d - 3)) / 2);
-  const i128 s3 = (b * (b - 1) * (2 * b - 1)) / 6 - b - (d * (d - 1) * (2 * d - 1)) / 6 + d;
-  // Sigma4 .. Sigma6 over the primes x_star < p <= x^(1/3)
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
-  const uint32_t b_lo = (uint32_t) v.host->pi((uint64_t) x_star) + 1;
-  const uint32_t b_hi = (uint32_t) v.host->pi(x13) + 1;
-  if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma");
-  PIX_HIP(pix_launch_sigma((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, sqrt_xy, v.primes,
-                           b_lo, b_hi, v.pix_bits, v.pix_cnt, A.d, v.stream));
+  launch_sigma(x, y, x_star, v, A.d);
   acc_read(A, v.stream);
-  return s0 + s1 + s2 + s3 + acc_value(A.h) * a + acc_value(A.h + 2) - acc_value(A.h + 4);
+  return sigma_value(p[0], p[1], p[2], p[3], p[4], A.h);
 }
 
-i128 run_phi0(i128 x, int64_t y, int64_t z, int64_t k, const EngineView& v)
+/// The tiny phi table of k on the device (uploaded once per buffer).
+const uint32_t* phi_tiny_device(const PhiTiny& t, const EngineView& v)
 {
-  if (k < 5 || k > 8)
-    throw pcg_error("Phi0(x, y, z, k): the device path needs 5 <= k <= 8");
-  static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
-  const PhiTiny& t = phi_tiny_table((uint32_t) k);
   uint32_t* dtbl = (uint32_t*) engine_buffer(3, (size_t) t.pp * 4);
   static const void* uploaded = nullptr;     // the device copy stays until the buffer moves
   static uint32_t uploaded_k = 0;
   static uint64_t uploaded_gen = 0;
-  if (dtbl != uploaded || uploaded_k != (uint32_t) k || uploaded_gen != engine_generation()) {
+  if (dtbl != uploaded || uploaded_k != t.k || uploaded_gen != engine_generation()) {
     PIX_HIP(hipMemcpyAsync(dtbl, t.tbl.data(), (size_t) t.pp * 4, hipMemcpyHostToDevice, v.stream));
     PIX_HIP(hipStreamSynchronize(v.stream));
     uploaded = dtbl;
-    uploaded_k = (uint32_t) k;
+    uploaded_k = t.k;
     uploaded_gen = engine_generation();
   }
+  return dtbl;
+}
+
+/// The sum of mu(n) phi(x / n, k) over the table entries n <= z, into acc[0..1].
+void launch_phi0(i128 x, int64_t z, int64_t k, const EngineView& v, const PhiTiny& t,
+                 const uint32_t* dtbl, unsigned long long* acc)
+{
+  static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
   const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) z) + 1;
   if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for Phi0");
+  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, nidx,
+                          (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, acc, v.stream));
+}
+
+const PhiTiny& phi0_table(int64_t k)
+{
+  if (k < 5 || k > 8)
+    throw pcg_error("Phi0(x, y, z, k): the device path needs 5 <= k <= 8");
+  return phi_tiny_table((uint32_t) k);
+}
+
+i128 run_phi0(i128 x, int64_t y, int64_t z, int64_t k, const EngineView& v)
+{
+  (void) y;
+  const PhiTiny& t = phi0_table(k);
+  const uint32_t* dtbl = phi_tiny_device(t, v);
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
-  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, nidx,
-                          (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, A.d, v.stream));
+  launch_phi0(x, z, k, v, t, dtbl, A.d);
   acc_read(A, v.stream);
   return phi_tiny_host(x, t) + acc_value(A.h);
+}
+
+/// B: the sum of pi(x / p) over y < p <= isqrt(x), into acc[0..1].
+void launch_B(i128 x, int64_t y, const EngineView& v, unsigned long long* acc)
+{
+  PIX_HIP(pix_launch_B((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, isqrt((u128) x),
+                       v.pix_bits, v.pix_cnt, acc, v.stream));
 }
 
 i128 run_B(i128 x, int64_t y, const EngineView& v)
@@ -189,8 +231,7 @@ i128 run_B(i128 x, int64_t y, const EngineView& v)
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
-  PIX_HIP(pix_launch_B((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, isqrt((u128) x),
-                       v.pix_bits, v.pix_cnt, A.d, v.stream));
+  launch_B(x, y, v, A.d);
   acc_read(A, v.stream);
   return acc_value(A.h);
 }
@@ -216,9 +257,11 @@ void add_desc(std::vector<PixLeafDesc>& list, uint64_t& total, uint64_t xp, uint
   total += i_hi - i_lo + 1;
 }
 
-LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P)
+/// The descriptors of the sieving primes b = t, t + T, t + 2T, ... of each
+/// part of A and C (term order does not matter to the sums).
+void ac_lists_part(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P,
+                   uint64_t t, uint64_t T, LeafLists& L)
 {
-  LeafLists L;
   const u128 ux = (u128) x;
   const uint64_t sqrtx = isqrt(ux);
   const uint64_t x13 = iroot<3>(ux);
@@ -235,7 +278,8 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   };
 
   // A: pi[x_star] < b <= pi[x^(1/3)] (AC.cpp:57-113, 456-465)
-  for (uint64_t b = pi((uint64_t) x_star) + 1; b <= pi(x13); b++) {
+  const uint64_t a0 = pi((uint64_t) x_star) + 1;
+  for (uint64_t b = a0 + t; b <= pi(x13); b += T) {
     const uint64_t p = P.primes[b];
     const uint64_t xp = xp_of(p);
     const uint64_t sqrt_xp = isqrt(xp);
@@ -245,7 +289,8 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   // C1: max(k, pi[(x/z)^(1/3)]) < b <= pi[sqrt(z)] (AC.cpp:115-214, 402-418)
   const uint64_t sqrtz = isqrt((u128) uz);
   const uint64_t xz = (uint64_t) (ux / uz);
-  for (uint64_t b = umax((uint64_t) k, pi(iroot<3>((u128) xz))) + 1; b <= pi(sqrtz); b++) {
+  const uint64_t c0 = umax((uint64_t) k, pi(iroot<3>((u128) xz))) + 1;
+  for (uint64_t b = c0 + t; b <= pi(sqrtz); b += T) {
     const uint64_t p = P.primes[b];
     const uint64_t xp = xp_of(p);
     const uint64_t max_m = umin(xp, uz);
@@ -274,7 +319,7 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   // C2: max(k, pi[(x/y)^(1/3)], pi[sqrt(z)]) < b <= pi[x_star] (AC.cpp:216-346, 420-454)
   const uint64_t xy = (uint64_t) (ux / uy);
   const uint64_t min_c2 = umax(umax((uint64_t) k, pi(iroot<3>((u128) xy))), pi(sqrtz));
-  for (uint64_t b = min_c2 + 1; b <= pi((uint64_t) x_star); b++) {
+  for (uint64_t b = min_c2 + 1 + t; b <= pi((uint64_t) x_star); b += T) {
     const uint64_t p = P.primes[b];
     const uint64_t xp = xp_of(p);
     const uint64_t max_m = umin(xp / p, uy);
@@ -284,7 +329,70 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
     add_desc(L.plus, L.n_plus, xp, ~0ull, lo, hi);
     if (hi >= lo) L.correction += (i128) (hi - lo + 1) * (2 - (i128) b);
   }
+}
+
+/// Descriptor lists of A and C, built by up to 16 host threads for large x.
+LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P)
+{
+  const uint64_t nb = P.pi(iroot<3>((u128) x));
+  const uint64_t T = nb < 4096 ? 1 : (uint64_t) std::min(16, default_threads());
+  std::vector<LeafLists> parts(T);
+  if (T == 1) {
+    ac_lists_part(x, y, z, k, x_star, P, 0, 1, parts[0]);
+    return std::move(parts[0]);
+  }
+  std::vector<std::exception_ptr> err(T);
+  std::vector<std::thread> th;
+  for (uint64_t t = 0; t < T; t++)
+    th.emplace_back([&, t] {
+      try {
+        ac_lists_part(x, y, z, k, x_star, P, t, T, parts[t]);
+      } catch (...) {
+        err[t] = std::current_exception();
+      }
+    });
+  for (auto& h : th) h.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+  LeafLists L;
+  size_t np = 0, nm = 0;
+  for (const auto& q : parts) {
+    np += q.plus.size();
+    nm += q.minus.size();
+  }
+  L.plus.reserve(np);
+  L.minus.reserve(nm);
+  for (const auto& q : parts) {
+    for (PixLeafDesc d : q.plus) {
+      d.off += L.n_plus;
+      L.plus.push_back(d);
+    }
+    for (PixLeafDesc d : q.minus) {
+      d.off += L.n_minus;
+      L.minus.push_back(d);
+    }
+    L.n_plus += q.n_plus;
+    L.n_minus += q.n_minus;
+    L.correction += q.correction;
+  }
   return L;
+}
+
+/// Uploads the descriptor lists to dd (room for both) and launches the leaf
+/// sums: plus into acc[0..1], minus into acc[2..3].  The lists must stay
+/// alive until the stream has passed the copies.
+void launch_ac(const LeafLists& L, const EngineView& v, PixLeafDesc* dd, unsigned long long* acc)
+{
+  const size_t bp = L.plus.size() * sizeof(PixLeafDesc);
+  const size_t bm = L.minus.size() * sizeof(PixLeafDesc);
+  PixLeafDesc* dplus = dd;
+  PixLeafDesc* dminus = dd + L.plus.size();
+  if (bp) PIX_HIP(hipMemcpyAsync(dplus, L.plus.data(), bp, hipMemcpyHostToDevice, v.stream));
+  if (bm) PIX_HIP(hipMemcpyAsync(dminus, L.minus.data(), bm, hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_bits,
+                             v.pix_cnt, acc, (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_bits,
+                             v.pix_cnt, acc + 2, (uint32_t) v.cus, v.stream));
 }
 
 i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const EngineView& v,
@@ -295,17 +403,8 @@ i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const Engin
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
-  const size_t bp = L.plus.size() * sizeof(PixLeafDesc);
-  const size_t bm = L.minus.size() * sizeof(PixLeafDesc);
-  char* dd = (char*) engine_buffer(4, bp + bm + 64);
-  PixLeafDesc* dplus = (PixLeafDesc*) dd;
-  PixLeafDesc* dminus = (PixLeafDesc*) (dd + bp);
-  if (bp) PIX_HIP(hipMemcpyAsync(dplus, L.plus.data(), bp, hipMemcpyHostToDevice, v.stream));
-  if (bm) PIX_HIP(hipMemcpyAsync(dminus, L.minus.data(), bm, hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_bits,
-                             v.pix_cnt, A.d, (uint32_t) v.cus, v.stream));
-  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_bits,
-                             v.pix_cnt, A.d + 2, (uint32_t) v.cus, v.stream));
+  PixLeafDesc* dd = (PixLeafDesc*) engine_buffer(4, (L.plus.size() + L.minus.size() + 1) * sizeof(PixLeafDesc));
+  launch_ac(L, v, dd, A.d);
   acc_read(A, v.stream);
   return acc_value(A.h) - acc_value(A.h + 2) + L.correction;
 }
@@ -394,86 +493,174 @@ i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, Pix
   return r;
 }
 
+namespace {
+
+/// Device words per query in pi_group: Sigma4-6 [0, 6), Phi0 [6, 8),
+/// B [8, 10), A/C plus [10, 12), A/C minus [12, 14).
+constexpr size_t kTermWords = 16;
+
+/// Adds the elapsed time between two recorded events to *sec.
+void add_elapsed(hipEvent_t a, hipEvent_t b, double* sec)
+{
+  float ms = 0;
+  if (hipEventElapsedTime(&ms, a, b) == hipSuccess) *sec += ms * 1e-3;
+}
+
+/// pi(x) for the queries xs[idx[j]] > 10^8 whose parameters gps[idx[j]] one
+/// table set serves (view v).  Sigma, Phi0, B and the A/C leaf sums of every
+/// query are launched back to back on the engine stream (the host builds
+/// the A/C descriptor lists meanwhile), then the D leaves of all of them in
+/// one persistent launch; the term sums are read back once.  Caller holds the
+/// engine lock.
+void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
+              const std::vector<GourdonParams>& gps, const EngineView& v,
+              std::vector<i128>& result, PixStats& s)
+{
+  const size_t n = idx.size();
+  if (n == 0) return;
+  static hipEvent_t ev[6] = {};
+  if (!ev[0])
+    for (auto& e : ev) PIX_HIP(hipEventCreate(&e));
+  unsigned long long* acc = (unsigned long long*) engine_buffer(0, n * kTermWords * 8);
+  uint64_t* dn = (uint64_t*) engine_buffer(1, n * 8);
+  uint64_t* dpts = (uint64_t*) engine_buffer(2, n * 8);
+  PIX_HIP(hsl_launch_fill_zero(acc, n * kTermWords * 8, v.stream));
+  // pi(isqrt(x)) for Sigma0
+  std::vector<uint64_t> pts(n);
+  for (size_t j = 0; j < n; j++) pts[j] = isqrt((u128) xs[idx[j]]);
+  for (uint64_t p : pts)
+    if (p > v.pix_n) throw pcg_error("internal: prime-count table too small");
+  PIX_HIP(hipMemcpyAsync(dn, pts.data(), n * 8, hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_pi_points(dn, dpts, (uint32_t) n, v.pix_bits, v.pix_cnt, v.stream));
+
+  PIX_HIP(hipEventRecord(ev[0], v.stream));
+  for (size_t j = 0; j < n; j++) {
+    const GourdonParams& gp = gps[idx[j]];
+    launch_sigma(xs[idx[j]], gp.y, gp.x_star, v, acc + kTermWords * j);
+  }
+  PIX_HIP(hipEventRecord(ev[1], v.stream));
+  for (size_t j = 0; j < n; j++) {
+    const GourdonParams& gp = gps[idx[j]];
+    const PhiTiny& t = phi0_table(gp.k);
+    launch_phi0(xs[idx[j]], gp.z, gp.k, v, t, phi_tiny_device(t, v), acc + kTermWords * j + 6);
+  }
+  PIX_HIP(hipEventRecord(ev[2], v.stream));
+  for (size_t j = 0; j < n; j++)
+    launch_B(xs[idx[j]], gps[idx[j]].y, v, acc + kTermWords * j + 8);
+  PIX_HIP(hipEventRecord(ev[3], v.stream));
+
+  // A and C: descriptor lists on the host while the kernels above run
+  std::vector<LeafLists> L(n);
+  size_t ndesc = 0;
+  for (size_t j = 0; j < n; j++) {
+    const GourdonParams& gp = gps[idx[j]];
+    L[j] = ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host);
+    ndesc += L[j].plus.size() + L[j].minus.size();
+    s.ac_leaves += L[j].n_plus + L[j].n_minus;
+  }
+  PixLeafDesc* dd = (PixLeafDesc*) engine_buffer(4, (ndesc + 1) * sizeof(PixLeafDesc));
+  for (size_t j = 0; j < n; j++) {
+    launch_ac(L[j], v, dd, acc + kTermWords * j + 10);
+    dd += L[j].plus.size() + L[j].minus.size();
+  }
+  PIX_HIP(hipEventRecord(ev[4], v.stream));
+
+  // D: the hard special leaves of all queries in one persistent launch
+  std::vector<HslQuery> dq(n);
+  for (size_t j = 0; j < n; j++) {
+    const GourdonParams& gp = gps[idx[j]];
+    dq[j].kind = LeafSum::D;
+    dq[j].x = xs[idx[j]];
+    dq[j].y = gp.y;
+    dq[j].z = gp.z;
+    dq[j].k = gp.k;
+  }
+  const std::vector<i128> d = hsl_compute_batch_locked(dq, nullptr);
+  PIX_HIP(hipEventRecord(ev[5], v.stream));
+
+  std::vector<uint64_t> h(n * kTermWords), pis(n);
+  PIX_HIP(hipMemcpyAsync(h.data(), acc, n * kTermWords * 8, hipMemcpyDeviceToHost, v.stream));
+  PIX_HIP(hipMemcpyAsync(pis.data(), dpts, n * 8, hipMemcpyDeviceToHost, v.stream));
+  PIX_HIP(hipStreamSynchronize(v.stream));
+  add_elapsed(ev[0], ev[1], &s.seconds_sigma);
+  add_elapsed(ev[1], ev[2], &s.seconds_phi0);
+  add_elapsed(ev[2], ev[3], &s.seconds_b);
+  add_elapsed(ev[3], ev[4], &s.seconds_ac);
+  add_elapsed(ev[4], ev[5], &s.seconds_d);
+  const PrimeTables& P = *v.host;
+  for (size_t j = 0; j < n; j++) {
+    const i128 x = xs[idx[j]];
+    const GourdonParams& gp = gps[idx[j]];
+    const uint64_t* hj = h.data() + kTermWords * j;
+    const uint64_t x13 = iroot<3>((u128) x);
+    const uint64_t sqrt_xy = isqrt((u128) x / (u128) gp.y);
+    const uint64_t top = umax(umax((uint64_t) gp.y, x13), umax(sqrt_xy, (uint64_t) gp.x_star));
+    if (top > P.max) throw pcg_error("internal: prime table too small for Sigma");
+    const i128 sigma = sigma_value(P.pi((uint64_t) gp.y), P.pi(x13), P.pi(sqrt_xy),
+                                   P.pi((uint64_t) gp.x_star), (i128) pis[j], hj);
+    const i128 phi0 = phi_tiny_host(x, phi0_table(gp.k)) + acc_value(hj + 6);
+    const i128 ac = acc_value(hj + 10) - acc_value(hj + 12) + L[j].correction;
+    result[idx[j]] = sigma + phi0 + ac - acc_value(hj + 8) + d[j];
+  }
+}
+
+} // namespace
+
 std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st)
 {
   PixStats s;
   const double t0 = now_s();
   std::vector<i128> result(xs.size(), 0);
-  // the tables of the largest query serve all (prefix valid)
+  // Small x: the prime-count table answers directly (it is built for at
+  // least 2^20 numbers, with the primes up to the square root of that).
+  // Larger x: one table group for the queries with y >= z (prefix-valid
+  // tables of the largest), one group per query with y < z (its
+  // FactorTableD depends on y).
+  std::vector<GourdonParams> gps(xs.size());
+  std::vector<size_t> small, shared;
+  std::vector<std::vector<size_t>> alone;
   int64_t ymax = 0;
   uint64_t need = 0;
-  bool any = false;
-  std::vector<GourdonParams> gps(xs.size());
   for (size_t i = 0; i < xs.size(); i++) {
     const i128 x = xs[i];
     if (x < 2) continue;
-    any = true;
     if (x <= (i128) 100000000) {
-      // small x: the prime-count table answers directly (it is built for at
-      // least 2^20 numbers, with the primes up to the square root of that)
+      small.push_back(i);
       ymax = std::max<int64_t>(ymax, (int64_t) isqrt((u128) std::max<i128>(x, (i128) 1 << 20)) + 1);
       need = std::max<uint64_t>(need, (uint64_t) x);
-    } else {
-      gps[i] = gourdon_params(x);
-      check_params(x, gps[i].y, gps[i].z, gps[i].k);
+      continue;
+    }
+    gps[i] = gourdon_params(x);
+    check_params(x, gps[i].y, gps[i].z, gps[i].k);
+    if (gps[i].y >= gps[i].z) {
+      shared.push_back(i);
       ymax = std::max<int64_t>(ymax, gps[i].z);
       need = std::max<uint64_t>(need, table_need(x, gps[i].y, gps[i].x_star));
+    } else {
+      alone.push_back({i});
     }
   }
-  if (!any) {
-    if (st) *st = s;
-    return result;
-  }
-  std::vector<HslQuery> dq;
-  std::vector<size_t> di;
-  {
+  if (!small.empty() || !shared.empty()) {
     auto lock = engine_lock();
     const EngineView v = engine_view(ymax, ymax, need);
-    double t = now_s();
-    s.seconds_table = t - t0;
+    s.seconds_table += now_s() - t0;
     s.table_limit = v.pix_n;
-    std::vector<uint64_t> small;
-    std::vector<size_t> si;
-    for (size_t i = 0; i < xs.size(); i++)
-      if (xs[i] >= 2 && xs[i] <= (i128) 100000000) {
-        small.push_back((uint64_t) xs[i]);
-        si.push_back(i);
-      }
     if (!small.empty()) {
-      const std::vector<uint64_t> p = pi_points(v, small);
-      for (size_t j = 0; j < si.size(); j++) result[si[j]] = (i128) p[j];
+      std::vector<uint64_t> ns;
+      for (size_t i : small) ns.push_back((uint64_t) xs[i]);
+      const std::vector<uint64_t> p = pi_points(v, ns);
+      for (size_t j = 0; j < small.size(); j++) result[small[j]] = (i128) p[j];
     }
-    for (size_t i = 0; i < xs.size(); i++) {
-      const i128 x = xs[i];
-      if (x <= (i128) 100000000) continue;
-      const GourdonParams& gp = gps[i];
-      double t1 = now_s();
-      result[i] += run_sigma(x, gp.y, gp.x_star, v);
-      double t2 = now_s(); s.seconds_sigma += t2 - t1; t1 = t2;
-      result[i] += run_phi0(x, gp.y, gp.z, gp.k, v);
-      t2 = now_s(); s.seconds_phi0 += t2 - t1; t1 = t2;
-      uint64_t leaves = 0;
-      result[i] += run_ac(x, gp.y, gp.z, gp.k, gp.x_star, v, &leaves);
-      s.ac_leaves += leaves;
-      t2 = now_s(); s.seconds_ac += t2 - t1; t1 = t2;
-      result[i] -= run_B(x, gp.y, v);
-      s.seconds_b += now_s() - t1;
-      HslQuery q;
-      q.kind = LeafSum::D;
-      q.x = x;
-      q.y = gp.y;
-      q.z = gp.z;
-      q.k = gp.k;
-      dq.push_back(q);
-      di.push_back(i);
-    }
+    pi_group(xs, shared, gps, v, result, s);
   }
-  if (!dq.empty()) {
-    // the hard special leaves of all queries in one persistent launch
+  for (const auto& g : alone) {
+    const GourdonParams& gp = gps[g[0]];
+    auto lock = engine_lock();
     const double t = now_s();
-    const std::vector<i128> d = hsl_compute_batch(dq, nullptr);
-    for (size_t j = 0; j < di.size(); j++) result[di[j]] += d[j];
-    s.seconds_d = now_s() - t;
+    const EngineView v = engine_view(gp.y, gp.z, table_need(xs[g[0]], gp.y, gp.x_star));
+    s.seconds_table += now_s() - t;
+    s.table_limit = std::max<uint64_t>((uint64_t) s.table_limit, (uint64_t) v.pix_n);
+    pi_group(xs, g, gps, v, result, s);
   }
   s.seconds_total = now_s() - t0;
   if (st) *st = s;

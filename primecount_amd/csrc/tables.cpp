@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cstring>
 #include <atomic>
+#include <array>
 #include <thread>
 
 namespace pcg {
@@ -72,16 +73,30 @@ const std::vector<int16_t>& coprime2310_index()
   return t;
 }
 
+namespace {
+
+/// le_masks()[r]: the bits of a wheel-30 word for the offsets <= r.
+const std::array<uint64_t, 240>& le_masks()
+{
+  static const std::array<uint64_t, 240> m = [] {
+    std::array<uint64_t, 240> a{};
+    for (int r = 0; r < 240; r++)
+      for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++)
+          if (30 * j + kWheel[i] <= r) a[r] |= 1ull << (8 * j + i);
+    return a;
+  }();
+  return m;
+}
+
+} // namespace
+
 uint64_t PrimeTables::pi(uint64_t n) const
 {
   if (n < 2) return 0;
   if (n < 7) return n < 3 ? 1 : (n < 5 ? 2 : 3);
-  uint64_t blk = n / 240, r = n % 240;
-  uint64_t mask = 0;
-  for (int j = 0; j < 8; j++)
-    for (int i = 0; i < 8; i++)
-      if ((uint64_t) (30 * j + kWheel[i]) <= r) mask |= 1ull << (8 * j + i);
-  return 3 + pi_cnt[blk] + (uint64_t) __builtin_popcountll(pi_bits[blk] & mask);
+  const uint64_t blk = n / 240, r = n % 240;
+  return 3 + pi_cnt[blk] + (uint64_t) __builtin_popcountll(pi_bits[blk] & le_masks()[r]);
 }
 
 PrimeTables build_prime_tables(uint64_t max, int threads)

@@ -123,3 +123,20 @@ def test_nth_prime_matches_reference(pc, oracle):
         assert pc.pi(p) == n and pc.pi(p - 1) == n - 1, n
     with pytest.raises(pc.PrimecountError):
         pc.nth_prime(0)
+
+
+def test_pi_with_alpha_z(pc):
+    """pi(x) does not depend on the tuning factor alpha_z: with z = 2y (a
+    FactorTableD that depends on y, so the query gets a table set of its
+    own) pi(10^n) still equals the README values."""
+    known = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    try:
+        pc.set_alpha_z(2.0)
+        for x in (10 ** 10, 10 ** 13, 10 ** 15, 10 ** 17):
+            p = pc.gourdon_params(x)
+            log("pi(%d) with y=%d z=%d" % (x, p["y"], p["z"]))
+            assert p["z"] > p["y"]
+            assert pc.pi(x) == known[x], x
+    finally:
+        pc.set_alpha_z(0)
+    assert pc.pi(10 ** 13) == known[10 ** 13]
