@@ -28,11 +28,22 @@ def test_table_builders_match_definitions(tmp_path):
 
 
 def test_leaf_quotient_estimate_is_exact(tmp_path):
-    # the kernel's FP64-reciprocal quotient with one integer fix-up, under a
-    # reciprocal that is up to 4096 ulps off
+    # the sieve kernel's FP64-reciprocal quotient with one integer fix-up,
+    # under a reciprocal with a relative error up to 2^-22 (the hardware's is
+    # near 2^-24)
     exe = tmp_path / "leaf_stop_check"
     subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe),
                     os.path.join(REPO, "tests", "native", "leaf_stop_check.cpp")], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+
+
+def test_easy_leaf_quotient_is_exact(tmp_path):
+    # the A/C leaf-sum quotient (reciprocal + two Newton steps + one fix-up)
+    # and the table word index derived from it, for quotients below 2^49
+    exe = tmp_path / "div49_check"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", str(exe),
+                    os.path.join(REPO, "tests", "native", "div49_check.cpp")], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
 
