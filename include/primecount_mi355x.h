@@ -108,7 +108,7 @@ int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out);
 /* ---- engine configuration and statistics ----------------------------- */
 
 /* Segment size in 64-bit sieve words (240 numbers each); a multiple of 128
- * in [256, 4096].  0 selects the autotuned default.  Segment size never changes
+ * in [128, 4096].  0 selects the autotuned default.  Segment size never changes
  * results (replaces the segment sizing of src/LoadBalancerS2.cpp:65-110). */
 int pcg_set_segment_words(uint32_t words);
 

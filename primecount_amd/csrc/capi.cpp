@@ -361,8 +361,8 @@ int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out)
 int pcg_set_segment_words(uint32_t words)
 {
   return guarded([&] {
-    if (words != 0 && (words < 256 || words > 4096 || (words % 128) != 0))
-      throw pcg_error("segment words must be 0 or a multiple of 128 in [256, 4096]");
+    if (words != 0 && (words < 128 || words > 4096 || (words % 128) != 0))
+      throw pcg_error("segment words must be 0 or a multiple of 128 in [128, 4096]");
     engine_config().segment_words = words;
   });
 }

@@ -337,19 +337,33 @@ void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t
   qp.nseg = qp.reach[0];
 }
 
-/// Segment size.  One wave owns a segment, so LDS per wave (8 W bytes) sets
-/// how many waves a CU keeps resident; on gfx950 W = 512 (122,880 numbers)
-/// measured fastest for large sieve limits (at x = 1e19 the sieve kernel took
-/// 300 ms at W = 512, 373 ms at 1024 and 400 ms at 256).  Small limits use
-/// smaller segments so that every resident wave gets work.
-uint32_t auto_segment_words(uint64_t limit, uint64_t slots)
+/// Segment size.  One wave owns a segment; smaller segments mean more waves
+/// in flight and a finer-grained work queue, larger ones fewer (segment, b)
+/// slots per number.  Measured on the MI355X for D with the default
+/// parameters (kernel + combine, ms; the sieve limit is x / z):
+///
+///   x      limit    W=128   256    384    512
+///   1e15   5.8e8     4.2    6.7   10.0   13.0
+///   1e16   2.2e9     6.6   11.4   15.9   20.3
+///   1e17   8.4e9    16.5   18.9   26.1   34.9
+///   1e18   3.1e10   66.7   47.2   50.4   65.1
+///   2e18   4.6e10  107.9   75.1   66.5   80.2
+///   4e18   6.8e10  158.9  111.2   99.0  100.0
+///   1e19   1.1e11  285.8  196.9  175.2  171.1
+///
+/// Small segments win while a few heavy first segments would otherwise
+/// finish last; in a batch the heavy segments of many queries overlap and
+/// larger segments win, at the same total: batches of 16, 64 and 256 of the
+/// reference x values (1e12 .. 1e18) ran fastest at W = 384, 512, 512.  So
+/// the rule takes the sum of the sieve limits of the launch.
+uint32_t auto_segment_words(uint64_t limit)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
-  uint32_t W = 512;
-  while (W > 256 && (limit + 240ull * W - 1) / (240ull * W) < 2 * slots)
-    W >>= 1;
-  return W;
+  if (limit < 15000000000ull) return 128;
+  if (limit < 35000000000ull) return 256;
+  if (limit < 80000000000ull) return 384;
+  return 512;
 }
 
 /// Table requirements of a group of queries.
@@ -466,15 +480,15 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   double t_tab = now();
 
   std::vector<QueryPlan> plans(qs.size());
-  uint64_t max_limit = 1;
+  uint64_t sum_limit = 1;
   for (size_t i = 0; i < qs.size(); i++) {
     plans[i] = plan_query(qs[i], T.P);
-    if (!plans[i].empty) max_limit = std::max(max_limit, plans[i].qd.limit);
+    if (!plans[i].empty) sum_limit += plans[i].qd.limit;
   }
   int bpc = 0;
-  uint32_t W = auto_segment_words(max_limit, (uint64_t) D.cus * 16);
-  if (W < 256 || W > 4096 || (W % 128) != 0)
-    throw pcg_error("segment words must be a multiple of 128 in [256, 4096]");
+  uint32_t W = auto_segment_words(sum_limit);
+  if (W < 128 || W > 4096 || (W % 128) != 0)
+    throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
   bpc = std::max(bpc, 1);
   const uint64_t slots_wg = (uint64_t) D.cus * bpc;

@@ -67,14 +67,16 @@ def test_random_vs_oracle(pc, oracle):
         assert pc.S2_hard(x, yd, x // yd, 8) == oracle.S2_hard(x, yd, x // yd, 8), (x, yd)
 
 
-@pytest.mark.parametrize("words", [256, 384, 512, 640, 1024, 2048, 4096])
+@pytest.mark.parametrize("words", [128, 256, 384, 512, 640, 1024, 2048, 4096])
 def test_segment_size_invariance(pc, words):
     log("segment words %d" % words)
     r = {int(v["x"]): v for v in golden("configs.json")["D"]}[10 ** 15]
+    s2 = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}[10 ** 15]
     try:
         pc.set_segment_words(words)
         assert pc.D(10 ** 15, int(r["y"]), int(r["z"]), int(r["k"])) == int(r["value"])
         assert pc.D128(10 ** 13, 241615, 241615, 8) == 304739017794
+        assert pc.S2_hard_default(10 ** 15) == int(s2["value"])
     finally:
         pc.set_segment_words(0)
 
