@@ -110,7 +110,7 @@ struct Device {
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
   DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
   DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans;
-  DevBuf pix_bits, pix_cnt, pix_seg;  // prime-count table over [0, pix_n]
+  DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
   uint64_t pix_n = 0;
   DevBuf extra[8];                    // scratch of the other pi(x) terms
   HostBuf stage[2];                   // per pass: spans, columns
@@ -153,7 +153,7 @@ struct Device {
       b->release();
     for (HostBuf* b : {&stage[0], &stage[1]})
       b->release();
-    for (DevBuf* b : {&pix_bits, &pix_cnt, &pix_seg})
+    for (DevBuf* b : {&pix_tab, &pix_seg})
       b->release();
     for (DevBuf& b : extra)
       b.release();
@@ -848,18 +848,16 @@ static void ensure_pix_table(uint64_t n)
   const uint64_t nwords = n / 240 + 1;
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
   D.pix_n = 0;
-  D.pix_bits.ensure(nwords * 8);
-  D.pix_cnt.ensure(nwords * 8);
+  D.pix_tab.ensure(nwords * sizeof(PixWord));
   D.pix_seg.ensure(nseg * 8);
   const uint32_t j_end = (uint32_t) T.P.pi(r) + 1;               // primes[4 .. pi(r)]
   const uint32_t j_small = (uint32_t) std::min<uint64_t>(T.P.pi(std::min<uint64_t>(r, 240ull * kPixSegWords / 64)) + 1, j_end);
   hipStream_t s = D.stream;
-  PCG_HIP(pix_launch_sieve((uint64_t*) D.pix_bits.p, (uint64_t*) D.pix_seg.p,
+  PCG_HIP(pix_launch_sieve((PixWord*) D.pix_tab.p, (uint64_t*) D.pix_seg.p,
                            (const uint32_t*) D.primes.p, std::max<uint32_t>(j_small, 4),
                            std::max<uint32_t>(j_end, 4), nwords, n, s));
   PCG_HIP(pix_launch_scan((uint64_t*) D.pix_seg.p, (uint32_t) nseg, s));
-  PCG_HIP(pix_launch_counts((const uint64_t*) D.pix_bits.p, (const uint64_t*) D.pix_seg.p,
-                            (uint64_t*) D.pix_cnt.p, nwords, s));
+  PCG_HIP(pix_launch_counts((PixWord*) D.pix_tab.p, (const uint64_t*) D.pix_seg.p, nwords, s));
   PCG_HIP(hipStreamSynchronize(s));
   D.pix_n = nwords * 240 - 1 < n ? nwords * 240 - 1 : n;
 }
@@ -885,8 +883,7 @@ EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n)
   v.factor = D.factor.p;
   v.factor32 = T.factor32;
   v.factor_n = T.factor32 ? T.f32.size() : T.f16.size();
-  v.pix_bits = (const uint64_t*) D.pix_bits.p;
-  v.pix_cnt = (const uint64_t*) D.pix_cnt.p;
+  v.pix_tab = (const PixWord*) D.pix_tab.p;
   v.pix_n = D.pix_n;
   return v;
 }

@@ -58,6 +58,7 @@ void release_device();
 // ---- shared with the other pi(x) terms (pix.cpp) --------------------------
 
 struct PrimeTables;
+struct PixWord;
 
 /// Device-resident data of the engine.  Valid while the caller holds the
 /// lock returned by engine_lock().
@@ -70,8 +71,7 @@ struct EngineView {
   const void* factor = nullptr;         // FactorTableD entries (D tables)
   bool factor32 = false;
   uint64_t factor_n = 0;                // entries of factor[]
-  const uint64_t* pix_bits = nullptr;   // prime-count table (see pix_kernels.hpp)
-  const uint64_t* pix_cnt = nullptr;
+  const PixWord* pix_tab = nullptr;     // prime-count table (see pix_kernels.hpp)
   uint64_t pix_n = 0;                   // it covers [0, pix_n]
 };
 

@@ -87,7 +87,7 @@ std::vector<uint64_t> pi_points(const EngineView& v, const std::vector<uint64_t>
   uint64_t* dn = (uint64_t*) engine_buffer(1, ns.size() * 8);
   uint64_t* dout = (uint64_t*) engine_buffer(2, ns.size() * 8);
   PIX_HIP(hipMemcpyAsync(dn, ns.data(), ns.size() * 8, hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_pi_points(dn, dout, (uint32_t) ns.size(), v.pix_bits, v.pix_cnt, v.stream));
+  PIX_HIP(pix_launch_pi_points(dn, dout, (uint32_t) ns.size(), v.pix_tab, v.stream));
   std::vector<uint64_t> out(ns.size());
   PIX_HIP(hipMemcpyAsync(out.data(), dout, ns.size() * 8, hipMemcpyDeviceToHost, v.stream));
   PIX_HIP(hipStreamSynchronize(v.stream));
@@ -142,7 +142,7 @@ void launch_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v, unsign
   const uint32_t b_hi = (uint32_t) v.host->pi(x13) + 1;
   if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma");
   PIX_HIP(pix_launch_sigma((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, sqrt_xy, v.primes,
-                           b_lo, b_hi, v.pix_bits, v.pix_cnt, acc, v.stream));
+                           b_lo, b_hi, v.pix_tab, acc, v.stream));
 }
 
 /// Sigma from a = pi(y), b = pi(x^(1/3)), c = pi(sqrt(x / y)), d = pi(x_star),
@@ -223,7 +223,7 @@ i128 run_phi0(i128 x, int64_t y, int64_t z, int64_t k, const EngineView& v)
 void launch_B(i128 x, int64_t y, const EngineView& v, unsigned long long* acc)
 {
   PIX_HIP(pix_launch_B((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, isqrt((u128) x),
-                       v.pix_bits, v.pix_cnt, acc, v.stream));
+                       v.pix_tab, acc, v.stream));
 }
 
 i128 run_B(i128 x, int64_t y, const EngineView& v)
@@ -389,10 +389,10 @@ void launch_ac(const LeafLists& L, const EngineView& v, PixLeafDesc* dd, unsigne
   PixLeafDesc* dminus = dd + L.plus.size();
   if (bp) PIX_HIP(hipMemcpyAsync(dplus, L.plus.data(), bp, hipMemcpyHostToDevice, v.stream));
   if (bm) PIX_HIP(hipMemcpyAsync(dminus, L.minus.data(), bm, hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_bits,
-                             v.pix_cnt, acc, (uint32_t) v.cus, v.stream));
-  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_bits,
-                             v.pix_cnt, acc + 2, (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_tab, acc,
+                             (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_tab,
+                             acc + 2, (uint32_t) v.cus, v.stream));
 }
 
 i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const EngineView& v,
@@ -531,7 +531,7 @@ void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
   for (uint64_t p : pts)
     if (p > v.pix_n) throw pcg_error("internal: prime-count table too small");
   PIX_HIP(hipMemcpyAsync(dn, pts.data(), n * 8, hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_pi_points(dn, dpts, (uint32_t) n, v.pix_bits, v.pix_cnt, v.stream));
+  PIX_HIP(pix_launch_pi_points(dn, dpts, (uint32_t) n, v.pix_tab, v.stream));
 
   PIX_HIP(hipEventRecord(ev[0], v.stream));
   for (size_t j = 0; j < n; j++) {
@@ -695,7 +695,7 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
     const EngineView v = engine_view(10001, 10001, 100000000);
     uint64_t* dout = (uint64_t*) engine_buffer(2, 8);
     for (size_t i : table_i) {
-      PIX_HIP(pix_launch_select(v.pix_bits, v.pix_cnt, v.pix_n / 240 + 1, ns[i], dout, v.stream));
+      PIX_HIP(pix_launch_select(v.pix_tab, v.pix_n / 240 + 1, ns[i], dout, v.stream));
       PIX_HIP(hipMemcpyAsync(&result[i], dout, 8, hipMemcpyDeviceToHost, v.stream));
       PIX_HIP(hipStreamSynchronize(v.stream));
     }
@@ -740,7 +740,7 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
       const uint64_t n_all = pr[0] >= 3 ? pr[0] - 3 : 0;       // sieving primes >= 7
       const uint64_t n_small = std::min<uint64_t>(pr[1] - 3, n_all);
       uint32_t* sp = (uint32_t*) engine_buffer(5, (n_all + 1) * 4);
-      if (n_all) PIX_HIP(pix_launch_extract(v.pix_bits, v.pix_cnt, r, sp, v.stream));
+      if (n_all) PIX_HIP(pix_launch_extract(v.pix_tab, r, sp, v.stream));
       const uint64_t nwords = hi / 240 - lo / 240 + 1;
       const uint64_t nblk = (nwords + kPixSegWords - 1) / kPixSegWords;
       uint64_t* bits = (uint64_t*) engine_buffer(6, nwords * 8);

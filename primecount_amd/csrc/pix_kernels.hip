@@ -18,14 +18,14 @@ typedef unsigned __int128 u128;
 typedef __int128 i128;
 
 /// pi(n) from the resident table.
-__device__ __forceinline__ uint64_t pix_pi(const uint64_t* __restrict__ bits,
-                                           const uint64_t* __restrict__ cnt, uint64_t n)
+__device__ __forceinline__ uint64_t pix_pi(const PixWord* __restrict__ tab, uint64_t n)
 {
   if (n < 7)
     return n < 2 ? 0 : (n < 3 ? 1 : (n < 5 ? 2 : 3));
   const uint64_t w = n / 240;
   const uint32_t r = (uint32_t) (n - w * 240);
-  return 3 + cnt[w] + (uint64_t) __popcll(bits[w] & le_mask(r));
+  const PixWord t = tab[w];
+  return 3 + t.cnt + (uint64_t) __popcll(t.bits & le_mask(r));
 }
 
 /// Adds a signed 128-bit value to a (lo, hi) pair with two 64-bit atomics; the
@@ -104,7 +104,7 @@ __device__ __forceinline__ uint64_t isqrt64(uint64_t n)
 /// of q, 32 threads per class), the others by one thread each.  Writes the
 /// bits and the segment's prime count.
 __global__ __launch_bounds__(kThreads) void pix_sieve_kernel(
-    uint64_t* __restrict__ bits, uint64_t* __restrict__ seg_tot,
+    PixWord* __restrict__ tab, uint64_t* __restrict__ seg_tot,
     const uint32_t* __restrict__ primes, uint32_t j_small_end, uint32_t j_end,
     uint64_t nwords, uint64_t N)
 {
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(kThreads) void pix_sieve_kernel(
   unsigned long long tot = 0;
   for (uint32_t w = t; w < nw; w += kThreads) {
     const uint64_t v = ((uint64_t) bm[2 * w + 1] << 32) | bm[2 * w];
-    bits[w0 + w] = v;
+    tab[w0 + w].bits = v;
     tot += (unsigned long long) __popcll(v);
   }
   red[t] = tot;
@@ -210,11 +210,10 @@ __global__ __launch_bounds__(1024) void pix_scan_kernel(uint64_t* __restrict__ s
   }
 }
 
-/// cnt[w] = primes >= 7 below 240 w: segment base plus the prefix of the
+/// tab[w].cnt = primes >= 7 below 240 w: segment base plus the prefix of the
 /// word popcounts inside the segment (16 words per thread, one block scan).
-__global__ __launch_bounds__(kThreads) void pix_count_kernel(const uint64_t* __restrict__ bits,
+__global__ __launch_bounds__(kThreads) void pix_count_kernel(PixWord* __restrict__ tab,
                                                              const uint64_t* __restrict__ seg_base,
-                                                             uint64_t* __restrict__ cnt,
                                                              uint64_t nwords)
 {
   __shared__ unsigned long long part[kThreads];
@@ -226,7 +225,7 @@ __global__ __launch_bounds__(kThreads) void pix_count_kernel(const uint64_t* __r
   unsigned long long s = 0;
   for (uint32_t i = 0; i < per; i++)
     if (a + i < nw)
-      s += (unsigned long long) __popcll(bits[w0 + a + i]);
+      s += (unsigned long long) __popcll(tab[w0 + a + i].bits);
   part[t] = s;
   __syncthreads();
   for (uint32_t d = 1; d < kThreads; d <<= 1) {
@@ -238,8 +237,9 @@ __global__ __launch_bounds__(kThreads) void pix_count_kernel(const uint64_t* __r
   unsigned long long run = seg_base[blockIdx.x] + part[t] - s;
   for (uint32_t i = 0; i < per; i++) {
     if (a + i < nw) {
-      cnt[w0 + a + i] = run;
-      run += (unsigned long long) __popcll(bits[w0 + a + i]);
+      const uint64_t b = tab[w0 + a + i].bits;
+      tab[w0 + a + i].cnt = run;
+      run += (unsigned long long) __popcll(b);
     }
   }
 }
@@ -275,8 +275,8 @@ __device__ __forceinline__ uint64_t div49(uint64_t a, double ad, uint32_t d, dou
 /// the table words sit in LDS.
 __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
     const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total, uint64_t per_wave,
-    const uint32_t* __restrict__ primes, const uint64_t* __restrict__ bits,
-    const uint64_t* __restrict__ cnt, unsigned long long* __restrict__ acc)
+    const uint32_t* __restrict__ primes, const PixWord* __restrict__ tab,
+    unsigned long long* __restrict__ acc)
 {
   constexpr uint32_t kGroup = 8;
   __shared__ uint64_t masks[240];
@@ -326,7 +326,8 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
 #pragma unroll
       for (uint32_t s = 0; s < kGroup; s++)
         q[s] = primes[idx[s]];
-      uint64_t w[kGroup], bw[kGroup], cw[kGroup];
+      uint64_t w[kGroup];
+      PixWord tw[kGroup];
       uint32_t rr[kGroup];
 #pragma unroll
       for (uint32_t s = 0; s < kGroup; s++) {
@@ -342,8 +343,7 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
       }
 #pragma unroll
       for (uint32_t s = 0; s < kGroup; s++) {
-        bw[s] = bits[w[s]];
-        cw[s] = cnt[w[s]];
+        tw[s] = tab[w[s]];
       }
       uint64_t part = 0;
 #pragma unroll
@@ -351,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
         const uint64_t v = xp[s];
         // pi(v) for v < 7 from a nibble table: 0, 0, 1, 2, 2, 3, 3
         const uint64_t pv = v < 7 ? (uint64_t) ((0x3322100u >> (4 * (uint32_t) v)) & 15)
-                                  : 3 + cw[s] + (uint64_t) __popcll(bw[s] & masks[rr[s]]);
+                                  : 3 + tw[s].cnt + (uint64_t) __popcll(tw[s].bits & masks[rr[s]]);
         part += q[s] <= q1[s] ? pv : 2 * pv;
       }
       sum += part;
@@ -364,8 +364,7 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
 /// table's bits (one thread per word).
 __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t x_hi, uint64_t y,
                                                          uint64_t sqrtx,
-                                                         const uint64_t* __restrict__ bits,
-                                                         const uint64_t* __restrict__ cnt,
+                                                         const PixWord* __restrict__ tab,
                                                          unsigned long long* __restrict__ acc)
 {
   const u128 x = ((u128) x_hi << 64) | x_lo;
@@ -374,13 +373,13 @@ __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t
   u128 sum = 0;
   for (uint64_t w = w_lo + (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
        w += (uint64_t) gridDim.x * kThreads) {
-    uint64_t v = bits[w];
+    uint64_t v = tab[w].bits;
     while (v) {
       const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
       v &= v - 1;
       const uint64_t p = 240 * w + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
       if (p > y && p <= sqrtx)
-        sum += pix_pi(bits, cnt, div_exact(xd, x_lo, p));
+        sum += pix_pi(tab, div_exact(xd, x_lo, p));
     }
   }
   block_add_i128(acc, (i128) sum);
@@ -391,18 +390,18 @@ __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t
 /// (p > sqrt(x / y)), acc[4..5] += pi(isqrt(x / p))^2.
 __global__ __launch_bounds__(kThreads) void pix_sigma_kernel(
     uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy, const uint32_t* __restrict__ primes,
-    uint32_t b_lo, uint32_t b_hi, const uint64_t* __restrict__ bits,
-    const uint64_t* __restrict__ cnt, unsigned long long* __restrict__ acc)
+    uint32_t b_lo, uint32_t b_hi, const PixWord* __restrict__ tab,
+    unsigned long long* __restrict__ acc)
 {
   const u128 x = ((u128) x_hi << 64) | x_lo;
   i128 s4 = 0, s5 = 0, s6 = 0;
   for (uint32_t b = b_lo + blockIdx.x * kThreads + threadIdx.x; b < b_hi; b += gridDim.x * kThreads) {
     const uint64_t p = primes[b];
     if (p <= sqrt_xy)
-      s4 += pix_pi(bits, cnt, (uint64_t) (x / ((u128) p * y)));
+      s4 += pix_pi(tab, (uint64_t) (x / ((u128) p * y)));
     else
-      s5 += pix_pi(bits, cnt, (uint64_t) (x / ((u128) p * p)));
-    const uint64_t t = pix_pi(bits, cnt, isqrt64((uint64_t) (x / p)));
+      s5 += pix_pi(tab, (uint64_t) (x / ((u128) p * p)));
+    const uint64_t t = pix_pi(tab, isqrt64((uint64_t) (x / p)));
     s6 += (i128) ((u128) t * t);
   }
   block_add_i128(acc, s4);
@@ -459,28 +458,26 @@ __global__ __launch_bounds__(kThreads) void pix_phi0_kernel(
 }
 
 __global__ void pix_pi_points_kernel(const uint64_t* __restrict__ n, uint64_t* __restrict__ out,
-                                     uint32_t count, const uint64_t* __restrict__ bits,
-                                     const uint64_t* __restrict__ cnt)
+                                     uint32_t count, const PixWord* __restrict__ tab)
 {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count)
-    out[i] = pix_pi(bits, cnt, n[i]);
+    out[i] = pix_pi(tab, n[i]);
 }
 
 // ---------------------------------------------------------------------------
 // nth prime: sieving an interval and selecting a prime
 
 /// The primes 7 <= p <= p_max in order, read from the table's bits: the
-/// prime of bit j of word w goes to out[cnt[w] + rank of j in the word].
-__global__ __launch_bounds__(kThreads) void pix_extract_kernel(const uint64_t* __restrict__ bits,
-                                                               const uint64_t* __restrict__ cnt,
+/// prime of bit j of word w goes to out[tab[w].cnt + rank of j in the word].
+__global__ __launch_bounds__(kThreads) void pix_extract_kernel(const PixWord* __restrict__ tab,
                                                                uint64_t p_max, uint32_t* __restrict__ out)
 {
   const uint64_t w_hi = p_max / 240;
   for (uint64_t w = (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
        w += (uint64_t) gridDim.x * kThreads) {
-    uint64_t v = bits[w];
-    uint64_t pos = cnt[w];
+    uint64_t v = tab[w].bits;
+    uint64_t pos = tab[w].cnt;
     while (v) {
       const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
       v &= v - 1;
@@ -601,20 +598,19 @@ __global__ __launch_bounds__(kThreads) void pix_block_count_kernel(const uint64_
 /// The n-th prime (n >= 4) from the table: the last word w with
 /// 3 + cnt[w] < n holds it (binary search), then its bit of rank
 /// n - 3 - cnt[w].  One thread.
-__global__ void pix_select_kernel(const uint64_t* __restrict__ bits,
-                                  const uint64_t* __restrict__ cnt, uint64_t nwords, uint64_t n,
+__global__ void pix_select_kernel(const PixWord* __restrict__ tab, uint64_t nwords, uint64_t n,
                                   uint64_t* __restrict__ out)
 {
   uint64_t lo = 0, hi = nwords - 1;
   while (lo < hi) {
     const uint64_t mid = (lo + hi + 1) / 2;
-    if (3 + cnt[mid] < n)
+    if (3 + tab[mid].cnt < n)
       lo = mid;
     else
       hi = mid - 1;
   }
-  uint64_t v = bits[lo];
-  uint64_t k = n - 3 - cnt[lo];                 // 1-based rank inside the word
+  uint64_t v = tab[lo].bits;
+  uint64_t k = n - 3 - tab[lo].cnt;             // 1-based rank inside the word
   uint64_t res = 0;
   while (v) {
     const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
@@ -635,12 +631,12 @@ uint32_t grid_for(uint64_t work, uint32_t cap)
 
 } // namespace
 
-hipError_t pix_launch_sieve(uint64_t* bits, uint64_t* seg_tot, const uint32_t* primes,
+hipError_t pix_launch_sieve(PixWord* tab, uint64_t* seg_tot, const uint32_t* primes,
                             uint32_t j_small_end, uint32_t j_end, uint64_t nwords, uint64_t N,
                             hipStream_t st)
 {
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
-  hipLaunchKernelGGL(pix_sieve_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, bits, seg_tot,
+  hipLaunchKernelGGL(pix_sieve_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, tab, seg_tot,
                      primes, j_small_end, j_end, nwords, N);
   return hipGetLastError();
 }
@@ -651,18 +647,18 @@ hipError_t pix_launch_scan(uint64_t* seg_tot, uint32_t nseg, hipStream_t st)
   return hipGetLastError();
 }
 
-hipError_t pix_launch_counts(const uint64_t* bits, const uint64_t* seg_base, uint64_t* cnt,
-                             uint64_t nwords, hipStream_t st)
+hipError_t pix_launch_counts(PixWord* tab, const uint64_t* seg_base, uint64_t nwords,
+                             hipStream_t st)
 {
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
-  hipLaunchKernelGGL(pix_count_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, bits, seg_base,
-                     cnt, nwords);
+  hipLaunchKernelGGL(pix_count_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, tab, seg_base,
+                     nwords);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
-                              const uint32_t* primes, const uint64_t* bits, const uint64_t* cnt,
-                              unsigned long long* acc, uint32_t cus, hipStream_t st)
+                              const uint32_t* primes, const PixWord* tab, unsigned long long* acc,
+                              uint32_t cus, hipStream_t st)
 {
   if (total == 0 || ndesc == 0) return hipSuccess;
   // contiguous ranges of at least 4,096 terms, up to 64 waves per CU
@@ -671,29 +667,27 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
   const uint64_t per_wave = ((total + waves - 1) / waves + 63) / 64 * 64;
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
   hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
-                     per_wave, primes, bits, cnt, acc);
+                     per_wave, primes, tab, acc);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
-                        const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
-                        hipStream_t st)
+                        const PixWord* tab, unsigned long long* acc, hipStream_t st)
 {
   if (sqrtx <= y) return hipSuccess;
   const uint64_t words = sqrtx / 240 - (y + 1) / 240 + 1;
   hipLaunchKernelGGL(pix_B_kernel, dim3(grid_for(words, 65536)), dim3(kThreads), 0, st, x_lo, x_hi,
-                     y, sqrtx, bits, cnt, acc);
+                     y, sqrtx, tab, acc);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
                             const uint32_t* primes, uint32_t b_lo, uint32_t b_hi,
-                            const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
-                            hipStream_t st)
+                            const PixWord* tab, unsigned long long* acc, hipStream_t st)
 {
   if (b_hi <= b_lo) return hipSuccess;
   hipLaunchKernelGGL(pix_sigma_kernel, dim3(grid_for(b_hi - b_lo, 4096)), dim3(kThreads), 0, st,
-                     x_lo, x_hi, y, sqrt_xy, primes, b_lo, b_hi, bits, cnt, acc);
+                     x_lo, x_hi, y, sqrt_xy, primes, b_lo, b_hi, tab, acc);
   return hipGetLastError();
 }
 
@@ -712,11 +706,10 @@ hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, boo
   return hipGetLastError();
 }
 
-hipError_t pix_launch_extract(const uint64_t* bits, const uint64_t* cnt, uint64_t p_max,
-                              uint32_t* out, hipStream_t st)
+hipError_t pix_launch_extract(const PixWord* tab, uint64_t p_max, uint32_t* out, hipStream_t st)
 {
   hipLaunchKernelGGL(pix_extract_kernel, dim3(grid_for(p_max / 240 + 1, 65536)), dim3(kThreads), 0,
-                     st, bits, cnt, p_max, out);
+                     st, tab, p_max, out);
   return hipGetLastError();
 }
 
@@ -743,19 +736,19 @@ hipError_t pix_launch_block_counts(const uint64_t* bits, uint64_t nwords, uint64
   return hipGetLastError();
 }
 
-hipError_t pix_launch_select(const uint64_t* bits, const uint64_t* cnt, uint64_t nwords, uint64_t n,
-                             uint64_t* out, hipStream_t st)
+hipError_t pix_launch_select(const PixWord* tab, uint64_t nwords, uint64_t n, uint64_t* out,
+                             hipStream_t st)
 {
-  hipLaunchKernelGGL(pix_select_kernel, dim3(1), dim3(1), 0, st, bits, cnt, nwords, n, out);
+  hipLaunchKernelGGL(pix_select_kernel, dim3(1), dim3(1), 0, st, tab, nwords, n, out);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_pi_points(const uint64_t* n, uint64_t* out, uint32_t count,
-                                const uint64_t* bits, const uint64_t* cnt, hipStream_t st)
+                                const PixWord* tab, hipStream_t st)
 {
   if (count == 0) return hipSuccess;
   hipLaunchKernelGGL(pix_pi_points_kernel, dim3((count + 255) / 256), dim3(256), 0, st, n, out,
-                     count, bits, cnt);
+                     count, tab);
   return hipGetLastError();
 }
 

@@ -3,16 +3,22 @@
 // (src/gourdon/pi_gourdon.cpp:82-87); D is the hard-special-leaf engine.
 //
 // All of them read pi(n) from a prime-count table resident in HBM: for every
-// 64-bit word w (numbers [240 w, 240 w + 240)) the wheel-30 bits of the
-// primes >= 7 and the count of primes >= 7 below 240 w (u64 each), so that
-// pi(n) = 3 + cnt[n / 240] + popcount(bits[n / 240] & mask(n % 240)), the
-// lookup of src/PiTable.hpp:46-57 with 64-bit counts.
+// 64-bit word w (numbers [240 w, 240 w + 240)) one 16-byte PixWord with the
+// wheel-30 bits of the primes >= 7 and the count of primes >= 7 below 240 w,
+// so that pi(n) = 3 + t[n / 240].cnt + popcount(t[n / 240].bits & mask(n % 240)),
+// the lookup of src/PiTable.hpp:46-57 with 64-bit counts; bits and count sit
+// side by side so that a lookup reads one 16-byte word.
 #pragma once
 
 #include <stdint.h>
 #include <hip/hip_runtime_api.h>
 
 namespace pcg {
+
+struct alignas(16) PixWord {
+  uint64_t bits;     // primes >= 7 in [240 w, 240 w + 240), wheel-30 bits
+  uint64_t cnt;      // primes >= 7 below 240 w
+};
 
 /// Leaves of one sieving prime for the flat leaf-sum kernel: terms
 /// w * pi(xp / primes[i]) for i = i_lo .. i_lo + (next off - off) - 1, with
@@ -28,34 +34,31 @@ struct alignas(16) PixLeafDesc {
 
 constexpr uint32_t kPixSegWords = 4096;       // words per sieve segment (983,040 numbers)
 
-hipError_t pix_launch_sieve(uint64_t* bits, uint64_t* seg_tot, const uint32_t* primes,
+hipError_t pix_launch_sieve(PixWord* tab, uint64_t* seg_tot, const uint32_t* primes,
                             uint32_t j_small_end, uint32_t j_end, uint64_t nwords, uint64_t N,
                             hipStream_t st);
 hipError_t pix_launch_scan(uint64_t* seg_tot, uint32_t nseg, hipStream_t st);
-hipError_t pix_launch_counts(const uint64_t* bits, const uint64_t* seg_base, uint64_t* cnt,
-                             uint64_t nwords, hipStream_t st);
+hipError_t pix_launch_counts(PixWord* tab, const uint64_t* seg_base, uint64_t nwords,
+                             hipStream_t st);
 hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
-                              const uint32_t* primes, const uint64_t* bits, const uint64_t* cnt,
-                              unsigned long long* acc, uint32_t cus, hipStream_t st);
+                              const uint32_t* primes, const PixWord* tab, unsigned long long* acc,
+                              uint32_t cus, hipStream_t st);
 hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
-                        const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
-                        hipStream_t st);
+                        const PixWord* tab, unsigned long long* acc, hipStream_t st);
 hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
                             const uint32_t* primes, uint32_t b_lo, uint32_t b_hi,
-                            const uint64_t* bits, const uint64_t* cnt, unsigned long long* acc,
-                            hipStream_t st);
+                            const PixWord* tab, unsigned long long* acc, hipStream_t st);
 hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, bool factor32,
                            uint64_t nidx, uint32_t k, uint32_t pk, const uint32_t* phi_tbl,
                            uint32_t pp, uint64_t phi_pp, unsigned long long* acc, hipStream_t st);
-hipError_t pix_launch_extract(const uint64_t* bits, const uint64_t* cnt, uint64_t p_max,
-                              uint32_t* out, hipStream_t st);
+hipError_t pix_launch_extract(const PixWord* tab, uint64_t p_max, uint32_t* out, hipStream_t st);
 hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const uint32_t* sp,
                                uint32_t n_small, uint64_t n_all, hipStream_t st);
 hipError_t pix_launch_block_counts(const uint64_t* bits, uint64_t nwords, uint64_t* counts,
                                    hipStream_t st);
-hipError_t pix_launch_select(const uint64_t* bits, const uint64_t* cnt, uint64_t nwords, uint64_t n,
-                             uint64_t* out, hipStream_t st);
+hipError_t pix_launch_select(const PixWord* tab, uint64_t nwords, uint64_t n, uint64_t* out,
+                             hipStream_t st);
 hipError_t pix_launch_pi_points(const uint64_t* n, uint64_t* out, uint32_t count,
-                                const uint64_t* bits, const uint64_t* cnt, hipStream_t st);
+                                const PixWord* tab, hipStream_t st);
 
 } // namespace pcg
