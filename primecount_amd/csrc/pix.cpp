@@ -378,35 +378,111 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   return L;
 }
 
-/// Uploads the descriptor lists to dd (room for both) and launches the leaf
-/// sums: plus into acc[0..1], minus into acc[2..3].  The lists must stay
-/// alive until the stream has passed the copies.
-void launch_ac(const LeafLists& L, const EngineView& v, PixLeafDesc* dd, unsigned long long* acc)
+/// The A/C leaves as the device takes them: the plus list as tile
+/// descriptors in groups of kPixTileGroup (tiled kernel), the minus list (the
+/// short q·r runs of C1) as flat descriptors.
+struct AcWork {
+  std::vector<PixTileDesc> tiles;
+  std::vector<PixTileGroup> groups;
+  uint64_t nitems = 0;
+  std::vector<PixLeafDesc> minus;
+  uint64_t n_minus = 0;
+  i128 offset = 0;          // 3 * (sum of the plus weights) + the (2 - b) parts of C
+  uint64_t leaves = 0;
+
+  size_t bytes() const
+  {
+    return (tiles.size() + groups.size() + minus.size() + 3) * 32;
+  }
+};
+
+AcWork ac_work(LeafLists&& L, const PrimeTables& P)
 {
-  const size_t bp = L.plus.size() * sizeof(PixLeafDesc);
-  const size_t bm = L.minus.size() * sizeof(PixLeafDesc);
-  PixLeafDesc* dplus = dd;
-  PixLeafDesc* dminus = dd + L.plus.size();
-  if (bp) PIX_HIP(hipMemcpyAsync(dplus, L.plus.data(), bp, hipMemcpyHostToDevice, v.stream));
-  if (bm) PIX_HIP(hipMemcpyAsync(dminus, L.minus.data(), bm, hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_leafsum(dplus, (uint32_t) L.plus.size(), L.n_plus, v.primes, v.pix_tab, acc,
-                             (uint32_t) v.cus, v.stream));
-  PIX_HIP(pix_launch_leafsum(dminus, (uint32_t) L.minus.size(), L.n_minus, v.primes, v.pix_tab,
+  AcWork W;
+  const size_t n = L.plus.size();
+  W.tiles.resize(n);
+  W.offset = L.correction;
+  for (size_t i = 0; i < n; i++) {
+    const PixLeafDesc& d = L.plus[i];
+    const uint64_t cnt = (i + 1 < n ? L.plus[i + 1].off : L.n_plus) - d.off;
+    PixTileDesc& t = W.tiles[i];
+    t.xp = d.xp;
+    t.xd = d.xd;
+    t.i_lo = d.i_lo;
+    t.i_hi = (uint32_t) (d.i_lo + cnt - 1);
+    t.q1 = d.q1;
+    t.pad = 0;
+    // the tiled kernel leaves out the pi(v) < 3 corrections: every quotient
+    // of the plus list is at least p >= 7
+    if (d.xp / P.primes[t.i_hi] < 7) throw pcg_error("internal: A/C quotient below 7");
+    // weight-2 leaves: primes[i] > q1
+    uint64_t two = 0;
+    if (d.q1 < P.max) {
+      const uint64_t pq = P.pi(d.q1);
+      if (pq < t.i_hi) two = t.i_hi - std::max<uint64_t>((uint64_t) t.i_lo - 1, pq);
+    }
+    W.offset += 3 * (i128) (cnt + two);
+  }
+  for (size_t g0 = 0; g0 < n; g0 += kPixTileGroup) {
+    PixTileGroup g;
+    g.item0 = W.nitems;
+    g.d0 = (uint32_t) g0;
+    g.nd = (uint32_t) std::min<size_t>(kPixTileGroup, n - g0);
+    g.t_lo = ~0u;
+    g.t_hi = 0;
+    for (size_t i = g0; i < g0 + g.nd; i++) {
+      g.t_lo = std::min(g.t_lo, W.tiles[i].i_lo / 64);
+      g.t_hi = std::max(g.t_hi, W.tiles[i].i_hi / 64);
+    }
+    W.nitems += (g.t_hi - g.t_lo + kPixTileChunk) / kPixTileChunk;
+    W.groups.push_back(g);
+  }
+  W.leaves = L.n_plus + L.n_minus;
+  W.minus = std::move(L.minus);
+  W.n_minus = L.n_minus;
+  return W;
+}
+
+/// Uploads the A/C work to dd (W.bytes() bytes) and launches the leaf sums:
+/// plus into acc[0..1], minus into acc[2..3].  W must stay alive until the
+/// stream has passed the copies.
+void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long long* acc)
+{
+  PixTileDesc* dt = (PixTileDesc*) dd;
+  PixTileGroup* dg = (PixTileGroup*) (dt + W.tiles.size() + 1);
+  PixLeafDesc* dm = (PixLeafDesc*) (dg + W.groups.size() + 1);
+  if (!W.tiles.empty())
+    PIX_HIP(hipMemcpyAsync(dt, W.tiles.data(), W.tiles.size() * sizeof(PixTileDesc),
+                           hipMemcpyHostToDevice, v.stream));
+  if (!W.groups.empty())
+    PIX_HIP(hipMemcpyAsync(dg, W.groups.data(), W.groups.size() * sizeof(PixTileGroup),
+                           hipMemcpyHostToDevice, v.stream));
+  if (!W.minus.empty())
+    PIX_HIP(hipMemcpyAsync(dm, W.minus.data(), W.minus.size() * sizeof(PixLeafDesc),
+                           hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_leaftiles(dt, dg, (uint32_t) W.groups.size(), W.nitems, v.primes,
+                               (uint32_t) v.nprimes, v.pix_tab, acc, (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dm, (uint32_t) W.minus.size(), W.n_minus, v.primes, v.pix_tab,
                              acc + 2, (uint32_t) v.cus, v.stream));
+}
+
+i128 ac_value(const AcWork& W, const uint64_t* h)
+{
+  return acc_value(h) - acc_value(h + 2) + W.offset;
 }
 
 i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const EngineView& v,
             uint64_t* leaves)
 {
-  LeafLists L = ac_lists(x, y, z, k, x_star, *v.host);
-  *leaves = L.n_plus + L.n_minus;
+  const AcWork W = ac_work(ac_lists(x, y, z, k, x_star, *v.host), *v.host);
+  *leaves = W.leaves;
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
-  PixLeafDesc* dd = (PixLeafDesc*) engine_buffer(4, (L.plus.size() + L.minus.size() + 1) * sizeof(PixLeafDesc));
-  launch_ac(L, v, dd, A.d);
+  char* dd = (char*) engine_buffer(4, W.bytes());
+  launch_ac(W, v, dd, A.d);
   acc_read(A, v.stream);
-  return acc_value(A.h) - acc_value(A.h + 2) + L.correction;
+  return ac_value(W, A.h);
 }
 
 void check_params(i128 x, int64_t y, int64_t z, int64_t k)
@@ -550,18 +626,18 @@ void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
   PIX_HIP(hipEventRecord(ev[3], v.stream));
 
   // A and C: descriptor lists on the host while the kernels above run
-  std::vector<LeafLists> L(n);
-  size_t ndesc = 0;
+  std::vector<AcWork> L(n);
+  size_t bytes = 0;
   for (size_t j = 0; j < n; j++) {
     const GourdonParams& gp = gps[idx[j]];
-    L[j] = ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host);
-    ndesc += L[j].plus.size() + L[j].minus.size();
-    s.ac_leaves += L[j].n_plus + L[j].n_minus;
+    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host);
+    bytes += L[j].bytes();
+    s.ac_leaves += L[j].leaves;
   }
-  PixLeafDesc* dd = (PixLeafDesc*) engine_buffer(4, (ndesc + 1) * sizeof(PixLeafDesc));
+  char* dd = (char*) engine_buffer(4, bytes + 64);
   for (size_t j = 0; j < n; j++) {
     launch_ac(L[j], v, dd, acc + kTermWords * j + 10);
-    dd += L[j].plus.size() + L[j].minus.size();
+    dd += L[j].bytes();
   }
   PIX_HIP(hipEventRecord(ev[4], v.stream));
 
@@ -599,7 +675,7 @@ void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
     const i128 sigma = sigma_value(P.pi((uint64_t) gp.y), P.pi(x13), P.pi(sqrt_xy),
                                    P.pi((uint64_t) gp.x_star), (i128) pis[j], hj);
     const i128 phi0 = phi_tiny_host(x, phi0_table(gp.k)) + acc_value(hj + 6);
-    const i128 ac = acc_value(hj + 10) - acc_value(hj + 12) + L[j].correction;
+    const i128 ac = ac_value(L[j], hj + 10);
     result[idx[j]] = sigma + phi0 + ac - acc_value(hj + 8) + d[j];
   }
 }

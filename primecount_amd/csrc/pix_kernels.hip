@@ -360,6 +360,96 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
   block_add_i128(acc, (i128) sum);
 }
 
+/// Sum of w * (pi(xp / q) - 3) over tiles x descriptors (the host adds the
+/// 3 w).  Lane l of a wave holds the prime q = primes[64 t + l] of tile t and
+/// its reciprocal (two Newton steps), and runs through the descriptors of
+/// the tile's group, which are wave-uniform (scalar loads): the eight
+/// quotients, then the eight table loads, then the sums.  Each wave takes
+/// one contiguous range of items (group, run of kPixTileChunk tiles),
+/// finding its first group by binary search.  Lanes outside a descriptor's
+/// range read word 0 with the empty mask, which adds nothing.
+__global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
+    const PixTileDesc* __restrict__ desc, const PixTileGroup* __restrict__ grp, uint32_t ngrp,
+    uint64_t nitems, uint64_t per_wave, const uint32_t* __restrict__ primes, uint32_t nprimes,
+    const PixWord* __restrict__ tab, unsigned long long* __restrict__ acc)
+{
+  constexpr uint32_t G = kPixTileGroup;
+  __shared__ uint64_t masks[240];
+  for (uint32_t r = threadIdx.x; r < 240; r += kThreads)
+    masks[r] = le_mask(r);
+  __syncthreads();
+
+  const uint32_t lane = threadIdx.x & 63;
+  // wave-uniform (scalar) from here on: group, item and descriptor data
+  const uint64_t wave = (uint64_t) blockIdx.x * (kThreads / 64) +
+                        (uint32_t) __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint64_t j0 = wave * per_wave;
+  const uint64_t j1 = j0 + per_wave < nitems ? j0 + per_wave : nitems;
+  u128 sum = 0;
+  if (j0 < nitems) {
+    uint32_t lo = 0, hi = ngrp - 1;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi + 1) >> 1;
+      if (grp[mid].item0 <= j0)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    uint32_t g = lo;
+    for (uint64_t j = j0; j < j1; j++) {
+      while (g + 1 < ngrp && grp[g + 1].item0 <= j)
+        g++;
+      const PixTileGroup gr = grp[g];
+      const uint32_t ta = gr.t_lo + (uint32_t) (j - gr.item0) * kPixTileChunk;
+      const uint32_t tb = gr.t_hi + 1 < ta + kPixTileChunk ? gr.t_hi + 1 : ta + kPixTileChunk;
+      for (uint32_t t = ta; t < tb; t++) {
+        const uint32_t idx = 64 * t + lane;
+        const uint32_t q0 = primes[idx < nprimes ? idx : nprimes - 1];
+        const uint32_t q = q0 > 1 ? q0 : 1u;
+        const double qd = (double) q;
+        double r = __builtin_amdgcn_rcp(qd);
+        r = __builtin_fma(r, __builtin_fma(-qd, r, 1.0), r);
+        r = __builtin_fma(r, __builtin_fma(-qd, r, 1.0), r);
+        uint64_t w[G];
+        uint32_t rr[G], sh[G];
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++) {
+          const PixTileDesc D = desc[gr.d0 + (k < gr.nd ? k : gr.nd - 1)];
+          const bool ok = k < gr.nd && idx >= D.i_lo && idx <= D.i_hi;
+          sh[k] = q > D.q1 ? 1u : 0u;
+          // quotient xp / q: the integer in the low 52 bits of qf + 2^52,
+          // fixed by the residual (exact modulo 2^64, |xp - s q| < 2 q, for
+          // the lanes in range)
+          const double qf = __builtin_trunc(D.xd * r);
+          const uint64_t s =
+              __builtin_bit_cast(uint64_t, qf + 4503599627370496.0) & ((1ull << 52) - 1);
+          const int64_t rem = (int64_t) (D.xp - s * q);
+          const int32_t fix = rem < 0 ? -1 : ((uint64_t) rem >= q ? 1 : 0);
+          const double vd = qf + (double) fix;
+          const double wf = __builtin_trunc(__builtin_fma(vd, 1.0 / 240.0, 0.5 / 240.0));
+          const uint64_t wv =
+              __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
+          const uint32_t v_lo = (uint32_t) s + (uint32_t) fix;
+          w[k] = ok ? wv : 0;
+          rr[k] = ok ? v_lo - 240u * (uint32_t) wv : 0u;
+        }
+        PixWord tw[G];
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++)
+          tw[k] = tab[w[k]];
+        uint64_t part = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++) {
+          const uint64_t pv = tw[k].cnt + (uint64_t) __popcll(tw[k].bits & masks[rr[k]]);
+          part += pv << sh[k];
+        }
+        sum += part;
+      }
+    }
+  }
+  block_add_i128(acc, (i128) sum);
+}
+
 /// B: sum of pi(x / p) over the primes y < p <= isqrt(x), read from the
 /// table's bits (one thread per word).
 __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t x_hi, uint64_t y,
@@ -668,6 +758,22 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
   hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
                      per_wave, primes, tab, acc);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp, uint32_t ngrp,
+                                uint64_t nitems, const uint32_t* primes, uint32_t nprimes,
+                                const PixWord* tab, unsigned long long* acc, uint32_t cus,
+                                hipStream_t st)
+{
+  if (nitems == 0 || ngrp == 0) return hipSuccess;
+  // contiguous item ranges, up to 64 waves per CU
+  const uint64_t max_waves = (uint64_t) cus * 64;
+  const uint64_t per_wave = (nitems + max_waves - 1) / max_waves;
+  const uint64_t waves = (nitems + per_wave - 1) / per_wave;
+  const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
+  hipLaunchKernelGGL(pix_leaftile_kernel, dim3(grid), dim3(kThreads), 0, st, desc, grp, ngrp,
+                     nitems, per_wave, primes, nprimes, tab, acc);
   return hipGetLastError();
 }
 
