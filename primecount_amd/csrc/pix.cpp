@@ -382,6 +382,8 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
 /// descriptors in groups of kPixTileGroup (tiled kernel), the minus list (the
 /// short q·r runs of C1) as flat descriptors.
 struct AcWork {
+  std::vector<PixLeafDesc> plus;      // flat plus list when a prime is >= 2^30
+  uint64_t n_plus = 0;
   std::vector<PixTileDesc> tiles;
   std::vector<PixTileGroup> groups;
   uint64_t nitems = 0;
@@ -392,7 +394,7 @@ struct AcWork {
 
   size_t bytes() const
   {
-    return (tiles.size() + groups.size() + minus.size() + 3) * 32;
+    return (plus.size() + tiles.size() + groups.size() + minus.size() + 4) * 32;
   }
 };
 
@@ -400,8 +402,22 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P)
 {
   AcWork W;
   const size_t n = L.plus.size();
-  W.tiles.resize(n);
   W.offset = L.correction;
+  W.leaves = L.n_plus + L.n_minus;
+  W.minus = std::move(L.minus);
+  W.n_minus = L.n_minus;
+  uint64_t top = 0;
+  for (size_t i = 0; i < n; i++) {
+    const uint64_t cnt = (i + 1 < n ? L.plus[i + 1].off : L.n_plus) - L.plus[i].off;
+    top = std::max<uint64_t>(top, P.primes[L.plus[i].i_lo + cnt - 1]);
+  }
+  if (top >= kPixTileMaxPrime) {
+    // the tiled kernel's 32-bit residual needs primes below 2^30
+    W.plus = std::move(L.plus);
+    W.n_plus = L.n_plus;
+    return W;
+  }
+  W.tiles.resize(n);
   for (size_t i = 0; i < n; i++) {
     const PixLeafDesc& d = L.plus[i];
     const uint64_t cnt = (i + 1 < n ? L.plus[i + 1].off : L.n_plus) - d.off;
@@ -437,9 +453,14 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P)
     W.nitems += (g.t_hi - g.t_lo + kPixTileChunk) / kPixTileChunk;
     W.groups.push_back(g);
   }
-  W.leaves = L.n_plus + L.n_minus;
-  W.minus = std::move(L.minus);
-  W.n_minus = L.n_minus;
+  // the last group padded with empty descriptors
+  while (W.tiles.size() % kPixTileGroup) {
+    PixTileDesc e{};
+    e.i_lo = 1;
+    e.i_hi = 0;
+    e.q1 = ~0u;
+    W.tiles.push_back(e);
+  }
   return W;
 }
 
@@ -448,9 +469,13 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P)
 /// stream has passed the copies.
 void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long long* acc)
 {
-  PixTileDesc* dt = (PixTileDesc*) dd;
+  PixLeafDesc* dp = (PixLeafDesc*) dd;
+  PixTileDesc* dt = (PixTileDesc*) (dp + W.plus.size() + 1);
   PixTileGroup* dg = (PixTileGroup*) (dt + W.tiles.size() + 1);
   PixLeafDesc* dm = (PixLeafDesc*) (dg + W.groups.size() + 1);
+  if (!W.plus.empty())
+    PIX_HIP(hipMemcpyAsync(dp, W.plus.data(), W.plus.size() * sizeof(PixLeafDesc),
+                           hipMemcpyHostToDevice, v.stream));
   if (!W.tiles.empty())
     PIX_HIP(hipMemcpyAsync(dt, W.tiles.data(), W.tiles.size() * sizeof(PixTileDesc),
                            hipMemcpyHostToDevice, v.stream));
@@ -460,8 +485,11 @@ void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long lon
   if (!W.minus.empty())
     PIX_HIP(hipMemcpyAsync(dm, W.minus.data(), W.minus.size() * sizeof(PixLeafDesc),
                            hipMemcpyHostToDevice, v.stream));
+  PIX_HIP(pix_launch_leafsum(dp, (uint32_t) W.plus.size(), W.n_plus, v.primes, v.pix_tab, acc,
+                             (uint32_t) v.cus, v.stream));
   PIX_HIP(pix_launch_leaftiles(dt, dg, (uint32_t) W.groups.size(), W.nitems, v.primes,
-                               (uint32_t) v.nprimes, v.pix_tab, acc, (uint32_t) v.cus, v.stream));
+                               (uint32_t) v.nprimes, v.pix_tab, v.pix_n, acc, (uint32_t) v.cus,
+                               v.stream));
   PIX_HIP(pix_launch_leafsum(dm, (uint32_t) W.minus.size(), W.n_minus, v.primes, v.pix_tab,
                              acc + 2, (uint32_t) v.cus, v.stream));
 }

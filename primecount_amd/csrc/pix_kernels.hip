@@ -410,28 +410,50 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
         double r = __builtin_amdgcn_rcp(qd);
         r = __builtin_fma(r, __builtin_fma(-qd, r, 1.0), r);
         r = __builtin_fma(r, __builtin_fma(-qd, r, 1.0), r);
-        uint64_t w[G];
-        uint32_t rr[G], sh[G];
+        double vd[G];
+        uint32_t v_lo[G], sh[G];
+        bool ok[G];
+        uint32_t bad = 0;
 #pragma unroll
         for (uint32_t k = 0; k < G; k++) {
-          const PixTileDesc D = desc[gr.d0 + (k < gr.nd ? k : gr.nd - 1)];
-          const bool ok = k < gr.nd && idx >= D.i_lo && idx <= D.i_hi;
+          // groups are padded to G descriptors with empty ranges
+          const PixTileDesc D = desc[gr.d0 + k];
+          ok[k] = (idx >= D.i_lo) & (idx <= D.i_hi);
           sh[k] = q > D.q1 ? 1u : 0u;
-          // quotient xp / q: the integer in the low 52 bits of qf + 2^52,
-          // fixed by the residual (exact modulo 2^64, |xp - s q| < 2 q, for
-          // the lanes in range)
-          const double qf = __builtin_trunc(D.xd * r);
-          const uint64_t s =
-              __builtin_bit_cast(uint64_t, qf + 4503599627370496.0) & ((1ull << 52) - 1);
-          const int64_t rem = (int64_t) (D.xp - s * q);
-          const int32_t fix = rem < 0 ? -1 : ((uint64_t) rem >= q ? 1 : 0);
-          const double vd = qf + (double) fix;
-          const double wf = __builtin_trunc(__builtin_fma(vd, 1.0 / 240.0, 0.5 / 240.0));
+          // quotient xp / q.  xd * r is within v 2^-50 of xp / q, so when
+          // its fraction lies in [v 2^-46, 1 - v 2^-46] its integer part is
+          // the quotient; otherwise (rare) the residual fixes it below.  The
+          // low word of vd + 2^52 is the low word of the integer vd.
+          const double pf = D.xd * r;
+          vd[k] = __builtin_trunc(pf);
+          const double h = (pf - vd[k]) - 0.5;
+          const double tol = 0.5 - vd[k] * 0x1p-46;
+          v_lo[k] = (uint32_t) __builtin_bit_cast(uint64_t, vd[k] + 4503599627370496.0);
+          bad |= (uint32_t) (ok[k] & (__builtin_fabs(h) > tol)) << k;
+        }
+        if (__builtin_amdgcn_ballot_w64(bad != 0) != 0) {
+          // the residual xp - s q lies in (-q, 2 q) for the lanes in range;
+          // its low 32 bits give it exactly for q < 2^30
+#pragma unroll
+          for (uint32_t k = 0; k < G; k++) {
+            if (bad & (1u << k)) {
+              const PixTileDesc D = desc[gr.d0 + k];
+              const int32_t rem = (int32_t) ((uint32_t) D.xp - v_lo[k] * q);
+              const int32_t fix = rem < 0 ? -1 : ((uint32_t) rem >= q ? 1 : 0);
+              vd[k] += (double) fix;
+              v_lo[k] += (uint32_t) fix;
+            }
+          }
+        }
+        uint64_t w[G];
+        uint32_t rr[G];
+#pragma unroll
+        for (uint32_t k = 0; k < G; k++) {
+          const double wf = __builtin_trunc(__builtin_fma(vd[k], 1.0 / 240.0, 0.5 / 240.0));
           const uint64_t wv =
               __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
-          const uint32_t v_lo = (uint32_t) s + (uint32_t) fix;
-          w[k] = ok ? wv : 0;
-          rr[k] = ok ? v_lo - 240u * (uint32_t) wv : 0u;
+          w[k] = ok[k] ? wv : 0;
+          rr[k] = ok[k] ? v_lo[k] - 240u * (uint32_t) wv : 0u;
         }
         PixWord tw[G];
 #pragma unroll
@@ -763,10 +785,12 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
 
 hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp, uint32_t ngrp,
                                 uint64_t nitems, const uint32_t* primes, uint32_t nprimes,
-                                const PixWord* tab, unsigned long long* acc, uint32_t cus,
-                                hipStream_t st)
+                                const PixWord* tab, uint64_t max_quotient,
+                                unsigned long long* acc, uint32_t cus, hipStream_t st)
 {
   if (nitems == 0 || ngrp == 0) return hipSuccess;
+  // the quotient guard v 2^-46 stays far below 1/2 for quotients below 2^41
+  if (max_quotient >= (1ull << 41)) return hipErrorInvalidValue;
   // contiguous item ranges, up to 64 waves per CU
   const uint64_t max_waves = (uint64_t) cus * 64;
   const uint64_t per_wave = (nitems + max_waves - 1) / max_waves;

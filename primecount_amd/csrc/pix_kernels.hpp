@@ -34,7 +34,8 @@ struct alignas(16) PixLeafDesc {
 
 /// Leaves of one sieving prime for the tiled leaf-sum kernel: terms
 /// w * pi(xp / primes[i]) for i_lo <= i <= i_hi, w = 1 if primes[i] <= q1 and
-/// 2 otherwise; every quotient is at least 7 and below 2^49.
+/// 2 otherwise; every quotient is at least 7 and below 2^49, and every
+/// prime below 2^30 (kPixTileMaxPrime).
 struct alignas(16) PixTileDesc {
   uint64_t xp;
   double xd;         // xp rounded to double
@@ -43,7 +44,8 @@ struct alignas(16) PixTileDesc {
   uint32_t pad;
 };
 
-/// Up to kPixTileGroup consecutive descriptors d0 .. d0 + nd - 1 and the
+/// kPixTileGroup consecutive descriptors d0 .. d0 + kPixTileGroup - 1 (nd of
+/// them real, the rest empty: i_lo = 1, i_hi = 0) and the
 /// tiles (64 consecutive prime indices: tile t holds i = 64 t .. 64 t + 63)
 /// their ranges cover, t_lo .. t_hi, cut into items of kPixTileChunk tiles;
 /// the group's items are item0, item0 + 1, ...
@@ -56,6 +58,7 @@ struct alignas(16) PixTileGroup {
 static_assert(sizeof(PixTileDesc) == 32 && sizeof(PixTileGroup) == 32, "32-byte records");
 
 constexpr uint32_t kPixTileGroup = 8;
+constexpr uint32_t kPixTileMaxPrime = 1u << 30;
 constexpr uint32_t kPixTileChunk = 16;
 
 constexpr uint32_t kPixSegWords = 4096;       // words per sieve segment (983,040 numbers)
@@ -69,10 +72,11 @@ hipError_t pix_launch_counts(PixWord* tab, const uint64_t* seg_base, uint64_t nw
 hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
                               const uint32_t* primes, const PixWord* tab, unsigned long long* acc,
                               uint32_t cus, hipStream_t st);
+/// max_quotient bounds xp / primes[i] over all terms (the table limit).
 hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp, uint32_t ngrp,
                                 uint64_t nitems, const uint32_t* primes, uint32_t nprimes,
-                                const PixWord* tab, unsigned long long* acc, uint32_t cus,
-                                hipStream_t st);
+                                const PixWord* tab, uint64_t max_quotient,
+                                unsigned long long* acc, uint32_t cus, hipStream_t st);
 hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
                         const PixWord* tab, unsigned long long* acc, hipStream_t st);
 hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
