@@ -70,19 +70,6 @@ __device__ __forceinline__ u128 divmod128_32(u128 v, uint32_t d, uint32_t* rem)
   return q;
 }
 
-/// floor(a / d) for a < 2^64, d < 2^32, from a double estimate and an exact
-/// integer correction.
-__device__ __forceinline__ uint64_t div64_32(uint64_t a, uint32_t d)
-{
-  uint64_t q = (uint64_t) ((double) a / (double) d);
-  // the estimate is within a few units; step until a - q d is in [0, d)
-  while ((u128) q * d > a)
-    q--;
-  while ((u128) (q + 1) * d <= a)
-    q++;
-  return q;
-}
-
 /// floor(sqrt(n)) for n < 2^64.
 __device__ __forceinline__ uint64_t isqrt64(uint64_t n)
 {
