@@ -143,6 +143,8 @@ typedef struct pcg_stats {
   uint64_t dense_primes;     /* primes crossed off one at a time              */
   uint64_t dense_steps;      /* their cross-off steps                         */
   uint64_t group_steps;      /* grouped cross-off steps                       */
+  uint64_t prefix_builds;    /* word-pair prefix rebuilds (a prime with leaves
+                                after first-time removals)                    */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

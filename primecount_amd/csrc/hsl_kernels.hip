@@ -318,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   // wave-level step counts: windows, primes with leaves, leaf-loop steps,
   // one-at-a-time cross-off primes and steps, grouped cross-off steps
   unsigned long long u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0,
-                     u_gsteps = 0;
+                     u_gsteps = 0, u_pbuilds = 0;
 
   for (;;) {
     uint32_t qi = 0;
@@ -364,6 +364,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
 
     uint32_t b = qd.b_first;
     bool done = false;
+    // the word-pair prefixes hold the current sieve state unless a
+    // first-time removal happened since they were built (wave-uniform)
+    bool dirty = true;
     while (!done) {
       // Window of up to 64 primes b .. b + 63, lane l <-> b + l, kept in
       // registers; an epoch reads its prime's values with v_readlane.  The
@@ -493,6 +496,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               removed += (uint32_t) __popcll(__ballot(first));
             }
           }
+          dirty |= removed != 0;
           if (lane == 0)
             w_rem[j] = removed;
         }
@@ -523,7 +527,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               wave_lds_sync();
             }
           }
-          if (has && (old & mask)) {
+          const bool fresh = has && (old & mask);
+          dirty |= __ballot(fresh) != 0;
+          if (fresh) {
             uint32_t attr = jo;
             if (!ordered && delta < pp.z) {             // q = q_lo + delta < p
               const uint32_t q = pp.x - pp.z + delta;
@@ -551,8 +557,12 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         const uint32_t T = Tt & 0x7fffffffu;
         const uint64_t xlo = readlane_u64(r_xlo, j);
         // word-pair prefixes of the current sieve state for the count queries
-        build_pair_prefix(bm, pre2, W, lane);
-        wave_lds_sync();
+        if (dirty) {
+          build_pair_prefix(bm, pre2, W, lane);
+          wave_lds_sync();
+          dirty = false;
+          u_pbuilds++;
+        }
         int M = 0;                                      // wave-uniform
         if (Tt >> 31) {
           // factor-table entries: leaves -mu(m) phi(xp / m, b - 1) for the m
@@ -638,6 +648,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     atomicAdd(&counters[7], u_dprimes);
     atomicAdd(&counters[8], u_dsteps);
     atomicAdd(&counters[9], u_gsteps);
+    atomicAdd(&counters[10], u_pbuilds);
   }
 }
 
