@@ -15,8 +15,11 @@ Workloads:
 Inter-arrival times are exponential with mean 1/rate, from SplitMix64 with the
 given seed.  --rate 0 submits every query at once (closed loop), which
 measures the capacity.  A client thread submits each query at its arrival
-time; the main thread polls.  Prints one JSON line: throughput, latency
-percentiles (submit to result), batching, and whether every answer matched.
+time; the main thread polls.  --plain instead answers the queries one call
+at a time (pi(), nth_prime(), D_default(): one query per launch), the
+baseline the serving context is compared with.  Prints one JSON line:
+throughput, latency percentiles (submit to result), batching, and whether
+every answer matched.
 """
 import argparse
 import json
@@ -118,6 +121,23 @@ def run(sv, qs, times):
     return res, elapsed, bad, s0, s1
 
 
+def run_plain(qs):
+    """One call per query, back to back: (latencies, elapsed, mismatches)."""
+    call = {pc.KIND_PI: pc.pi, pc.KIND_NTH_PRIME: pc.nth_prime, pc.KIND_D: pc.D_default}
+    lat, bad = [], 0
+    t_start = time.perf_counter()
+    last = t_start
+    for i, (k, a, v) in enumerate(qs):
+        t = time.perf_counter()
+        got = call[k](a)
+        lat.append(time.perf_counter() - t)
+        bad += got != v
+        if time.perf_counter() - last > 30:
+            print("plain: %d of %d" % (i + 1, len(qs)), file=sys.stderr, flush=True)
+            last = time.perf_counter()
+    return lat, time.perf_counter() - t_start, bad
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", choices=("mix", "d"), default="mix")
@@ -128,6 +148,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0xC5C5)
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--fixture", default=None, help="serving-mix answers (JSON or JSON lines)")
+    ap.add_argument("--plain", action="store_true", help="one call per query, no serving context")
     a = ap.parse_args()
     rate = a.rate if a.rate is not None else (20.0 if a.workload == "mix" else 100.0)
 
@@ -141,6 +162,34 @@ def main():
         z_max = 0
         warm = [max((q for q in qs if q[0] == pc.KIND_PI), key=lambda q: q[1]),
                 max((q for q in qs if q[0] == pc.KIND_NTH_PRIME), key=lambda q: q[1])]
+    if a.workload == "mix":
+        npi = sum(1 for q in qs if q[0] == pc.KIND_PI)
+        desc = ("serving mix: %d pi(x), x = 10^(9+10u); %d nth_prime(n), n = 10^(8+9u)"
+                % (npi, len(qs) - npi))
+    else:
+        desc = "D(x), x from the 256 reference x values (1e12..1e18)"
+
+    if a.plain:
+        call = {pc.KIND_PI: pc.pi, pc.KIND_NTH_PRIME: pc.nth_prime, pc.KIND_D: pc.D_default}
+        t0 = time.perf_counter()
+        for k, x, _ in warm:
+            call[k](x)
+        warm_s = time.perf_counter() - t0
+        lat, elapsed, bad = run_plain(qs)
+        lat.sort()
+
+        def pctp(p):
+            return round(lat[min(len(lat) - 1, int(p * len(lat)))] * 1000, 3)
+
+        print(json.dumps({
+            "workload": desc, "arrivals": "one call per query, back to back (plain path)",
+            "queries": len(qs), "bit_exact": bad == 0, "mismatches": bad,
+            "throughput_qps": round(len(qs) / elapsed, 2), "elapsed_s": round(elapsed, 3),
+            "latency_ms": {"p50": pctp(0.50), "p90": pctp(0.90), "p99": pctp(0.99),
+                           "max": round(lat[-1] * 1000, 3)},
+            "warmup_s": round(warm_s, 3),
+        }), flush=True)
+        return 0 if bad == 0 else 1
     sv = pc.Server(z_max=z_max, max_batch=a.max_batch)
     # warm up: the largest queries build the resident tables
     t0 = time.perf_counter()
@@ -156,12 +205,6 @@ def main():
         return round(lat[min(len(lat) - 1, int(p * len(lat)))] * 1000, 3)
 
     batches = s1["batches"] - s0["batches"]
-    if a.workload == "mix":
-        npi = sum(1 for q in qs if q[0] == pc.KIND_PI)
-        desc = ("serving mix: %d pi(x), x = 10^(9+10u); %d nth_prime(n), n = 10^(8+9u)"
-                % (npi, len(qs) - npi))
-    else:
-        desc = "D(x), x from the 256 reference x values (1e12..1e18)"
     print(json.dumps({
         "workload": desc,
         "arrivals": "all at once" if rate <= 0 else "Poisson, %.3g/s" % rate,
