@@ -112,6 +112,12 @@ int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out);
  * results (replaces the segment sizing of src/LoadBalancerS2.cpp:65-110). */
 int pcg_set_segment_words(uint32_t words);
 
+/* Whether the queries of one launch (pcg_D_batch, serving, pi(x) batches)
+ * share the sieve of the segments they have in common: each segment is
+ * sieved once for up to 4 queries that cover it in full, and each answers its
+ * own leaves.  1 (default) on, 0 off.  Never changes results. */
+int pcg_set_shared_segments(int on);
+
 /* (segment, prime) slots per kernel pass (0 = default, 2^31).  C and M take
  * 8 bytes of HBM per slot; larger queries run in several passes whose carry is
  * kept on the device.  The pass size never changes results.  No counterpart in

@@ -133,6 +133,7 @@ def load():
         "pcg_set_segment_words": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
+        "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
         "pcg_pi": ([_I128, PI128], ctypes.c_int),
@@ -307,6 +308,10 @@ def set_alpha(a):
 
 def set_segment_words(w):
     _check(load().pcg_set_segment_words(int(w)))
+
+
+def set_shared_segments(on):
+    _check(load().pcg_set_shared_segments(1 if on else 0))
 
 
 def set_pass_slots(n):

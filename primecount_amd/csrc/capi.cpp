@@ -367,6 +367,11 @@ int pcg_set_segment_words(uint32_t words)
   });
 }
 
+int pcg_set_shared_segments(int on)
+{
+  return guarded([&] { engine_config().share_segments = on != 0; });
+}
+
 int pcg_set_pass_slots(uint64_t slots)
 {
   return guarded([&] {

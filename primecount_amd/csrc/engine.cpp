@@ -109,7 +109,7 @@ struct Device {
   std::vector<hipEvent_t> ev;         // timing events, 3 per pass
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
   DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
-  DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans;
+  DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans, groups;
   DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
   uint64_t pix_n = 0;
   DevBuf extra[8];                    // scratch of the other pi(x) terms
@@ -149,7 +149,7 @@ struct Device {
   {
     for (DevBuf* b : {&queries, &items, &binfo, &primes, &factor, &pi_bits, &pi_cnt, &tc, &tm,
                       &sacc, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry, &reach,
-                      &sfx, &spans})
+                      &sfx, &spans, &groups})
       b->release();
     for (HostBuf* b : {&stage[0], &stage[1]})
       b->release();
@@ -553,41 +553,86 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   // Passes: runs of (query, segment) items with at most max_slots slots (a
   // single segment may exceed it).  A pass is a list of spans, one per query
   // it touches; boundaries are found by binary search on the slot count F.
+  // With shared segments a pass is a segment range [S0, S1) of every query,
+  // so that the queries of a segment meet in one pass.
   struct Pass { std::vector<PassSpan> spans; uint32_t nitems = 0; uint64_t slots = 0; };
   std::vector<Pass> passes(1);
-  for (uint32_t qsl = 0; qsl < nq; qsl++) {
-    const QueryPlan& qp = plans[slot_query[qsl]];
-    uint64_t s0 = 0;
-    while (s0 < qp.nseg) {
-      Pass& ps = passes.back();
-      const uint64_t f0 = qp.F(s0);
-      const uint64_t cap = max_slots > ps.slots ? max_slots - ps.slots : 0;
-      // largest s1 in (s0, nseg] with F(s1) - F(s0) <= cap
-      uint64_t lo = s0, hi = qp.nseg;
+  const bool share = nq > 1 && engine_config().share_segments;
+  if (share) {
+    uint64_t smax = 0;
+    for (uint32_t qsl = 0; qsl < nq; qsl++) smax = std::max<uint64_t>(smax, plans[slot_query[qsl]].nseg);
+    auto slots_upto = [&](uint64_t S) {          // slots of segments [0, S) of all queries
+      uint64_t t = 0;
+      for (uint32_t qsl = 0; qsl < nq; qsl++) {
+        const QueryPlan& qp = plans[slot_query[qsl]];
+        t += qp.F(std::min<uint64_t>(S, qp.nseg));
+      }
+      return t;
+    };
+    uint64_t S0 = 0;
+    passes.clear();
+    while (S0 < smax) {
+      const uint64_t f0 = slots_upto(S0);
+      uint64_t lo = S0, hi = smax;               // largest S1 with slots(S1) - slots(S0) <= cap
       while (lo < hi) {
         const uint64_t mid = (lo + hi + 1) / 2;
-        if (qp.F(mid) - f0 <= cap) lo = mid;
+        if (slots_upto(mid) - f0 <= max_slots) lo = mid;
         else hi = mid - 1;
       }
-      uint64_t s1 = lo;
-      if (s1 == s0) {
-        if (ps.nitems > 0) {           // full: start a new pass
-          passes.emplace_back();
-          continue;
-        }
-        s1 = s0 + 1;                   // one segment larger than a pass
+      const uint64_t S1 = std::max(lo, S0 + 1);
+      Pass ps;
+      for (uint32_t qsl = 0; qsl < nq; qsl++) {
+        const QueryPlan& qp = plans[slot_query[qsl]];
+        const uint64_t s0 = std::min<uint64_t>(S0, qp.nseg), s1 = std::min<uint64_t>(S1, qp.nseg);
+        if (s1 <= s0) continue;
+        PassSpan sp{};
+        sp.query = qsl;
+        sp.s0 = (uint32_t) s0;
+        sp.s1 = (uint32_t) s1;
+        sp.item_base = ps.nitems;
+        sp.t_base = ps.slots - qp.F(s0);       // modulo 2^64
+        ps.spans.push_back(sp);
+        ps.nitems += (uint32_t) (s1 - s0);
+        ps.slots += qp.F(s1) - qp.F(s0);
       }
-      PassSpan sp{};
-      sp.query = qsl;
-      sp.s0 = (uint32_t) s0;
-      sp.s1 = (uint32_t) s1;
-      sp.item_base = ps.nitems;
-      sp.t_base = ps.slots - f0;       // modulo 2^64
-      ps.spans.push_back(sp);
-      ps.nitems += (uint32_t) (s1 - s0);
-      ps.slots += qp.F(s1) - f0;
-      s0 = s1;
-      if (s0 < qp.nseg) passes.emplace_back();
+      passes.push_back(ps);
+      S0 = S1;
+    }
+  } else {
+    for (uint32_t qsl = 0; qsl < nq; qsl++) {
+      const QueryPlan& qp = plans[slot_query[qsl]];
+      uint64_t s0 = 0;
+      while (s0 < qp.nseg) {
+        Pass& ps = passes.back();
+        const uint64_t f0 = qp.F(s0);
+        const uint64_t cap = max_slots > ps.slots ? max_slots - ps.slots : 0;
+        // largest s1 in (s0, nseg] with F(s1) - F(s0) <= cap
+        uint64_t lo = s0, hi = qp.nseg;
+        while (lo < hi) {
+          const uint64_t mid = (lo + hi + 1) / 2;
+          if (qp.F(mid) - f0 <= cap) lo = mid;
+          else hi = mid - 1;
+        }
+        uint64_t s1 = lo;
+        if (s1 == s0) {
+          if (ps.nitems > 0) {           // full: start a new pass
+            passes.emplace_back();
+            continue;
+          }
+          s1 = s0 + 1;                   // one segment larger than a pass
+        }
+        PassSpan sp{};
+        sp.query = qsl;
+        sp.s0 = (uint32_t) s0;
+        sp.s1 = (uint32_t) s1;
+        sp.item_base = ps.nitems;
+        sp.t_base = ps.slots - f0;       // modulo 2^64
+        ps.spans.push_back(sp);
+        ps.nitems += (uint32_t) (s1 - s0);
+        ps.slots += qp.F(s1) - f0;
+        s0 = s1;
+        if (s0 < qp.nseg) passes.emplace_back();
+      }
     }
   }
   if (passes.back().nitems == 0) passes.pop_back();
@@ -609,6 +654,56 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   D.queue.ensure(npass * 4);
   PCG_HIP(hsl_launch_fill_zero(D.queue.p, npass * 4, s));
   for (auto& hb : D.stage) hb.ensure(spans_bytes + cols_bytes);
+
+  // Shared segments: segment s of every query that covers it in full is
+  // sieved once for groups of up to kGroupMax queries with the same first b;
+  // a query's last (possibly partial) segment goes alone.  Groups are
+  // ordered by segment, the largest queries first within one.
+  std::vector<std::vector<SegGroup>> pass_groups(npass);
+  size_t max_groups = 0;
+  if (share) {
+    std::vector<uint32_t> order(nq);
+    for (uint32_t i = 0; i < nq; i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+      return plans[slot_query[a]].nseg > plans[slot_query[b]].nseg;
+    });
+    std::vector<uint32_t> item_base(nq);
+    for (size_t pi = 0; pi < npass; pi++) {
+      std::vector<SegGroup>& groups = pass_groups[pi];
+      uint32_t S0 = UINT32_MAX, S1 = 0;
+      for (const PassSpan& sp : passes[pi].spans) {
+        item_base[sp.query] = sp.item_base - sp.s0;      // item of segment sg: item_base + sg
+        S0 = std::min(S0, sp.s0);
+        S1 = std::max(S1, sp.s1);
+      }
+      size_t active = nq;                        // order[0 .. active) have nseg > sg
+      for (uint64_t sg = S0; sg < S1; sg++) {
+        while (active > 0 && plans[slot_query[order[active - 1]]].nseg <= sg) active--;
+        SegGroup g{};
+        uint32_t g_first = 0;
+        for (size_t r = 0; r < active; r++) {
+          const uint32_t q = order[r];
+          const uint32_t idx = item_base[q] + (uint32_t) sg;
+          if (sg + 1 == plans[slot_query[q]].nseg) {   // last segment: alone
+            SegGroup one{};
+            one.n = 1;
+            one.item[0] = idx;
+            groups.push_back(one);
+            continue;
+          }
+          if (g.n == (uint32_t) kGroupMax || (g.n > 0 && qdesc[q].b_first != g_first)) {
+            groups.push_back(g);
+            g = SegGroup{};
+          }
+          if (g.n == 0) g_first = qdesc[q].b_first;
+          g.item[g.n++] = idx;
+        }
+        if (g.n > 0) groups.push_back(g);
+      }
+      max_groups = std::max(max_groups, groups.size());
+    }
+    D.groups.ensure(max_groups * sizeof(SegGroup));
+  }
   double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
 
   for (size_t pi = 0; pi < npass; pi++) {
@@ -661,8 +756,18 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     a.counters = (unsigned long long*) D.counters.p;
     a.queue = (unsigned int*) D.queue.p + pi;
     a.nitems = ps.nitems;
+    a.groups = nullptr;
+    if (share) {
+      // the previous pass's kernel may still read the groups buffer
+      const std::vector<SegGroup>& groups = pass_groups[pi];
+      if (pi > 0) PCG_HIP(hipStreamSynchronize(s));
+      PCG_HIP(hipMemcpyAsync(D.groups.p, groups.data(), groups.size() * sizeof(SegGroup),
+                             hipMemcpyHostToDevice, s));
+      a.groups = (const SegGroup*) D.groups.p;
+      a.nitems = (uint32_t) groups.size();
+    }
     a.W = W;
-    uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, ps.nitems);
+    uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);
 
     PCG_HIP(hipEventRecord(D.event(3 * pi), s));
     PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));

@@ -26,6 +26,7 @@ struct EngineConfig {
   uint32_t segment_words = 0;   // 0 = automatic
   int host_threads = 0;         // 0 = all hardware threads
   uint64_t max_slots_per_pass = 0;   // 0 = default (2^31 slots, 16 GiB)
+  bool share_segments = true;   // queries of a launch share the sieve of a segment
 };
 
 EngineConfig& engine_config();

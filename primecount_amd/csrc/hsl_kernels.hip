@@ -284,13 +284,21 @@ __device__ __forceinline__ uint32_t wave_total_u32(uint32_t v)
 /// barriers (a wave's LDS operations execute in order), and the per-prime
 /// bookkeeping runs once per segment and prime.
 ///
+/// With NQ > 1 a work item is a group of up to NQ queries that share the
+/// segment (the same interval [low, low + 240 W), the same first b and
+/// pre-sieve): the sieve -- bitmap, cross-offs, first-time removals, pair
+/// prefixes, the counts C(s, b) -- depends only on the segment and the
+/// primes, so it runs once and every member answers its own leaves on it and
+/// writes its own slots.  Each member stops at its own first empty leaf
+/// range (its "goto next_segment"); the group stops when all have.
+///
 /// Read-only tables and written arrays come in as separate __restrict__
 /// pointers so the compiler knows the tables are never written by the kernel
 /// and can read uniform entries (BInfo, work items) through the scalar cache.
-template <typename FT>
+template <typename FT, int NQ>
 __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
-    const BInfo* __restrict__ binfo_all,
+    const SegGroup* __restrict__ groups, const BInfo* __restrict__ binfo_all,
     const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
     const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,
     uint32_t* __restrict__ T_C, int32_t* __restrict__ T_M, unsigned long long* __restrict__ S_acc,
@@ -303,7 +311,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   uint16_t* cop = reinterpret_cast<uint16_t*>(pre2 + W / 2);  // 480 u16
   uint8_t* lut = reinterpret_cast<uint8_t*>(cop + 480);       // 240 u8
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
-  __shared__ int w_m[64];                                     // per window prime: leaf signs
+  __shared__ int w_m[NQ][64];                                 // per member and prime: leaf signs
   __shared__ uint4 wp[64];                                    // per window prime: cross-off data
 
   const uint32_t lane = threadIdx.x;
@@ -316,7 +324,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   unsigned long long n_slots = 0, u_hits = 0;
   unsigned long long u_leaves = 0, u_entries = 0;   // wave-uniform
   // wave-level step counts: windows, primes with leaves, leaf-loop steps,
-  // one-at-a-time cross-off primes and steps, grouped cross-off steps
+  // one-at-a-time cross-off primes and steps, grouped cross-off steps,
+  // pair-prefix builds
   unsigned long long u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0,
                      u_gsteps = 0, u_pbuilds = 0;
 
@@ -327,17 +336,43 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     qi = __builtin_amdgcn_readfirstlane(qi);
     if (qi >= nitems)
       break;
-    const SegItem it = items[qi];
-    const QueryDesc qd = queries[it.query];
-    const BInfo* binfo = binfo_all + qd.binfo_off;
-    uint32_t* TC = T_C + it.t_off;
-    int* TM = T_M + it.t_off;
-    const uint32_t b_stop = qd.b_first + it.nbs - 1;
-    const uint64_t low = (uint64_t) it.seg * SEG;
+    // members of the work item (all wave-uniform)
+    uint32_t nmem = 1;
+    SegItem its[NQ];
+    if constexpr (NQ == 1) {
+      its[0] = items[qi];
+    } else {
+      const SegGroup gr = groups[qi];
+      nmem = gr.n;
+#pragma unroll
+      for (int k = 0; k < NQ; k++)
+        its[k] = items[gr.item[k < (int) nmem ? k : 0]];
+    }
+    const QueryDesc qd = queries[its[0].query];     // b_first, pre_last and limit are shared
+    const BInfo* binfo[NQ];
+    uint32_t* TC[NQ];
+    int* TM[NQ];
+    uint32_t b_stop_m[NQ], stop_b[NQ];
+    bool live[NQ];
+    long long S[NQ];
+    uint32_t b_stop = 0;
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      const QueryDesc q = queries[its[k].query];
+      binfo[k] = binfo_all + q.binfo_off;
+      TC[k] = T_C + its[k].t_off;
+      TM[k] = T_M + its[k].t_off;
+      b_stop_m[k] = qd.b_first + its[k].nbs - 1;
+      live[k] = k < (int) nmem;
+      stop_b[k] = live[k] ? b_stop_m[k] + 1 : 0u;
+      S[k] = 0;
+      if (live[k])
+        b_stop = max(b_stop, b_stop_m[k]);
+    }
+    const uint64_t low = (uint64_t) its[0].seg * SEG;
     const uint64_t high = min(low + SEG, qd.limit);
     const uint32_t seg_len = (uint32_t) (high - low);
     const uint64_t low1 = low > 0 ? low : 1;
-    long long S = 0;
 
     // numbers coprime to 30 in [low, high)
     for (uint32_t w = lane; w < W; w += kBlock) {
@@ -369,47 +404,61 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     bool dirty = true;
     while (!done) {
       // Window of up to 64 primes b .. b + 63, lane l <-> b + l, kept in
-      // registers; an epoch reads its prime's values with v_readlane.  The
-      // window ends before the first b whose leaf range is provably empty
-      // (that b and all later ones stop the segment, src/gourdon/D.cpp:
-      // 131-132, 289-290).
+      // registers; an epoch reads its prime's values with v_readlane.  A
+      // member's window ends before its first b whose leaf range is provably
+      // empty (that b and all later ones stop the member in this segment,
+      // src/gourdon/D.cpp:131-132, 289-290); the shared window ends when
+      // every member has stopped.
       const uint32_t bb = b + lane;
-      uint32_t r_lo = 0, r_hi = 0, r_p = 0, r_T = 0;
-      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0, r_h = 0;
-      uint64_t r_xlo = 0;
-      bool brk = true, par = false;
+      uint32_t r_lo[NQ], r_hi[NQ], r_T[NQ];
+      uint64_t r_xlo[NQ];
+      bool brk[NQ];
+      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0, r_h = 0, r_p = 0;
+      bool par = false;
       uint4 prm = make_uint4(0, 0, 0, 0);
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
+        r_lo[k] = r_hi[k] = r_T[k] = 0;
+        r_xlo[k] = 0;
+        brk[k] = true;
+      }
       if (bb <= b_stop) {
-        const BInfo bi = binfo[bb - qd.b_first];
-        const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
-        uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
-        m_lo = m_lo > bi.B ? m_lo : bi.B;
-        if (bi.type == 1) {
-          brk = bi.p >= m_hi;
-          if (!brk) {
-            r_hi = (uint32_t) ft_to_index(m_hi);
-            r_lo = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : r_hi;
+        const uint32_t p = primes[bb];
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+          if (bb < stop_b[k]) {
+            const BInfo bi = binfo[k][bb - qd.b_first];
+            const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
+            uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
+            m_lo = m_lo > bi.B ? m_lo : bi.B;
+            if (bi.type == 1) {
+              brk[k] = bi.p >= m_hi;
+              if (!brk[k]) {
+                r_hi[k] = (uint32_t) ft_to_index(m_hi);
+                r_lo[k] = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : r_hi[k];
+              }
+            } else {
+              r_hi[k] = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
+              brk[k] = r_hi[k] <= bb;
+              if (!brk[k])
+                r_lo[k] = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : r_hi[k];
+            }
+            r_T[k] = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
+            r_xlo[k] = bi.xp_lo;
           }
-        } else {
-          r_hi = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
-          brk = r_hi <= bb;
-          if (!brk)
-            r_lo = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : r_hi;
         }
-        r_p = bi.p;
-        r_T = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
-        r_xlo = bi.xp_lo;
+        r_p = p;
         // cofactors q of p with p*q in (low, high): q_lo .. q_lo + nq - 1
-        const uint64_t q_lo = div_exact((double) low, low, bi.p) + 1;
-        const uint64_t q_hi = div_exact((double) (high - 1), high - 1, bi.p);
+        const uint64_t q_lo = div_exact((double) low, low, p) + 1;
+        const uint64_t q_hi = div_exact((double) (high - 1), high - 1, p);
         r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
         r_qm = (uint32_t) (q_lo % 30);
-        r_off0 = (uint32_t) (bi.p * q_lo - low);
+        r_off0 = (uint32_t) (p * q_lo - low);
         // Every cofactor is below p^2: a number of the segment that is still
         // unsieved when p is crossed off and is a multiple of p is p * q with
         // q = 1 or q prime, and its least prime factor is min(p, q).
-        par = q_hi < (uint64_t) bi.p * bi.p;
-        if (r_nq > 0 && bi.p >= 7) {
+        par = q_hi < (uint64_t) p * p;
+        if (r_nq > 0 && p >= 7) {
           // hits: cofactors coprime to 30; c0 = wheel residues below q_lo % 30
           // (2, 3 and 5 are not in the wheel and have no hits)
           const uint32_t c0 = r_qm ? le_count(r_qm - 1) : 0u;
@@ -418,20 +467,33 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           r_h = 8 * vb + le_count(v - 30 * vb) - c0;
           prm.y = r_off0 | (r_qm << 20) | (c0 << 25);   // off0 < 2^20 when nq > 0
         }
-        prm.x = bi.p;
-        prm.z = q_lo < bi.p ? (uint32_t) (bi.p - q_lo) : 0u;
+        prm.x = p;
+        prm.z = q_lo < p ? (uint32_t) (p - q_lo) : 0u;
       }
-      const unsigned long long bal = __ballot(brk);
-      const uint32_t nwin = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
+      // per member: its window end and its primes with factor entries
+      uint32_t nwin_m[NQ];
+      unsigned long long lmask_m[NQ];
+      uint32_t nwin = 0;
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
+        const unsigned long long bal = __ballot(brk[k]);
+        nwin_m[k] = bal ? (uint32_t) __ffsll((long long) bal) - 1 : 64u;
+        nwin = max(nwin, nwin_m[k]);
+        const unsigned long long in_k = nwin_m[k] >= 64 ? ~0ull : ((1ull << nwin_m[k]) - 1);
+        lmask_m[k] = __ballot(r_hi[k] > r_lo[k]) & in_k;
+      }
       if (nwin < 64)
         done = true;
       const unsigned long long in_win = nwin >= 64 ? ~0ull : ((1ull << nwin) - 1);
       // par (monotone in p) holds for the primes [seq_end, nwin) of the window
       const unsigned long long pbal = __ballot(par) & in_win;
       const uint32_t seq_end = pbal ? (uint32_t) __ffsll((long long) pbal) - 1 : nwin;
-      // primes with factor entries; the cross-offs between two of them are
-      // applied together just before the count of the second
-      const unsigned long long lmask = __ballot(r_hi > r_lo) & in_win;
+      // primes with factor entries of any member; the cross-offs between two
+      // of them are applied together just before the count of the second
+      unsigned long long lmask = 0;
+#pragma unroll
+      for (int k = 0; k < NQ; k++)
+        lmask |= lmask_m[k];
       const uint32_t h_win = lane < nwin ? r_h : 0u;
       // every prime of the window is crossed off by the window's end: its
       // hits are its cofactors coprime to 30
@@ -448,7 +510,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       prm.w = r_h;
       wp[lane] = prm;
       w_rem[lane] = 0;
-      w_m[lane] = 0;
+#pragma unroll
+      for (int k = 0; k < NQ; k++)
+        w_m[k][lane] = 0;
       wave_lds_sync();
 
       // Cross off primes [r0, r1) of the window, in two parts:
@@ -550,12 +614,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       for (uint32_t j = next_leaf(0); j < nwin; j = next_leaf(j + 1)) {
         flush(run, j);
         run = j;
-        u_lprimes++;
-        const uint32_t lo = readlane_u32(r_lo, j);
-        const uint32_t hi = readlane_u32(r_hi, j);
-        const uint32_t Tt = readlane_u32(r_T, j);
-        const uint32_t T = Tt & 0x7fffffffu;
-        const uint64_t xlo = readlane_u64(r_xlo, j);
         // word-pair prefixes of the current sieve state for the count queries
         if (dirty) {
           build_pair_prefix(bm, pre2, W, lane);
@@ -563,76 +621,102 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           dirty = false;
           u_pbuilds++;
         }
-        int M = 0;                                      // wave-uniform
-        if (Tt >> 31) {
-          // factor-table entries: leaves -mu(m) phi(xp / m, b - 1) for the m
-          // whose entry exceeds T (src/gourdon/D.cpp:148-188)
-          u_entries += hi - lo;
-          u_lsteps += (hi - lo + kBlock - 1) / kBlock;
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+          if (!((lmask_m[k] >> j) & 1))
+            continue;
+          u_lprimes++;
+          const uint32_t lo = readlane_u32(r_lo[k], j);
+          const uint32_t hi = readlane_u32(r_hi[k], j);
+          const uint32_t Tt = readlane_u32(r_T[k], j);
+          const uint32_t T = Tt & 0x7fffffffu;
+          const uint64_t xlo = readlane_u64(r_xlo[k], j);
+          int M = 0;                                      // wave-uniform
+          if (Tt >> 31) {
+            // factor-table entries: leaves -mu(m) phi(xp / m, b - 1) for the m
+            // whose entry exceeds T (src/gourdon/D.cpp:148-188)
+            u_entries += hi - lo;
+            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
 #pragma nounroll
-          for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
-            const uint32_t idx = base + lane;
-            const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
-            const bool leaf = f > T;
-            const bool pos = leaf && (f & 1);
-            const uint32_t nl = (uint32_t) __popcll(__ballot(leaf));
-            const uint32_t np = (uint32_t) __popcll(__ballot(pos));
-            u_leaves += nl;
-            M += (int) (2 * np) - (int) nl;
-            if (leaf) {
-              const uint32_t qq = idx / 480;
-              const uint32_t m = 2310 * qq + cop[idx - qq * 480];
-              const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
-              S += pos ? (long long) cnt : -(long long) cnt;
+            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+              const uint32_t idx = base + lane;
+              const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
+              const bool leaf = f > T;
+              const bool pos = leaf && (f & 1);
+              const uint32_t nl = (uint32_t) __popcll(__ballot(leaf));
+              const uint32_t np = (uint32_t) __popcll(__ballot(pos));
+              u_leaves += nl;
+              M += (int) (2 * np) - (int) nl;
+              if (leaf) {
+                const uint32_t qq = idx / 480;
+                const uint32_t m = 2310 * qq + cop[idx - qq * 480];
+                const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
+                S[k] += pos ? (long long) cnt : -(long long) cnt;
+              }
+            }
+          } else {
+            // primes q: leaves +phi(xp / q, b - 1) (src/gourdon/D.cpp:279-302)
+            u_leaves += hi - lo;
+            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
+            M = (int) (hi - lo);
+#pragma nounroll
+            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+              const uint32_t idx = base + lane;
+              if (idx <= hi)
+                S[k] += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
             }
           }
-        } else {
-          // primes q: leaves +phi(xp / q, b - 1) (src/gourdon/D.cpp:279-302)
-          u_leaves += hi - lo;
-          u_lsteps += (hi - lo + kBlock - 1) / kBlock;
-          M = (int) (hi - lo);
-#pragma nounroll
-          for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
-            const uint32_t idx = base + lane;
-            if (idx <= hi)
-              S += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
-          }
+          if (lane == 0)
+            w_m[k][j] = M;
         }
-        if (lane == 0)
-          w_m[j] = M;
       }
       flush(run, nwin);
       wave_lds_sync();
       {
         // C(s, b + l) = count at the window start minus the removals of the
-        // window's earlier primes
+        // window's earlier primes; every member writes the primes before its
+        // own window end
         const uint32_t rem = lane < nwin ? w_rem[lane] : 0u;
         const uint32_t incl = wave_incl_scan(rem);
-        if (lane < nwin) {
-          TC[b + lane - qd.b_first] = total - (incl - rem);
-          TM[b + lane - qd.b_first] = w_m[lane];
+        const uint32_t C = total - (incl - rem);
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+          if (live[k] && b + lane < stop_b[k] && lane < nwin_m[k]) {
+            TC[k][b + lane - qd.b_first] = C;
+            TM[k][b + lane - qd.b_first] = w_m[k][lane];
+          }
         }
         total -= readlane_u32(incl, 63);
+      }
+#pragma unroll
+      for (int k = 0; k < NQ; k++) {
+        if (live[k] && nwin_m[k] < 64)
+          stop_b[k] = min(stop_b[k], b + nwin_m[k]);
       }
       n_slots += nwin;
       u_win++;
       b += nwin;
     }
-    // slots the segment never reached hold zero
-    for (uint32_t bq = b + lane; bq <= b_stop; bq += kBlock) {
-      TC[bq - qd.b_first] = 0;
-      TM[bq - qd.b_first] = 0;
-    }
-    const long long tot = wave_sum_i64(S);
-    if (lane == 0 && tot != 0) {
-      // 128-bit accumulation from two 64-bit atomics: the add whose low word
-      // wraps carries into the high word (exact under any interleaving)
-      unsigned long long* acc = S_acc + 2 * it.query;
-      const unsigned long long v = (unsigned long long) tot;
-      const unsigned long long old = atomicAdd(&acc[0], v);
-      const unsigned long long inc = (old + v < old ? 1ull : 0ull) - (tot < 0 ? 1ull : 0ull);
-      if (inc != 0)
-        atomicAdd(&acc[1], inc);
+#pragma unroll
+    for (int k = 0; k < NQ; k++) {
+      if (!live[k])
+        continue;
+      // slots the member never reached hold zero
+      for (uint32_t bq = stop_b[k] + lane; bq <= b_stop_m[k]; bq += kBlock) {
+        TC[k][bq - qd.b_first] = 0;
+        TM[k][bq - qd.b_first] = 0;
+      }
+      const long long tot = wave_sum_i64(S[k]);
+      if (lane == 0 && tot != 0) {
+        // 128-bit accumulation from two 64-bit atomics: the add whose low word
+        // wraps carries into the high word (exact under any interleaving)
+        unsigned long long* acc = S_acc + 2 * its[k].query;
+        const unsigned long long v = (unsigned long long) tot;
+        const unsigned long long old = atomicAdd(&acc[0], v);
+        const unsigned long long inc = (old + v < old ? 1ull : 0ull) - (tot < 0 ? 1ull : 0ull);
+        if (inc != 0)
+          atomicAdd(&acc[1], inc);
+      }
     }
   }
 
@@ -652,11 +736,11 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   }
 }
 
-template <typename FT>
+template <typename FT, int NQ>
 static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st)
 {
-  hipLaunchKernelGGL(hsl_sieve_kernel<FT>, dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
-                     a.queries, a.items, a.binfo, a.primes,
+  hipLaunchKernelGGL((hsl_sieve_kernel<FT, NQ>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
+                     a.queries, a.items, a.groups, a.binfo, a.primes,
                      reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_C, a.T_M,
                      a.S_acc, a.counters, a.queue, a.nitems, a.W);
   return hipGetLastError();
@@ -815,15 +899,18 @@ hipError_t hsl_upload_constants()
 
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st)
 {
-  return factor32 ? launch_sieve_t<uint32_t>(a, grid, st) : launch_sieve_t<uint16_t>(a, grid, st);
+  if (a.groups)
+    return factor32 ? launch_sieve_t<uint32_t, kGroupMax>(a, grid, st)
+                    : launch_sieve_t<uint16_t, kGroupMax>(a, grid, st);
+  return factor32 ? launch_sieve_t<uint32_t, 1>(a, grid, st) : launch_sieve_t<uint16_t, 1>(a, grid, st);
 }
 
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu)
 {
   size_t lds = hsl_lds_bytes(W);
   if (factor32)
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint32_t>, kBlock, lds);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint16_t>, kBlock, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint32_t, 1>, kBlock, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint16_t, 1>, kBlock, lds);
 }
 
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st)

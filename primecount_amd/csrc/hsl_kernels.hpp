@@ -81,9 +81,20 @@ struct alignas(16) SegItem {
   uint64_t pad1;
 };
 
+/// Queries that share one segment in a launch (kGroupMax of them at most):
+/// the work item is item[0 .. n); all of them cover the full segment and
+/// have the same first b, so the sieve of the segment is the same for all.
+constexpr int kGroupMax = 2;
+struct alignas(16) SegGroup {
+  uint32_t n;
+  uint32_t item[kGroupMax];
+  uint32_t pad[3];
+};
+
 struct HslArgs {
   const QueryDesc* queries;
   const SegItem* items;
+  const SegGroup* groups;       // null: one work item per SegItem
   const BInfo* binfo;
   const uint32_t* primes;       // primes[0] = 0, primes[1] = 2, ...
   const void* factor;           // uint16_t or uint32_t factor table
@@ -94,8 +105,8 @@ struct HslArgs {
   unsigned long long* S_acc;    // per query: 128-bit sum of sign * in-segment count (lo, hi)
   unsigned long long* counters; // leaves, hits, factor entries, slots
   unsigned int* queue;          // next work item
-  uint32_t nitems;
-  uint32_t W;                   // words per segment (multiple of 128, 256..4096)
+  uint32_t nitems;              // work items: SegItems, or SegGroups when groups is set
+  uint32_t W;                   // words per segment (multiple of 128, 128..4096)
 };
 
 constexpr int kBlock = 64;     // one wave per segment

@@ -155,6 +155,47 @@ def test_batch_matches_reference(pc):
     assert pc.D_batch(sub) == [want[xs.index(v)] for v in sub]
 
 
+@pytest.mark.parametrize("words", [128, 512])
+def test_shared_segments(pc, words):
+    # queries of one launch sieve their common segments once (groups of up
+    # to 4); the result of each query is the same with and without sharing
+    # and equals the reference's value
+    fx = golden("batch_D.json")["rows"]
+    xs = [int(r["x"]) for r in fx][:96]
+    want = [int(r["D"]) for r in fx][:96]
+    try:
+        pc.set_segment_words(words)
+        for on in (True, False):
+            pc.set_shared_segments(on)
+            log("D batch of %d, %d-word segments, shared %s" % (len(xs), words, on))
+            assert pc.D_batch(xs) == want
+            # a few large queries and many small ones
+            big = sorted(range(len(xs)), key=lambda i: -xs[i])[:6] + list(range(0, 96, 9))
+            assert pc.D_batch([xs[i] for i in big]) == [want[i] for i in big]
+    finally:
+        pc.set_shared_segments(True)
+        pc.set_segment_words(0)
+
+
+def test_shared_segments_mixed_k(pc, oracle):
+    # explicit parameters with y = z and different k in one serving launch:
+    # queries with another first b are not grouped together
+    rng = random.Random(77)
+    qs = []
+    for _ in range(12):
+        x = rng.randrange(10 ** 9, 3 * 10 ** 10)
+        x13 = round(x ** (1 / 3))
+        y = rng.randint(x13 + 1, 4 * x13)
+        qs.append((x, y, y, rng.choice([5, 6, 7, 8])))
+    log("serving launch of %d D queries with mixed k" % len(qs))
+    with pc.Server() as sv:
+        ids = sv.submit([(pc.KIND_D, x, y, z, k) for x, y, z, k in qs])
+        got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=120000)}
+    for i, (x, y, z, k) in zip(ids, qs):
+        assert got[i]["status"] == 0
+        assert got[i]["value"] == oracle.D(x, y, z, k), (x, y, z, k)
+
+
 @pytest.mark.parametrize("slots", [1 << 14, 1 << 20])
 def test_pass_size_invariance(pc, slots):
     # many kernel passes (carry kept on the device between them) give the
