@@ -656,11 +656,15 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   for (auto& hb : D.stage) hb.ensure(spans_bytes + cols_bytes);
 
   // Shared segments: segment s of every query that covers it in full is
-  // sieved once for groups of up to kGroupMax queries with the same first b;
-  // a query's last (possibly partial) segment goes alone.  Groups are
-  // ordered by segment, the largest queries first within one.
+  // sieved once for groups of up to gmax queries with the same first b; a
+  // query's last (possibly partial) segment goes alone.  Groups are ordered
+  // by segment, the largest queries first within one.  Groups of 4 pay off
+  // for large batches (256 reference x values: 947 ms against 1073 ms with
+  // groups of 2); below that their heavier items and registers cost more
+  // (64 values: 294 ms against 253 ms).
   std::vector<std::vector<SegGroup>> pass_groups(npass);
   size_t max_groups = 0;
+  const uint32_t gmax = nq >= 128 ? 4u : 2u;
   if (share) {
     std::vector<uint32_t> order(nq);
     for (uint32_t i = 0; i < nq; i++) order[i] = i;
@@ -691,7 +695,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
             groups.push_back(one);
             continue;
           }
-          if (g.n == (uint32_t) kGroupMax || (g.n > 0 && qdesc[q].b_first != g_first)) {
+          if (g.n == gmax || (g.n > 0 && qdesc[q].b_first != g_first)) {
             groups.push_back(g);
             g = SegGroup{};
           }
@@ -765,6 +769,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
                              hipMemcpyHostToDevice, s));
       a.groups = (const SegGroup*) D.groups.p;
       a.nitems = (uint32_t) groups.size();
+      a.group_max = gmax;
     }
     a.W = W;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);

@@ -899,9 +899,12 @@ hipError_t hsl_upload_constants()
 
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st)
 {
+  if (a.groups && a.group_max > 2)
+    return factor32 ? launch_sieve_t<uint32_t, 4>(a, grid, st)
+                    : launch_sieve_t<uint16_t, 4>(a, grid, st);
   if (a.groups)
-    return factor32 ? launch_sieve_t<uint32_t, kGroupMax>(a, grid, st)
-                    : launch_sieve_t<uint16_t, kGroupMax>(a, grid, st);
+    return factor32 ? launch_sieve_t<uint32_t, 2>(a, grid, st)
+                    : launch_sieve_t<uint16_t, 2>(a, grid, st);
   return factor32 ? launch_sieve_t<uint32_t, 1>(a, grid, st) : launch_sieve_t<uint16_t, 1>(a, grid, st);
 }
 

@@ -81,10 +81,11 @@ struct alignas(16) SegItem {
   uint64_t pad1;
 };
 
-/// Queries that share one segment in a launch (kGroupMax of them at most):
-/// the work item is item[0 .. n); all of them cover the full segment and
-/// have the same first b, so the sieve of the segment is the same for all.
-constexpr int kGroupMax = 2;
+/// Queries that share one segment in a launch (kGroupMax of them at most;
+/// a launch uses groups of 2 or of 4): the work item is item[0 .. n); all of
+/// them cover the full segment and have the same first b, so the sieve of the
+/// segment is the same for all.
+constexpr int kGroupMax = 4;
 struct alignas(16) SegGroup {
   uint32_t n;
   uint32_t item[kGroupMax];
@@ -106,6 +107,7 @@ struct HslArgs {
   unsigned long long* counters; // leaves, hits, factor entries, slots
   unsigned int* queue;          // next work item
   uint32_t nitems;              // work items: SegItems, or SegGroups when groups is set
+  uint32_t group_max;           // 2 or 4: the largest group (when groups is set)
   uint32_t W;                   // words per segment (multiple of 128, 128..4096)
 };
 
