@@ -126,15 +126,19 @@ double estimate_ms(const ServeQuery& q)
 
 /// Heavy work taken per round: queries in arrival order until their
 /// estimates pass this (at least one), so that light queries arriving
-/// meanwhile wait for at most about this long.
+/// meanwhile wait for at most about this long.  With a backlog of more than
+/// kBacklog heavy queries the round takes ten times as much, for throughput
+/// (the queries of a launch share the sieve of their common segments).
 constexpr double kHeavyRoundMs = 300.0;
+constexpr size_t kBacklog = 16;
 
 } // namespace
 
 /// Worker: per round, what is pending (up to max_batch) in two launch sets,
 /// each published as soon as it is done: all light queries first, then heavy
-/// ones in arrival order up to about kHeavyRoundMs of work (the others go
-/// back to the front of the queue); a set runs one launch sequence per kind.  When a launch fails as a whole,
+/// ones in arrival order up to the round's budget of estimated work (the
+/// others go back to the front of the queue); a set runs one launch
+/// sequence per kind.  When a launch fails as a whole,
 /// its queries are rerun one by one so that every query gets its own status.
 /// Drains the queue before stopping.
 void Server::run()
@@ -152,11 +156,14 @@ void Server::run()
     }
     std::vector<size_t> sets[2];
     std::vector<Pending> later;
+    size_t nheavy = 0;
+    for (const Pending& p : batch) nheavy += !light(p.q);
+    const double budget = nheavy > kBacklog ? 10 * kHeavyRoundMs : kHeavyRoundMs;
     double heavy_ms = 0;
     for (size_t i = 0; i < batch.size(); i++) {
       if (light(batch[i].q)) {
         sets[0].push_back(i);
-      } else if (sets[1].empty() || heavy_ms < kHeavyRoundMs) {
+      } else if (sets[1].empty() || heavy_ms < budget) {
         heavy_ms += estimate_ms(batch[i].q);
         sets[1].push_back(i);
       } else {
