@@ -82,6 +82,41 @@ def measure_pi(pc, x):
             "ac_leaves": st["ac_leaves"], "table_limit": st["table_limit"]}
 
 
+def measure_serving(pc, n, rate):
+    """The first n queries of the serving mix (tests/golden/serve_c5.json:
+    80% pi(x), x = 10^(9+10u); 20% nth_prime(n), n = 10^(8+9u); with the
+    reference's answers): one call per query back to back (the plain path),
+    then through a serving context with Poisson arrivals at `rate` per
+    second.  None when the fixture is absent."""
+    path = os.path.join(REPO, "tests", "golden", "serve_c5.json")
+    if n <= 0 or not os.path.exists(path):
+        return None
+    from primecount_amd import serving_bench as sb
+    qs = sb.load_workload("mix", n, 0)
+    for k, a, _ in sorted(qs, key=lambda q: -q[1])[:2]:       # tables for the largest
+        (pc.pi if k == pc.KIND_PI else pc.nth_prime)(a)
+
+    def pct(v, q):
+        return round(v[min(len(v) - 1, int(q * len(v)))] * 1000, 3)
+
+    lat, el, bad = sb.run_plain(qs)
+    lat.sort()
+    plain = {"queries_per_s": round(len(qs) / el, 2), "p50_ms": pct(lat, 0.5),
+             "p99_ms": pct(lat, 0.99), "bit_exact": bad == 0}
+    sv = pc.Server()
+    try:
+        res, el2, bad2, s0, s1 = sb.run(sv, qs, sb.arrivals(len(qs), rate, 0xC5C5))
+    finally:
+        sv.close()
+    lat2 = sorted(r["seconds"] for r in res)
+    served = {"queries_per_s": round(len(qs) / el2, 2), "p50_ms": pct(lat2, 0.5),
+              "p99_ms": pct(lat2, 0.99), "bit_exact": bad2 == 0,
+              "launch_sets": s1["batches"] - s0["batches"]}
+    return {"workload": "first %d queries of the serving mix" % len(qs),
+            "arrivals": "Poisson, %.3g/s (seed 0xC5C5)" % rate,
+            "plain": plain, "served": served}
+
+
 def traffic_from_profile(x):
     """HBM bytes per sieve-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
@@ -150,6 +185,10 @@ def main():
     ap.add_argument("--segment-words", type=int, default=0,
                     help="64-bit words per sieve segment (0 = automatic)")
     ap.add_argument("--no-pi", action="store_true", help="skip the pi(x) measurement")
+    ap.add_argument("--serving", type=int, default=512,
+                    help="queries of the serving mix to run plain and served (0 = skip)")
+    ap.add_argument("--serving-rate", type=float, default=40.0,
+                    help="Poisson arrival rate of the served run (queries/s)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -257,6 +296,8 @@ def main():
     }
     if not a.no_pi:
         out["pi"] = measure_pi(pc, x)
+    if a.serving:
+        out["serving"] = measure_serving(pc, a.serving, a.serving_rate)
     if a.calibrate_copy:
         out["calibration_copy_bytes"] = device_copy(1 << 30)
     print(json.dumps(out), flush=True)
