@@ -612,6 +612,28 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
 
       uint32_t run = 0;                 // first prime whose cross-off is pending
       for (uint32_t j = next_leaf(0); j < nwin; j = next_leaf(j + 1)) {
+        // members with leaves at j: factor entries that fit one step and
+        // hold no leaf (every entry <= T, a quarter of the primes with
+        // entries at 1e19) are passed over, and the cross-offs stay pending
+        uint32_t active = 0;
+#pragma unroll
+        for (int k = 0; k < NQ; k++) {
+          if (!((lmask_m[k] >> j) & 1))
+            continue;
+          const uint32_t lo = readlane_u32(r_lo[k], j);
+          const uint32_t hi = readlane_u32(r_hi[k], j);
+          const uint32_t Tt = readlane_u32(r_T[k], j);
+          bool has = true;
+          if ((Tt >> 31) && hi - lo <= (uint32_t) kBlock) {
+            const uint32_t idx = lo + 1 + lane;
+            const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
+            has = __ballot(f > (Tt & 0x7fffffffu)) != 0;
+          }
+          if (has)
+            active |= 1u << k;
+        }
+        if (!active)
+          continue;
         flush(run, j);
         run = j;
         // word-pair prefixes of the current sieve state for the count queries
@@ -623,7 +645,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         }
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
-          if (!((lmask_m[k] >> j) & 1))
+          if (!((active >> k) & 1))
             continue;
           u_lprimes++;
           const uint32_t lo = readlane_u32(r_lo[k], j);

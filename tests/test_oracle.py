@@ -119,3 +119,64 @@ def test_oracle_pi_known_values(oracle):
         total = sum(s * oracle.gourdon_term(t, x, y, z, k, xs)
                     for s, t in ((1, "AC"), (-1, "B"), (1, "Phi0"), (1, "Sigma")))
         assert total + oracle.D(x, y, z, k) == known[x]
+
+
+def _is_prime64(n):
+    """Deterministic Miller-Rabin for n < 3.3e24 (bases 2 .. 41)."""
+    if n < 2:
+        return False
+    small = (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41)
+    for p in small:
+        if n % p == 0:
+            return n == p
+    d, r = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        r += 1
+    for a in small:
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(r - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+def test_serving_mix_fixture():
+    """The serving-mix answers (printed by the reference build) belong to the
+    seeded query list, and they agree with what the mathematics fixes: every
+    nth_prime answer is prime, pi(x) never decreases with x and lies between
+    x / ln x and 1.26 x / ln x (Rosser-Schoenfeld), the n-th prime lies
+    between n ln n and n (ln n + ln ln n) (Rosser, Dusart, n >= 6), and
+    pi(nth_prime(n)) = n wherever the list holds both."""
+    import importlib.util
+    import math
+    import os
+    fx = golden("serve_c5.json")
+    spec = importlib.util.spec_from_file_location(
+        "gen_fixtures", os.path.join(os.path.dirname(__file__), "golden", "gen_fixtures.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    qs = gen.serve_queries(fx["seed"], len(fx["rows"]))
+    assert len(qs) == 4096
+    assert [(r["kind"], int(r["arg"])) for r in fx["rows"]] == qs
+    pis = sorted((int(r["arg"]), int(r["value"])) for r in fx["rows"] if r["kind"] == "pi")
+    for (x0, v0), (x1, v1) in zip(pis, pis[1:]):
+        assert v0 <= v1 and v1 - v0 <= x1 - x0
+    for x, v in pis:
+        lx = math.log(x)
+        assert x / lx < v < 1.26 * x / lx, x
+    pi_of = dict(pis)
+    for r in fx["rows"]:
+        if r["kind"] != "nth":
+            continue
+        n, p = int(r["arg"]), int(r["value"])
+        assert _is_prime64(p), n
+        ln = math.log(n)
+        assert n * ln < p < n * (ln + math.log(ln)), n
+        if p in pi_of:
+            assert pi_of[p] == n
