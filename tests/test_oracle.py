@@ -156,6 +156,8 @@ def test_serving_mix_fixture():
     import importlib.util
     import math
     import os
+    if not os.path.exists(os.path.join(os.path.dirname(__file__), "golden", "serve_c5.json")):
+        pytest.skip("serving-mix fixture not generated (tests/golden/gen_fixtures.py --only serve)")
     fx = golden("serve_c5.json")
     spec = importlib.util.spec_from_file_location(
         "gen_fixtures", os.path.join(os.path.dirname(__file__), "golden", "gen_fixtures.py"))
