@@ -189,6 +189,10 @@ def main():
                     help="queries of the serving mix to run plain and served (0 = skip)")
     ap.add_argument("--serving-rate", type=float, default=40.0,
                     help="Poisson arrival rate of the served run (queries/s)")
+    ap.add_argument("--no-lds-probe", action="store_true",
+                    help="use the nominal LDS peak instead of running the LDS probe kernel "
+                         "(for runs under rocprofv3, where launching the probe library's "
+                         "kernel crashed the host process)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -251,7 +255,7 @@ def main():
         "frac": round(hbm_bytes / k_avg / 1e9 / HBM_PEAK_GBS, 6),
         "traffic": traffic,
     }
-    lds_peak = measured_lds_peak()
+    lds_peak = None if a.no_lds_probe else measured_lds_peak()
     lds_peak_used = lds_peak if lds_peak else LDS_PEAK_GBS
     roofline_lds = {
         "bound": "lds",
