@@ -616,7 +616,7 @@ void add_elapsed(hipEvent_t a, hipEvent_t b, double* sec)
 /// the A/C descriptor lists meanwhile), then the D leaves of all of them in
 /// one persistent launch; the term sums are read back once.  Caller holds the
 /// engine lock.
-void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
+void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector<size_t>& idx,
               const std::vector<GourdonParams>& gps, const EngineView& v,
               std::vector<i128>& result, PixStats& s)
 {
@@ -679,7 +679,7 @@ void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
     dq[j].z = gp.z;
     dq[j].k = gp.k;
   }
-  const std::vector<i128> d = hsl_compute_batch_locked(dq, nullptr);
+  const std::vector<i128> d = hsl_compute_batch_locked(dq, nullptr, max_slots);
   PIX_HIP(hipEventRecord(ev[5], v.stream));
 
   std::vector<uint64_t> h(n * kTermWords), pis(n);
@@ -710,7 +710,7 @@ void pi_group(const std::vector<i128>& xs, const std::vector<size_t>& idx,
 
 } // namespace
 
-std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st)
+std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st, uint64_t max_slots)
 {
   PixStats s;
   const double t0 = now_s();
@@ -755,7 +755,7 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st)
       const std::vector<uint64_t> p = pi_points(v, ns);
       for (size_t j = 0; j < small.size(); j++) result[small[j]] = (i128) p[j];
     }
-    pi_group(xs, shared, gps, v, result, s);
+    pi_group(max_slots, xs, shared, gps, v, result, s);
   }
   for (const auto& g : alone) {
     const GourdonParams& gp = gps[g[0]];
@@ -764,7 +764,7 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st)
     const EngineView v = engine_view(gp.y, gp.z, table_need(xs[g[0]], gp.y, gp.x_star));
     s.seconds_table += now_s() - t;
     s.table_limit = std::max<uint64_t>((uint64_t) s.table_limit, (uint64_t) v.pix_n);
-    pi_group(xs, g, gps, v, result, s);
+    pi_group(max_slots, xs, g, gps, v, result, s);
   }
   s.seconds_total = now_s() - t0;
   if (st) *st = s;
@@ -776,7 +776,8 @@ i128 pi_gourdon(i128 x, PixStats* st)
   return pi_gourdon_batch({x}, st)[0];
 }
 
-std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st)
+std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,
+                                      uint64_t max_slots)
 {
   // primes below 2^63, as the reference's 64-bit nth_prime (src/nth_prime.cpp:95-100)
   const uint64_t max_n = 216289611853439384ull;
@@ -816,7 +817,7 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
     if (x0 >= ((uint64_t) 1 << 62)) throw pcg_error("nth_prime(n): n is too large");
     x0s.push_back((i128) x0);
   }
-  const std::vector<i128> cs = pi_gourdon_batch(x0s, &s);
+  const std::vector<i128> cs = pi_gourdon_batch(x0s, &s, max_slots);
   int64_t ymax = 0;
   for (i128 x0 : x0s) ymax = std::max<int64_t>(ymax, gourdon_params(x0).z);
   auto lock = engine_lock();

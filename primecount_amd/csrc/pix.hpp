@@ -26,7 +26,9 @@ i128 pi_gourdon(i128 x, PixStats* st);
 
 /// pi(x) for many x: the terms other than D query by query over shared
 /// resident tables, D for all queries in one persistent launch.
-std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st);
+/// max_slots bounds the slot arrays of a D pass (0 = engine setting).
+std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st,
+                                   uint64_t max_slots = 0);
 
 /// The n-th prime, n <= 216289611853439384 (primes below 2^63): pi(x0) at
 /// x0 = R^-1(n), then a sieve of the interval between x0 and the answer on
@@ -35,6 +37,7 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st);
 uint64_t nth_prime(uint64_t n, PixStats* st);
 
 /// nth_prime for many n (pi at the starting points as one pi_gourdon_batch).
-std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st);
+std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,
+                                      uint64_t max_slots = 0);
 
 } // namespace pcg

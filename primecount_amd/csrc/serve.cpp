@@ -194,11 +194,11 @@ void Server::run()
           } else if (kind == ServeQuery::PI) {
             std::vector<i128> xs;
             for (size_t i : part) xs.push_back(batch[i].q.arg);
-            v = pi_gourdon_batch(xs, nullptr);
+            v = pi_gourdon_batch(xs, nullptr, max_slots_);
           } else {
             std::vector<uint64_t> ns;
             for (size_t i : part) ns.push_back((uint64_t) batch[i].q.arg);
-            for (uint64_t p : nth_prime_batch(ns, nullptr)) v.push_back((i128) p);
+            for (uint64_t p : nth_prime_batch(ns, nullptr, max_slots_)) v.push_back((i128) p);
           }
           for (size_t j = 0; j < part.size(); j++) res[part[j]].value = v[j];
         };

@@ -132,3 +132,19 @@ def test_serving_pi_and_nth_prime(pc, oracle):
         got = {r["id"]: r for r in sv.poll_all(len(ids2), timeout_ms=180000)}
         for i, q, w in zip(ids2, qs, want):
             assert got[i]["status"] == 0 and got[i]["value"] == w, q
+
+
+def test_serving_small_hbm_budget(pc):
+    """A small slot budget (hbm_budget) cuts the D launches of pi(x) and D
+    queries into many passes by segment range; the answers do not change."""
+    known = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
+             if int(r["x"]) <= 10 ** 15]
+    drow = fixture_rows(10 ** 15)[:8]
+    with pc.Server(hbm_budget=1 << 20) as sv:
+        log("serving with a 1 MiB slot budget: %d pi, %d D" % (len(known), len(drow)))
+        qs = [(pc.KIND_PI, x) for x, _ in known] + [(pc.KIND_D, x) for x, _ in drow]
+        want = [v for _, v in known] + [v for _, v in drow]
+        ids = sv.submit(qs)
+        got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=180000)}
+        for i, q, w in zip(ids, qs, want):
+            assert got[i]["status"] == 0 and got[i]["value"] == w, q
