@@ -187,7 +187,7 @@ def main():
     ap.add_argument("--no-pi", action="store_true", help="skip the pi(x) measurement")
     ap.add_argument("--serving", type=int, default=512,
                     help="queries of the serving mix to run plain and served (0 = skip)")
-    ap.add_argument("--serving-rate", type=float, default=40.0,
+    ap.add_argument("--serving-rate", type=float, default=25.0,
                     help="Poisson arrival rate of the served run (queries/s)")
     ap.add_argument("--no-lds-probe", action="store_true",
                     help="use the nominal LDS peak instead of running the LDS probe kernel "
