@@ -118,6 +118,14 @@ int pcg_set_segment_words(uint32_t words);
  * own leaves.  1 (default) on, 0 off.  Never changes results. */
 int pcg_set_shared_segments(int on);
 
+/* Self-check of the device factor-table builder: builds the FactorTableD
+ * (kind PCG_KIND_D, entries up to limit with prime factors <= y) or the lpf
+ * FactorTable (PCG_KIND_S2_HARD) on the device and on the host and stores the
+ * number of differing entries in *mismatches.  No counterpart in the
+ * reference (src/gourdon/FactorTableD.hpp, src/FactorTable.hpp build on the
+ * host); limit < 2^32. */
+int pcg_factor_table_check(int kind, int64_t y, int64_t limit, uint64_t* mismatches);
+
 /* (segment, prime) slots per kernel pass (0 = default, 2^31).  C and M take
  * 8 bytes of HBM per slot; larger queries run in several passes whose carry is
  * kept on the device.  The pass size never changes results.  No counterpart in

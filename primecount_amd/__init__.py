@@ -134,6 +134,8 @@ def load():
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
+        "pcg_factor_table_check": ([ctypes.c_int, I64, I64, ctypes.POINTER(ctypes.c_uint64)],
+                                   ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
         "pcg_pi": ([_I128, PI128], ctypes.c_int),
@@ -308,6 +310,13 @@ def set_alpha(a):
 
 def set_segment_words(w):
     _check(load().pcg_set_segment_words(int(w)))
+
+
+def factor_table_check(kind, y, limit):
+    """Entries in which the device-built factor table differs from the host's."""
+    out = ctypes.c_uint64()
+    _check(load().pcg_factor_table_check(int(kind), int(y), int(limit), ctypes.byref(out)))
+    return out.value
 
 
 def set_shared_segments(on):

@@ -367,6 +367,16 @@ int pcg_set_segment_words(uint32_t words)
   });
 }
 
+int pcg_factor_table_check(int kind, int64_t y, int64_t limit, uint64_t* mismatches)
+{
+  return guarded([&] {
+    if (!mismatches) throw pcg_error("mismatches must not be NULL");
+    if (kind != PCG_KIND_D && kind != PCG_KIND_S2_HARD) throw pcg_error("unknown table kind");
+    *mismatches = factor_table_mismatches(kind == PCG_KIND_D ? LeafSum::D : LeafSum::S2_hard, y,
+                                          limit);
+  });
+}
+
 int pcg_set_shared_segments(int on)
 {
   return guarded([&] { engine_config().share_segments = on != 0; });

@@ -169,9 +169,8 @@ struct TableSet {
   LeafSum kind = LeafSum::D;
   bool factor32 = false;
   int64_t ft_y = 0, ft_limit = 0;   // factor table parameters
+  uint64_t factor_n = 0;            // factor table entries (built on the device)
   PrimeTables P;
-  std::vector<uint16_t> f16;
-  std::vector<uint32_t> f32;
 };
 
 /// Everything the device needs for one query.
@@ -416,16 +415,13 @@ TableSet build_tables(const TableNeed& n)
   int threads = engine_config().host_threads > 0 ? engine_config().host_threads : default_threads();
   T.P = build_prime_tables(n.pmax, threads);
   T.ft_y = n.fy;
-  T.ft_limit = n.flim;
-  if (T.kind == LeafSum::D) {
-    T.factor32 = n.flim > kFactorTableD16Max;
-    if (T.factor32) T.f32 = build_factor_table_D<uint32_t>(n.fy, n.flim, T.P, threads);
-    else T.f16 = build_factor_table_D<uint16_t>(n.fy, n.flim, T.P, threads);
-  } else {
-    T.factor32 = n.flim > kFactorTable16Max;
-    if (T.factor32) T.f32 = build_factor_table_lpf<uint32_t>(n.flim, T.P, threads);
-    else T.f16 = build_factor_table_lpf<uint16_t>(n.flim, T.P, threads);
-  }
+  T.ft_limit = std::max<int64_t>(1, n.flim);
+  T.factor32 = T.kind == LeafSum::D ? n.flim > kFactorTableD16Max : n.flim > kFactorTable16Max;
+  T.factor_n = (uint64_t) ft_to_index((uint64_t) T.ft_limit) + 1;
+  if ((uint64_t) T.ft_limit >= (1ull << 32))
+    throw pcg_error("factor table limit must be below 2^32");
+  if ((uint64_t) isqrt((u128) T.ft_limit) > T.P.max)
+    throw pcg_error("internal: factor table needs primes up to its square root");
   return T;
 }
 
@@ -452,8 +448,13 @@ const TableSet& resident_tables(const std::vector<HslQuery>& qs)
   upload(D.primes, T.P.primes.data(), T.P.primes.size() * 4);
   upload(D.pi_bits, T.P.pi_bits.data(), T.P.pi_bits.size() * 8);
   upload(D.pi_cnt, T.P.pi_cnt.data(), T.P.pi_cnt.size() * 4);
-  if (T.factor32) upload(D.factor, T.f32.data(), T.f32.size() * 4);
-  else upload(D.factor, T.f16.data(), T.f16.size() * 2);
+  // the factor table is built where it is used: on the device
+  D.factor.ensure(T.factor_n * (T.factor32 ? 4 : 2));
+  PCG_HIP(hsl_launch_factor_table(D.factor.p, T.factor32, T.kind == LeafSum::D,
+                                  (uint64_t) T.ft_limit, (uint64_t) T.ft_y,
+                                  (const uint32_t*) D.primes.p, (uint32_t) T.P.primes.size(),
+                                  D.stream));
+  PCG_HIP(hipStreamSynchronize(D.stream));
   g_tables_valid = true;
   return g_tables;
 }
@@ -905,6 +906,52 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
   return out;
 }
 
+uint64_t factor_table_mismatches(LeafSum kind, int64_t y, int64_t limit)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_dev.init();
+  if (y < 1 || limit < 1 || (uint64_t) limit >= (1ull << 32))
+    throw pcg_error("factor table check needs y >= 1 and 1 <= limit < 2^32");
+  const bool d_table = kind == LeafSum::D;
+  const bool f32 = d_table ? limit > kFactorTableD16Max : limit > kFactorTable16Max;
+  const int threads = default_threads();
+  const PrimeTables P = build_prime_tables(std::max<uint64_t>(isqrt((u128) limit), 13) + 1, threads);
+  std::vector<uint32_t> host;
+  if (d_table) {
+    if (f32) host = build_factor_table_D<uint32_t>(y, limit, P, threads);
+    else for (uint16_t v : build_factor_table_D<uint16_t>(y, limit, P, threads)) host.push_back(v);
+  } else {
+    if (f32) host = build_factor_table_lpf<uint32_t>(limit, P, threads);
+    else for (uint16_t v : build_factor_table_lpf<uint16_t>(limit, P, threads)) host.push_back(v);
+  }
+  DevBuf dp, dout;
+  dp.ensure(P.primes.size() * 4);
+  dout.ensure(host.size() * (f32 ? 4 : 2));
+  uint64_t bad = 0;
+  try {
+    PCG_HIP(hipMemcpy(dp.p, P.primes.data(), P.primes.size() * 4, hipMemcpyHostToDevice));
+    PCG_HIP(hsl_launch_factor_table(dout.p, f32, d_table, (uint64_t) limit, (uint64_t) y,
+                                    (const uint32_t*) dp.p, (uint32_t) P.primes.size(), g_dev.stream));
+    PCG_HIP(hipStreamSynchronize(g_dev.stream));
+    if (f32) {
+      std::vector<uint32_t> dev(host.size());
+      PCG_HIP(hipMemcpy(dev.data(), dout.p, dev.size() * 4, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < host.size(); i++) bad += dev[i] != host[i];
+    } else {
+      std::vector<uint16_t> dev(host.size());
+      PCG_HIP(hipMemcpy(dev.data(), dout.p, dev.size() * 2, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < host.size(); i++) bad += dev[i] != host[i];
+    }
+  } catch (...) {
+    dp.release();
+    dout.release();
+    throw;
+  }
+  dp.release();
+  dout.release();
+  return bad;
+}
+
 void hsl_validate(const HslQuery& q)
 {
   validate(q);
@@ -994,7 +1041,7 @@ EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n)
   v.host = &T.P;
   v.factor = D.factor.p;
   v.factor32 = T.factor32;
-  v.factor_n = T.factor32 ? T.f32.size() : T.f16.size();
+  v.factor_n = T.factor_n;
   v.pix_tab = (const PixWord*) D.pix_tab.p;
   v.pix_n = D.pix_n;
   return v;

@@ -45,6 +45,11 @@ std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* 
 std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_stats* stats,
                                            uint64_t max_slots = 0);
 
+/// Builds the factor table of (kind, y, limit) on the device and on the host
+/// (tables.cpp, checked entry by entry against the definitions) and returns
+/// the number of entries that differ.
+uint64_t factor_table_mismatches(LeafSum kind, int64_t y, int64_t limit);
+
 /// Throws pcg_error if the engine cannot take the query (domain checks only;
 /// no device work).
 void hsl_validate(const HslQuery& q);

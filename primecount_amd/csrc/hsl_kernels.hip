@@ -899,8 +899,129 @@ __global__ void hsl_combine_final(const CombineCol* __restrict__ cols, uint32_t 
   out[g] += acc;
 }
 
+/// Factor table built on the device (FactorTableD for D, the lpf FactorTable
+/// for S2_hard; src/gourdon/FactorTableD.hpp:69-260, src/FactorTable.hpp:
+/// 66-219): one workgroup factors the numbers coprime to 2310 in
+/// kFtBlocks consecutive blocks of 2310 numbers.  For every prime p >= 13 with
+/// p^2 below the end of the range (in order, one barrier apart, so that the
+/// updates of one prime never meet), the threads walk the multiples p q with q
+/// >= 2 coprime to 2310 and record the smallest and largest prime index, the
+/// number of prime factors, the cofactor and whether p^2 divides.  A number
+/// with no recorded factor is 1 or a prime; a cofactor above 1 is the one
+/// prime factor above the square root.  Entries: D (kD): 1 -> T_MAX - 1, a
+/// prime <= y -> T_MAX, not square-free or a prime factor > y -> 0, else
+/// 2 pi(lpf) + [mu = -1]; lpf: 1 -> T_MAX - 1, a prime -> T_MAX, not
+/// square-free -> 0, else lpf - [mu = +1].
+constexpr uint32_t kFtBlocks = 8;
+
+template <typename T, bool kD>
+__global__ __launch_bounds__(256) void hsl_factor_table_kernel(T* __restrict__ out, uint64_t limit,
+                                                               uint64_t y,
+                                                               const uint32_t* __restrict__ primes,
+                                                               uint32_t nprimes)
+{
+  constexpr uint32_t NE = kFtBlocks * 480;
+  __shared__ uint32_t rem[NE];
+  __shared__ uint16_t lpfi[NE], mpfi[NE];
+  __shared__ uint8_t nf[NE], sqf[NE];
+  __shared__ uint16_t cop[480];
+  __shared__ int16_t copi[2310];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i < 480; i += 256)
+    cop[i] = c_cop[i];
+  for (uint32_t i = t; i < 2310; i += 256)
+    copi[i] = c_cop_index[i];
+  const uint64_t q0 = (uint64_t) blockIdx.x * kFtBlocks;     // first 2310-block
+  const uint64_t n_lo = q0 * 2310;
+  const uint64_t n_hi = min((q0 + kFtBlocks) * 2310, limit + 1);   // [n_lo, n_hi)
+  const uint64_t i0 = q0 * 480;
+  __syncthreads();
+  for (uint32_t e = t; e < NE; e += 256) {
+    rem[e] = (uint32_t) (2310 * (q0 + e / 480) + cop[e % 480]);
+    lpfi[e] = 0;
+    mpfi[e] = 0;
+    nf[e] = 0;
+    sqf[e] = 1;
+  }
+  __syncthreads();
+  for (uint32_t j = 6; j < nprimes; j++) {
+    const uint64_t p = primes[j];
+    if (p * p >= n_hi)
+      break;
+    // cofactors q >= max(2, ceil(n_lo / p)) coprime to 2310 with p q < n_hi;
+    // q = 1 is left out so that a prime keeps no recorded factor
+    const uint64_t qmin = max<uint64_t>(2, (n_lo + p - 1) / p);
+    const uint64_t qmax = (n_hi - 1) / p;
+    if (qmax >= qmin) {
+      const uint64_t qi_lo = 480 * ((qmin - 1) / 2310) + copi[(qmin - 1) % 2310] + 1;
+      const uint64_t qi_hi = 480 * (qmax / 2310) + copi[qmax % 2310];
+      for (uint64_t qi = qi_lo + t; qi <= qi_hi; qi += 256) {
+        const uint64_t q = 2310 * (qi / 480) + cop[qi % 480];
+        const uint64_t m = p * q;
+        const uint32_t e = (uint32_t) (480 * (m / 2310) + copi[m % 2310] - i0);
+        if (lpfi[e] == 0)
+          lpfi[e] = (uint16_t) j;
+        mpfi[e] = (uint16_t) j;
+        nf[e]++;
+        const uint32_t r = rem[e] / (uint32_t) p;
+        rem[e] = r;
+        if (r % (uint32_t) p == 0)
+          sqf[e] = 0;
+      }
+    }
+    __syncthreads();
+  }
+  const T T_MAX = (T) ~(T) 0;
+  for (uint32_t e = t; e < NE; e += 256) {
+    const uint64_t n = 2310 * (q0 + e / 480) + cop[e % 480];
+    if (n > limit)
+      continue;
+    const uint32_t r = rem[e];
+    const uint32_t nfac = nf[e] + (r > 1 ? 1u : 0u);
+    T v;
+    if (n == 1)
+      v = (T) (T_MAX - 1);
+    else if (lpfi[e] == 0)
+      v = kD ? (n <= y ? T_MAX : (T) 0) : T_MAX;          // n is prime
+    else if (!sqf[e])
+      v = 0;
+    else if (kD) {
+      const uint64_t mpf = r > 1 ? r : primes[mpfi[e]];
+      v = mpf > y ? (T) 0 : (T) (2u * lpfi[e] + (nfac & 1));
+    } else {
+      const uint32_t lpf = primes[lpfi[e]];
+      v = (T) ((nfac & 1) ? lpf : lpf - 1);
+    }
+    out[i0 + e] = v;
+  }
+}
+
 // ------------------------------------------------------------------------
 // host-side launch wrappers
+
+hipError_t hsl_launch_factor_table(void* out, bool factor32, bool d_table, uint64_t limit,
+                                   uint64_t y, const uint32_t* primes, uint32_t nprimes,
+                                   hipStream_t st)
+{
+  const uint64_t nblk = (limit / 2310 + kFtBlocks) / kFtBlocks;
+  const dim3 g((uint32_t) nblk), b(256);
+  if (factor32) {
+    if (d_table)
+      hipLaunchKernelGGL((hsl_factor_table_kernel<uint32_t, true>), g, b, 0, st, (uint32_t*) out,
+                         limit, y, primes, nprimes);
+    else
+      hipLaunchKernelGGL((hsl_factor_table_kernel<uint32_t, false>), g, b, 0, st, (uint32_t*) out,
+                         limit, y, primes, nprimes);
+  } else {
+    if (d_table)
+      hipLaunchKernelGGL((hsl_factor_table_kernel<uint16_t, true>), g, b, 0, st, (uint16_t*) out,
+                         limit, y, primes, nprimes);
+    else
+      hipLaunchKernelGGL((hsl_factor_table_kernel<uint16_t, false>), g, b, 0, st, (uint16_t*) out,
+                         limit, y, primes, nprimes);
+  }
+  return hipGetLastError();
+}
 
 hipError_t hsl_upload_constants()
 {

@@ -138,6 +138,14 @@ hipError_t hsl_upload_constants();
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
+
+/// Factor table entries for the indices 0 .. ft_to_index(limit) (numbers
+/// coprime to 2310 up to limit < 2^32): the FactorTableD encoding for D
+/// (d_table, leaves need mpf <= y) or the lpf encoding for S2_hard; primes
+/// [0, nprimes) must cover sqrt(limit).
+hipError_t hsl_launch_factor_table(void* out, bool factor32, bool d_table, uint64_t limit,
+                                   uint64_t y, const uint32_t* primes, uint32_t nprimes,
+                                   hipStream_t st);
 hipError_t hsl_launch_build_items(const QueryDesc* queries, const PassSpan* spans, uint32_t nspans,
                                   const uint32_t* reach, const uint64_t* sfx, SegItem* items,
                                   uint32_t nitems, hipStream_t st);

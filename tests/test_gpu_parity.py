@@ -215,3 +215,16 @@ def test_pass_size_invariance(pc, slots):
         assert pc.D_batch(xs) == want
     finally:
         pc.set_pass_slots(0)
+
+
+@pytest.mark.parametrize("kind,y,limit", [
+    ("D", 241615, 241615), ("D", 11914935, 11914935), ("D", 88560164, 88560164),
+    ("D", 1000, 100000), ("D", 5000, 2000000), ("D", 13, 10 ** 6),
+    ("S2", 494134, 494134), ("S2", 10620399, 10620399), ("S2", 1, 50000)])
+def test_device_factor_table(pc, kind, y, limit):
+    # the factor tables are built on the device; every entry equals the host
+    # builder's, which tests/native/tables_check.cpp checks against the
+    # definitions (lpf, mpf, mu)
+    log("factor table %s y=%d limit=%d" % (kind, y, limit))
+    k = pc.KIND_D if kind == "D" else pc.KIND_S2_HARD
+    assert pc.factor_table_check(k, y, limit) == 0
