@@ -188,6 +188,25 @@ int pcg_B(pcg_i128 x, int64_t y, pcg_i128* out);
 int pcg_Phi0(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);
 int pcg_Sigma(pcg_i128 x, int64_t y, pcg_i128* out);
 
+/* pi(x) by the Deleglise-Rivat algorithm, pi(x) = S1 + S2_trivial + S2_easy
+ * + S2_hard + pi(y) - 1 - P2, with the reference's parameters (y, z, c of
+ * pcg_dr_params); every term on the device (S2_hard on the hard-special-leaf
+ * engine, S1 from its lpf factor table, S2_easy and P2 over the prime-count
+ * table; S2_trivial is a closed form over pi(n) lookups).  Replaces
+ * pi_deleglise_rivat (include/primecount-internal.hpp:33, 59 ->
+ * src/deleglise-rivat/pi_deleglise_rivat.cpp:40-140).  Domain as pcg_pi. */
+int pcg_pi_deleglise_rivat(pcg_i128 x, pcg_i128* out);
+
+/* The Deleglise-Rivat terms with explicit parameters: S1, S2_trivial,
+ * S2_easy (include/S.hpp:22-24, 29-31; src/S1.cpp,
+ * src/deleglise-rivat/S2_trivial.cpp, S2_easy.cpp; S1 needs 5 <= c <= 8) and
+ * P2 (include/primecount-internal.hpp:43, 61; src/P2.cpp; pi(y) is computed
+ * here).  Domain: x <= y * 2^62, y < 2^32, z < 2^50, x / y < 2^50. */
+int pcg_S1(pcg_i128 x, int64_t y, int64_t c, pcg_i128* out);
+int pcg_S2_trivial(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);
+int pcg_S2_easy(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);
+int pcg_P2(pcg_i128 x, int64_t y, pcg_i128* out);
+
 /* The n-th prime, 1 <= n <= 216289611853439384 (the primes below 2^63):
  * pi(x0) on the device at x0 = R^-1(n) (Riemann's R), then a device sieve of
  * the interval between x0 and the answer.  Replaces nth_prime /

@@ -145,6 +145,11 @@ def load():
         "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
         "pcg_Phi0": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_Sigma": ([_I128, I64, PI128], ctypes.c_int),
+        "pcg_pi_deleglise_rivat": ([_I128, PI128], ctypes.c_int),
+        "pcg_S1": ([_I128, I64, I64, PI128], ctypes.c_int),
+        "pcg_S2_trivial": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_S2_easy": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_P2": ([_I128, I64, PI128], ctypes.c_int),
         "pcg_ctx_create": ([I64, I64, ctypes.c_uint64, ctypes.c_uint32,
                             ctypes.POINTER(ctypes.c_void_p)], ctypes.c_int),
         "pcg_submit": ([ctypes.c_void_p, ctypes.POINTER(_Query), ctypes.c_size_t,
@@ -280,6 +285,37 @@ def Phi0(x, y, z, k):
 def Sigma(x, y):
     out = _I128()
     _check(load().pcg_Sigma(_i128(x), int(y), ctypes.byref(out)))
+    return _from128(out)
+
+
+def pi_deleglise_rivat(x):
+    """pi(x) by the Deleglise-Rivat algorithm, every term on the device."""
+    out = _I128()
+    _check(load().pcg_pi_deleglise_rivat(_i128(x), ctypes.byref(out)))
+    return _from128(out)
+
+
+def S1(x, y, c):
+    out = _I128()
+    _check(load().pcg_S1(_i128(x), int(y), int(c), ctypes.byref(out)))
+    return _from128(out)
+
+
+def S2_trivial(x, y, z, c):
+    out = _I128()
+    _check(load().pcg_S2_trivial(_i128(x), int(y), int(z), int(c), ctypes.byref(out)))
+    return _from128(out)
+
+
+def S2_easy(x, y, z, c):
+    out = _I128()
+    _check(load().pcg_S2_easy(_i128(x), int(y), int(z), int(c), ctypes.byref(out)))
+    return _from128(out)
+
+
+def P2(x, y):
+    out = _I128()
+    _check(load().pcg_P2(_i128(x), int(y), ctypes.byref(out)))
     return _from128(out)
 
 

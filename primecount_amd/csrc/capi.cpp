@@ -239,6 +239,46 @@ int pcg_Sigma(pcg_i128 x, int64_t y, pcg_i128* out)
   });
 }
 
+int pcg_pi_deleglise_rivat(pcg_i128 x, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(pi_deleglise_rivat(from_abi(x)));
+  });
+}
+
+int pcg_S1(pcg_i128 x, int64_t y, int64_t c, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(dr_term(DRTerm::S1, from_abi(x), y, y, c));
+  });
+}
+
+int pcg_S2_trivial(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(dr_term(DRTerm::S2_trivial, from_abi(x), y, z, c));
+  });
+}
+
+int pcg_S2_easy(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(dr_term(DRTerm::S2_easy, from_abi(x), y, z, c));
+  });
+}
+
+int pcg_P2(pcg_i128 x, int64_t y, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    *out = to_abi(dr_term(DRTerm::P2, from_abi(x), y, y, 8));
+  });
+}
+
 int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t max_batch,
                    pcg_ctx** ctx)
 {

@@ -436,9 +436,10 @@ void upload(DevBuf& b, const void* src, size_t bytes)
 TableSet g_tables;
 bool g_tables_valid = false;
 
-const TableSet& resident_tables(const std::vector<HslQuery>& qs)
+const TableSet& resident_tables(const std::vector<HslQuery>& qs, uint64_t pmin = 0)
 {
   TableNeed need = table_need(qs);
+  need.pmax = std::max(need.pmax, pmin);
   if (g_tables_valid && covers(g_tables, need))
     return g_tables;
   g_tables_valid = false;
@@ -1021,17 +1022,17 @@ static void ensure_pix_table(uint64_t n)
   D.pix_n = nwords * 240 - 1 < n ? nwords * 240 - 1 : n;
 }
 
-EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n)
+EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n, LeafSum kind, uint64_t pmin)
 {
   Device& D = g_dev;
   D.init();
   HslQuery q;
-  q.kind = LeafSum::D;
+  q.kind = kind;
   q.x = 1;
   q.y = y;
   q.z = z;
   validate(q);
-  const TableSet& T = resident_tables({q});
+  const TableSet& T = resident_tables({q}, pmin);
   if (pix_n) ensure_pix_table(pix_n);
   EngineView v;
   v.stream = D.stream;

@@ -83,9 +83,12 @@ struct EngineView {
 
 std::unique_lock<std::mutex> engine_lock();
 
-/// Makes the D tables for (y, z) (y >= z shares one prefix-valid table) and
-/// a prime-count table covering [0, pix_n] resident; caller holds the lock.
-EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n);
+/// Makes the D tables for (y, z) (y >= z shares one prefix-valid table), or
+/// the S2_hard tables (kind), and a prime-count table covering [0, pix_n]
+/// resident, with host and device prime tables covering at least [0, pmin];
+/// caller holds the lock.
+EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n, LeafSum kind = LeafSum::D,
+                       uint64_t pmin = 0);
 
 /// A device buffer of at least `bytes`, kept between calls (slot 0 .. 7);
 /// caller holds the lock.

@@ -195,8 +195,8 @@ void launch_phi0(i128 x, int64_t z, int64_t k, const EngineView& v, const PhiTin
   static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
   const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) z) + 1;
   if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for Phi0");
-  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, nidx,
-                          (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, acc, v.stream));
+  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, false,
+                          nidx, (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, acc, v.stream));
 }
 
 const PhiTiny& phi0_table(int64_t k)
@@ -774,6 +774,212 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st, ui
 i128 pi_gourdon(i128 x, PixStats* st)
 {
   return pi_gourdon_batch({x}, st)[0];
+}
+
+// ---------------------------------------------------------------------------
+// Deleglise-Rivat: pi(x) = S1 + S2_trivial + S2_easy + S2_hard + pi(y) - 1 - P2
+// (src/deleglise-rivat/pi_deleglise_rivat.cpp:40-53, 86-92)
+
+namespace {
+
+/// S2_easy leaves as descriptors (src/deleglise-rivat/S2_easy.cpp:53-160):
+/// for max(c, pi(sqrt y)) < b <= pi(x^(1/3)) the leaves pi(x / (p_b q)) - b + 2
+/// over the primes q of the clustered range (A, B] = (min_clustered,
+/// min_trivial] and then of the sparse range (min_sparse, i].  The sparse
+/// leaves are summed one by one.  The clustered ones have q > sqrt(x / p_b), so
+/// their quotients v = x / (p_b q) are below sqrt(x / p_b) and repeat; the
+/// reference walks the runs of equal pi(v).  Here the same sum is counted from
+/// the other side, over the pairs of primes (q, r) with r q <= x / p_b:
+///   sum_{A < q <= B} pi(xp / q) = pi(xp / B) (pi(B) - pi(A))
+///       + sum_{xp / B < r <= xp / (A + 1)} (pi(xp / r) - pi(A)),
+/// one leaf pi(xp / r) per prime r below sqrt(x / p_b) instead of one per q
+/// (at x = 1e19: 3.2e10 leaves instead of 2.0e11).  The -b + 2 parts and the
+/// pi(A), pi(B) products go to the correction.
+LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P)
+{
+  LeafLists L;
+  const u128 ux = (u128) x;
+  const uint64_t uy = (uint64_t) y, uz = (uint64_t) z;
+  auto pi = [&](uint64_t n) -> uint64_t {
+    if (n > P.max) throw pcg_error("internal: prime table too small for S2_easy");
+    return P.pi(n);
+  };
+  const uint64_t pi_sqrty = pi(isqrt((u128) uy));
+  const uint64_t pi_x13 = pi(iroot<3>(ux));
+  for (uint64_t b = umax((uint64_t) c, pi_sqrty) + 1; b <= pi_x13; b++) {
+    const uint64_t prime = P.primes[b];
+    const u128 xp128 = ux / prime;
+    if (xp128 >> 64) throw pcg_error("S2_easy: x / p does not fit in 64 bits");
+    const uint64_t xp = (uint64_t) xp128;
+    const uint64_t min_trivial = umin(xp / prime, uy);
+    const uint64_t min_clustered = in_between<uint64_t>(prime, isqrt((u128) xp), uy);
+    const uint64_t min_sparse = in_between<uint64_t>(prime, uz / prime, uy);
+    uint64_t i = pi(min_trivial);
+    const uint64_t pi_min_clustered = pi(min_clustered);
+    const uint64_t pi_min_sparse = pi(min_sparse);
+    if (i > pi_min_clustered) {
+      // q in (A, B]: A = min_clustered >= isqrt(xp), B = min_trivial
+      const uint64_t nq = i - pi_min_clustered;
+      const uint64_t r_lo = pi(xp / min_trivial);            // pi(xp / B)
+      const uint64_t r_hi = pi(xp / (min_clustered + 1));    // pi(xp / (A + 1))
+      L.correction += (i128) r_lo * nq + (i128) nq * (2 - (i128) b);
+      if (r_hi > r_lo) {
+        add_desc(L.plus, L.n_plus, xp, ~0ull, r_lo + 1, r_hi);
+        L.correction -= (i128) (r_hi - r_lo) * (i128) pi_min_clustered;
+      }
+      i = pi_min_clustered;
+    }
+    if (i > pi_min_sparse) {
+      add_desc(L.plus, L.n_plus, xp, ~0ull, pi_min_sparse + 1, i);
+      L.correction += (i128) (i - pi_min_sparse) * (2 - (i128) b);
+    }
+  }
+  return L;
+}
+
+/// S2_trivial (src/deleglise-rivat/S2_trivial.cpp:30-77), on the host: for
+/// the primes max(p_c, sqrt z) < p < y with x / p^2 > p, pi(y) - pi(x / p^2);
+/// from the first other prime below y on, the arithmetic series of pi(y) - b.
+i128 s2_trivial_host(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P)
+{
+  if (y < 2) return 0;
+  const uint64_t uy = (uint64_t) y;
+  if (uy > P.max) throw pcg_error("internal: prime table too small for S2_trivial");
+  const i128 pi_y = (i128) P.pi(uy);
+  const uint64_t sqrtz = isqrt((u128) z);
+  const uint64_t prime_c = P.primes[(size_t) c];
+  const uint64_t start = umax(prime_c, sqrtz) + 1;
+  if (start >= uy) return 0;
+  i128 sum = 0;
+  uint64_t prime = uy;                 // the prime the loop stopped at (>= y: none)
+  for (uint64_t b = P.pi(start - 1) + 1; b < P.primes.size(); b++) {
+    const uint64_t p = P.primes[b];
+    if (p >= uy) break;
+    const uint64_t xpp = (uint64_t) ((u128) x / ((u128) p * p));
+    if (xpp <= p) {
+      prime = p;
+      break;
+    }
+    sum += pi_y - (i128) P.pi(xpp);
+  }
+  if (prime < uy) {
+    const i128 n = (i128) P.pi(uy - 1) - (i128) P.pi(prime) + 1;
+    const i128 a1 = pi_y - (i128) P.pi(uy - 1);
+    const i128 a2 = pi_y - (i128) P.pi(prime);
+    sum += n * (a1 + a2) / 2;
+  }
+  return sum;
+}
+
+/// P2 (src/P2.cpp:95-127): the sum of pi(x / p) over y < p <= sqrt(x) (the
+/// B kernel) plus (a - 2)(a + 1)/2 - (b - 2)(b + 1)/2, a = pi(y), b = pi(sqrt x).
+i128 run_p2(i128 x, int64_t y, const EngineView& v)
+{
+  if (x < 4) return 0;
+  const uint64_t sqrtx = isqrt((u128) x);
+  if ((uint64_t) y >= sqrtx) return 0;
+  const i128 a = (i128) v.host->pi((uint64_t) y);
+  const i128 b = (i128) pi_points(v, {sqrtx})[0];
+  return (a - 2) * (a + 1) / 2 - (b - 2) * (b + 1) / 2 + run_B(x, y, v);
+}
+
+/// S1 (src/S1.cpp:29-70): phi(x, c) plus the sum of mu(n) phi(x / n, c) over
+/// the square-free n <= y with least prime factor above p_c, from the lpf
+/// factor table of the S2_hard tables.
+i128 run_s1(i128 x, int64_t y, int64_t c, const EngineView& v)
+{
+  static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
+  if (c < 5 || c > 8)
+    throw pcg_error("S1(x, y, c): the device path needs 5 <= c <= 8");
+  const PhiTiny& t = phi_tiny_table((uint32_t) c);
+  const uint32_t* dtbl = phi_tiny_device(t, v);
+  const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) y) + 1;
+  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for S1");
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, true,
+                          nidx, (uint32_t) c, small[c - 1], dtbl, t.pp, t.phi_pp, A.d, v.stream));
+  acc_read(A, v.stream);
+  return phi_tiny_host(x, t) + acc_value(A.h);
+}
+
+i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
+{
+  const AcWork W = ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host);
+  Acc A;
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  char* dd = (char*) engine_buffer(4, W.bytes());
+  launch_ac(W, v, dd, A.d);
+  acc_read(A, v.stream);
+  return ac_value(W, A.h);
+}
+
+void check_dr(i128 x, int64_t y, int64_t z, int64_t c)
+{
+  if (x < 1 || y < 1 || z < 1)
+    throw pcg_error("Deleglise-Rivat terms need x, y, z >= 1");
+  if (c < 0 || c > 8)
+    throw pcg_error("c must be in [0, 8]");
+  if (x > ((i128) y << 62))
+    throw pcg_error("x is too large (x > y * 2^62)");
+  if ((uint64_t) y >= (1ull << 32) || (uint64_t) z >= (1ull << 50))
+    throw pcg_error("Deleglise-Rivat terms need y < 2^32 and z < 2^50");
+}
+
+/// Largest n the terms read pi(n) for: x / y (P2), sqrt(x), y.
+uint64_t dr_table_need(i128 x, int64_t y)
+{
+  u128 n = (u128) x / (u128) y;
+  n = std::max<u128>(n, isqrt((u128) x));
+  n = std::max<u128>(n, (u128) y);
+  if (n >= ((u128) 1 << 50))
+    throw pcg_error("pi(x): x / y must be < 2^50 (prime-count table limit)");
+  return (uint64_t) n + 1;
+}
+
+} // namespace
+
+i128 dr_term(DRTerm term, i128 x, int64_t y, int64_t z, int64_t c)
+{
+  check_dr(x, y, z, c);
+  auto lock = engine_lock();
+  const EngineView v =
+      engine_view(y, z, dr_table_need(x, y), LeafSum::S2_hard, (uint64_t) y + 1);
+  if ((uint64_t) y > v.host->max) throw pcg_error("internal: prime table below y");
+  switch (term) {
+    case DRTerm::S1: return run_s1(x, y, c, v);
+    case DRTerm::S2_trivial: return s2_trivial_host(x, y, z, c, *v.host);
+    case DRTerm::S2_easy: return run_s2_easy(x, y, z, c, v);
+    case DRTerm::P2: return run_p2(x, y, v);
+  }
+  return 0;
+}
+
+i128 pi_deleglise_rivat(i128 x)
+{
+  if (x < 2) return 0;
+  if (x <= (i128) 100000000) return pi_gourdon(x, nullptr);     // the table answers
+  const DRParams p = deleglise_rivat_params(x);
+  check_dr(x, p.y, p.z, p.c);
+  auto lock = engine_lock();
+  const EngineView v = engine_view(p.y, p.z, dr_table_need(x, p.y), LeafSum::S2_hard,
+                                     (uint64_t) p.y + 1);
+  if ((uint64_t) p.y > v.host->max) throw pcg_error("internal: prime table below y");
+  i128 r = run_s1(x, p.y, p.c, v);
+  r += s2_trivial_host(x, p.y, p.z, p.c, *v.host);
+  r += run_s2_easy(x, p.y, p.z, p.c, v);
+  r -= run_p2(x, p.y, v);
+  r += (i128) v.host->pi((uint64_t) p.y) - 1;
+  HslQuery q;
+  q.kind = LeafSum::S2_hard;
+  q.x = x;
+  q.y = p.y;
+  q.z = p.z;
+  q.k = p.c;
+  r += hsl_compute_batch_locked({q}, nullptr)[0];
+  return r;
 }
 
 std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,

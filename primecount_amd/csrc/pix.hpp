@@ -24,6 +24,16 @@ i128 gourdon_term(GourdonTerm t, i128 x, int64_t y, int64_t z, int64_t k, PixSta
 /// pi(x) with the reference's default parameters.  Throws pcg_error.
 i128 pi_gourdon(i128 x, PixStats* st);
 
+/// The Deleglise-Rivat terms other than S2_hard, on the device with the
+/// S2_hard tables (src/S1.cpp, src/deleglise-rivat/S2_trivial.cpp,
+/// S2_easy.cpp, src/P2.cpp).
+enum class DRTerm { S1, S2_trivial, S2_easy, P2 };
+i128 dr_term(DRTerm term, i128 x, int64_t y, int64_t z, int64_t c);
+
+/// pi(x) by the Deleglise-Rivat algorithm with the reference's parameters
+/// (src/deleglise-rivat/pi_deleglise_rivat.cpp:40-53, 66-92).
+i128 pi_deleglise_rivat(i128 x);
+
 /// pi(x) for many x: the terms other than D query by query over shared
 /// resident tables, D for all queries in one persistent launch.
 /// max_slots bounds the slot arrays of a D pass (0 = engine setting).

@@ -514,7 +514,11 @@ __global__ __launch_bounds__(kThreads) void pix_sigma_kernel(
 /// T_MAX = prime <= y (mu = -1), 0 = not square-free or a factor > y,
 /// otherwise 2 pi(lpf) + [mu = -1].  phi(v, k) = (v / pp) phi(pp) +
 /// phi_tbl[v mod pp] with pp the product of the first k primes.
-template <typename FT>
+/// With kLpf the entries are those of the lpf FactorTable (T_MAX = prime,
+/// 0 = not square-free, lpf for mu = -1, lpf - 1 for mu = +1) and the sum is
+/// the Deleglise-Rivat S1 over the square-free n <= limit with lpf > p_k
+/// (src/S1.cpp:29-70).
+template <typename FT, bool kLpf>
 __global__ __launch_bounds__(kThreads) void pix_phi0_kernel(
     uint64_t x_lo, uint64_t x_hi, const FT* __restrict__ factor, uint64_t nidx, uint32_t k,
     uint32_t pk, const uint32_t* __restrict__ phi_tbl, uint32_t pp, uint64_t phi_pp,
@@ -542,6 +546,11 @@ __global__ __launch_bounds__(kThreads) void pix_phi0_kernel(
       if (n <= pk)
         continue;
       mu = -1;
+    } else if (kLpf) {
+      const uint32_t lpf = (f & 1) ? (uint32_t) f : (uint32_t) f + 1;
+      if (lpf <= pk)
+        continue;
+      mu = (f & 1) ? -1 : 1;
     } else {
       if ((uint64_t) f <= 2ull * k + 1)
         continue;
@@ -809,17 +818,24 @@ hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t s
 }
 
 hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, bool factor32,
-                           uint64_t nidx, uint32_t k, uint32_t pk, const uint32_t* phi_tbl,
-                           uint32_t pp, uint64_t phi_pp, unsigned long long* acc, hipStream_t st)
+                           bool lpf_table, uint64_t nidx, uint32_t k, uint32_t pk,
+                           const uint32_t* phi_tbl, uint32_t pp, uint64_t phi_pp,
+                           unsigned long long* acc, hipStream_t st)
 {
   if (nidx <= 1) return hipSuccess;
   const uint32_t grid = grid_for(nidx, 16384);
-  if (factor32)
-    hipLaunchKernelGGL(pix_phi0_kernel<uint32_t>, dim3(grid), dim3(kThreads), 0, st, x_lo, x_hi,
-                       (const uint32_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  if (factor32 && lpf_table)
+    hipLaunchKernelGGL((pix_phi0_kernel<uint32_t, true>), dim3(grid), dim3(kThreads), 0, st, x_lo,
+                       x_hi, (const uint32_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  else if (factor32)
+    hipLaunchKernelGGL((pix_phi0_kernel<uint32_t, false>), dim3(grid), dim3(kThreads), 0, st, x_lo,
+                       x_hi, (const uint32_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+  else if (lpf_table)
+    hipLaunchKernelGGL((pix_phi0_kernel<uint16_t, true>), dim3(grid), dim3(kThreads), 0, st, x_lo,
+                       x_hi, (const uint16_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
   else
-    hipLaunchKernelGGL(pix_phi0_kernel<uint16_t>, dim3(grid), dim3(kThreads), 0, st, x_lo, x_hi,
-                       (const uint16_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
+    hipLaunchKernelGGL((pix_phi0_kernel<uint16_t, false>), dim3(grid), dim3(kThreads), 0, st, x_lo,
+                       x_hi, (const uint16_t*) factor, nidx, k, pk, phi_tbl, pp, phi_pp, acc);
   return hipGetLastError();
 }
 

@@ -82,9 +82,12 @@ hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx
 hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
                             const uint32_t* primes, uint32_t b_lo, uint32_t b_hi,
                             const PixWord* tab, unsigned long long* acc, hipStream_t st);
+/// lpf_table: the factor table has the lpf encoding (S1 of Deleglise-Rivat)
+/// instead of FactorTableD's (Phi0 of Gourdon).
 hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, bool factor32,
-                           uint64_t nidx, uint32_t k, uint32_t pk, const uint32_t* phi_tbl,
-                           uint32_t pp, uint64_t phi_pp, unsigned long long* acc, hipStream_t st);
+                           bool lpf_table, uint64_t nidx, uint32_t k, uint32_t pk,
+                           const uint32_t* phi_tbl, uint32_t pp, uint64_t phi_pp,
+                           unsigned long long* acc, hipStream_t st);
 hipError_t pix_launch_extract(const PixWord* tab, uint64_t p_max, uint32_t* out, hipStream_t st);
 hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const uint32_t* sp,
                                uint32_t n_small, uint64_t n_all, hipStream_t st);

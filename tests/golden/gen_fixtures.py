@@ -34,6 +34,7 @@ import os
 import re
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
@@ -252,6 +253,31 @@ def gen_nth():
               open(os.path.join(HERE, "nth_prime.json"), "w"), indent=1)
 
 
+def gen_dr():
+    """The Deleglise-Rivat terms printed by the reference build
+    (src/deleglise-rivat/pi_deleglise_rivat.cpp: pi(x) = S1 + S2_trivial +
+    S2_easy + S2_hard + pi(y) - 1 - P2) at 10^10 .. 10^19 and 8 seeded x."""
+    xs = [10 ** e for e in range(10, 20)] + batch_xs(0xD12, n=8, lo_exp=10.0, span=6.0)
+    rows = []
+    for x in xs:
+        r = {"x": str(x)}
+        for opt, key in (("--S1", "S1"), ("--S2-trivial", "S2_trivial"), ("--S2-easy", "S2_easy"),
+                         ("--S2-hard", "S2_hard"), ("--P2", "P2")):
+            v = run_cli([x, opt, "-s", "-t", "8"])
+            r[key] = v[key]
+            for k in ("y", "z", "c"):
+                if k in v:
+                    r[k] = v[k]
+        out = subprocess.run([CLI, str(x), "--deleglise-rivat", "-t", "8"], capture_output=True,
+                             text=True, timeout=3600, check=True).stdout.strip()
+        r["pi"] = out.splitlines()[-1].strip()
+        rows.append(r)
+        print(r, flush=True)
+    json.dump({"source": "oracle/_ref primecount X --S1/--S2-trivial/--S2-easy/--S2-hard/--P2 -s "
+                         "and --deleglise-rivat", "rows": rows},
+              open(os.path.join(HERE, "dr_terms.json"), "w"), indent=1)
+
+
 SERVE_SEED = 0xC5C5
 
 
@@ -275,9 +301,10 @@ def serve_queries(seed=SERVE_SEED, n=4096):
 def gen_serve(threads=7):
     """Answers of the 4096-query serving mix from the reference CLI
     (`primecount X` and `primecount N --nth-prime`).  Progress goes to a
-    JSON-lines file under _work/ so that an interrupted run resumes."""
+    JSON-lines file in the system's temporary directory so that an
+    interrupted run resumes."""
     qs = serve_queries()
-    part = os.path.join(REPO, "_work", "serve_c5.partial")
+    part = os.path.join(tempfile.gettempdir(), "primecount_serve_c5.partial")
     os.makedirs(os.path.dirname(part), exist_ok=True)
     done = {}
     if os.path.exists(part):
@@ -311,7 +338,7 @@ def main():
         sys.exit("build the reference first: make -f oracle/ref.mk -j8")
     steps = {"kat": gen_kat, "params": gen_params, "configs": lambda: gen_configs(a.big),
              "batch": gen_batch, "alpha": gen_alpha, "big": gen_big, "gourdon": gen_gourdon,
-             "nth": gen_nth, "serve": gen_serve}
+             "nth": gen_nth, "serve": gen_serve, "dr": gen_dr}
     for name, fn in steps.items():
         if (a.only and a.only != name) or (not a.only and name in ("big", "serve")):
             continue

@@ -1,4 +1,5 @@
-"""ctypes access to the CPU oracle (oracle/hsl_oracle.c, oracle/gourdon_oracle.c).
+"""ctypes access to the CPU oracle (oracle/hsl_oracle.c, oracle/gourdon_oracle.c,
+oracle/dr_oracle.c).
 
 Test infrastructure: the oracle is the checker the HIP engine is compared
 against; it is built into oracle/liboracle.so by build() here (or
@@ -11,7 +12,8 @@ import subprocess
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB = os.path.join(REPO, "oracle", "liboracle.so")
 SRCS = [os.path.join(REPO, "oracle", "hsl_oracle.c"),
-        os.path.join(REPO, "oracle", "gourdon_oracle.c")]
+        os.path.join(REPO, "oracle", "gourdon_oracle.c"),
+        os.path.join(REPO, "oracle", "dr_oracle.c")]
 SRC = SRCS[0]
 
 
@@ -31,6 +33,11 @@ class Oracle:
                                             ctypes.c_int64, ctypes.POINTER(ctypes.c_uint64),
                                             ctypes.POINTER(ctypes.c_int64)]
         lib.oracle_gourdon_term.restype = None
+        lib.oracle_dr_term.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_int64,
+                                       ctypes.c_int64, ctypes.c_int64, ctypes.c_int64,
+                                       ctypes.POINTER(ctypes.c_uint64),
+                                       ctypes.POINTER(ctypes.c_int64)]
+        lib.oracle_dr_term.restype = None
         lib.oracle_pi_sieve.argtypes = [ctypes.c_int64]
         lib.oracle_pi_sieve.restype = ctypes.c_int64
 
@@ -59,6 +66,16 @@ class Oracle:
         self.lib.oracle_gourdon_term(idx, u & ((1 << 64) - 1), ctypes.c_int64(u >> 64).value,
                                      int(y), int(z), int(k), int(x_star),
                                      ctypes.byref(lo), ctypes.byref(hi))
+        return (hi.value << 64) | lo.value
+
+    def dr_term(self, which, x, y, z, c):
+        """which: "S1", "S2_trivial", "S2_easy" or "P2" (plain sums over the
+        definitions of the Deleglise-Rivat terms)."""
+        idx = {"S1": 0, "S2_trivial": 1, "S2_easy": 2, "P2": 3}[which]
+        u = int(x) & ((1 << 128) - 1)
+        lo, hi = ctypes.c_uint64(), ctypes.c_int64()
+        self.lib.oracle_dr_term(idx, u & ((1 << 64) - 1), ctypes.c_int64(u >> 64).value,
+                                int(y), int(z), int(c), ctypes.byref(lo), ctypes.byref(hi))
         return (hi.value << 64) | lo.value
 
     def pi_sieve(self, n):

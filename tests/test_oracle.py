@@ -8,6 +8,7 @@
 """
 import random
 
+import numpy as np
 import pytest
 
 from tests import closed_form
@@ -119,6 +120,69 @@ def test_oracle_pi_known_values(oracle):
         total = sum(s * oracle.gourdon_term(t, x, y, z, k, xs)
                     for s, t in ((1, "AC"), (-1, "B"), (1, "Phi0"), (1, "Sigma")))
         assert total + oracle.D(x, y, z, k) == known[x]
+
+
+def test_oracle_dr_terms_match_reference(oracle):
+    """Plain sums over the definitions of S1, S2_trivial, S2_easy and P2
+    reproduce the values the reference build printed, and the printed terms
+    assemble pi(x) = S1 + S2_trivial + S2_easy + S2_hard + pi(y) - 1 - P2,
+    equal to the README's pi(10^n)."""
+    rows = golden("dr_terms.json")["rows"]
+    small = [r for r in rows if int(r["x"]) <= 10 ** 12]
+    assert len(small) >= 6
+    for r in small:
+        x, y, z, c = (int(r[t]) for t in ("x", "y", "z", "c"))
+        for t in ("S1", "S2_trivial", "S2_easy", "P2"):
+            assert oracle.dr_term(t, x, y, z, c) == int(r[t]), (x, t)
+    known = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    for r in rows:
+        x, y = int(r["x"]), int(r["y"])
+        if y > 3 * 10 ** 7:
+            continue
+        total = sum(int(r[t]) for t in ("S1", "S2_trivial", "S2_easy", "S2_hard")) \
+            + oracle.pi_sieve(y) - 1 - int(r["P2"])
+        assert total == int(r["pi"]), x
+        if x in known:
+            assert total == known[x], x
+
+
+def _phi_numpy(x, primes):
+    """phi(x, a) by marking the multiples of the first a primes."""
+    keep = np.ones(x + 1, dtype=bool)
+    keep[0] = False
+    for p in primes:
+        keep[p::p] = False
+    return int(keep.sum())
+
+
+def test_oracle_dr_identities_brute_force(oracle):
+    """At x = 2e7, for c = 5 .. 8 and two y: S1 + S2_trivial + S2_easy +
+    S2_hard = phi(x, pi(y)) by brute force (each c moves the boundaries of
+    S1, S2_trivial and S2_easy differently), and P2 = the number of
+    n = p q <= x with y < p <= q, counted directly."""
+    x = 2 * 10 ** 7
+    sieve = np.ones(x + 1, dtype=bool)
+    sieve[:2] = False
+    for i in range(2, int(x ** 0.5) + 1):
+        if sieve[i]:
+            sieve[i * i::i] = False
+    primes = np.nonzero(sieve)[0]
+    pi = np.cumsum(sieve)
+    for y in (272, 1000):
+        z = x // y
+        a = int(pi[y])
+        phi = _phi_numpy(x, primes[:a])
+        for c in (5, 6, 7, 8):
+            t = {n: oracle.dr_term(n, x, y, z, c) for n in ("S1", "S2_trivial", "S2_easy")}
+            assert sum(t.values()) + oracle.S2_hard(x, y, z, c) == phi, (y, c)
+        big = primes[primes > y]
+        p2 = sum(int(pi[x // p]) - int(pi[p]) + 1 for p in big[big * big <= x])
+        semi = 0
+        for p in big[big * big <= x]:
+            semi += int(np.count_nonzero((big >= p) & (big <= x // p)))
+        assert p2 == semi
+        assert oracle.dr_term("P2", x, y, z, 8) == semi, y
+        assert phi + a - 1 - semi == int(pi[x])
 
 
 def _is_prime64(n):
