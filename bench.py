@@ -11,7 +11,8 @@ path: planning, the sieve kernel, the carry combine and the read-back.
 Prints ONE JSON line.  `roofline` is measured on the sieve kernel with HIP
 events on the engine's stream; `cpu_baseline` times the reference build
 (oracle/_ref/primecount, when present) on this host; `pi` reports the full
-pi(x) (all Gourdon terms on the device) outside the timed D steps.
+pi(x) (all Gourdon terms on the device, and beside it the Deleglise-Rivat
+formula, every term on the device) outside the timed D steps.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--x 1e19] [--no-cpu]
 """
@@ -76,10 +77,20 @@ def measure_pi(pc, x):
         times.append(time.perf_counter() - t)
     st = pc.last_pi_stats()
     times.sort()
+    # the Deleglise-Rivat formula on the same device (S1 + S2 + pi(y) - 1 - P2)
+    dr = pc.pi_deleglise_rivat(x)
+    dr_times = []
+    for _ in range(3):
+        t = time.perf_counter()
+        dr = pc.pi_deleglise_rivat(x)
+        dr_times.append(time.perf_counter() - t)
+    dr_times.sort()
     return {"x": str(x), "value": str(got), "bit_exact": known.get(x) == got,
             "seconds": round(times[1], 6),
             "terms_s": {k: round(st["seconds_" + k], 6) for k in ("sigma", "phi0", "ac", "b", "d")},
-            "ac_leaves": st["ac_leaves"], "table_limit": st["table_limit"]}
+            "ac_leaves": st["ac_leaves"], "table_limit": st["table_limit"],
+            "deleglise_rivat": {"value": str(dr), "bit_exact": known.get(x) == dr,
+                                "seconds": round(dr_times[1], 6)}}
 
 
 def measure_serving(pc, n, rate):

@@ -793,7 +793,8 @@ namespace {
 ///   sum_{A < q <= B} pi(xp / q) = pi(xp / B) (pi(B) - pi(A))
 ///       + sum_{xp / B < r <= xp / (A + 1)} (pi(xp / r) - pi(A)),
 /// one leaf pi(xp / r) per prime r below sqrt(x / p_b) instead of one per q
-/// (at x = 1e19: 3.2e10 leaves instead of 2.0e11).  The -b + 2 parts and the
+/// (at x = 1e19: 3.2e10 leaves instead of 2.0e11); the reflection that
+/// src/gourdon/AC.cpp:271-320 applies to the clustered leaves of C2.  The -b + 2 parts and the
 /// pi(A), pi(B) products go to the correction.
 LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P)
 {
