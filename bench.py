@@ -163,7 +163,12 @@ def cpu_baseline(x, timeout):
     cli = os.path.join(REPO, "oracle", "_ref", "primecount")
     if not os.path.exists(cli):
         return None
-    cores = os.cpu_count() or 1
+    # this process's CPU share, at most 16 threads (one GPU's share of a host)
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
     try:
         r = subprocess.run([cli, str(x), "--D", "-s", "-t", str(cores)], capture_output=True,
                            text=True, timeout=timeout)

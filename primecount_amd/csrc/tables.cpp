@@ -8,6 +8,7 @@
 #include <cstring>
 #include <atomic>
 #include <array>
+#include <sched.h>
 #include <thread>
 
 namespace pcg {
@@ -16,8 +17,15 @@ const uint8_t kWheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
 
 int default_threads()
 {
-  unsigned n = std::thread::hardware_concurrency();
-  return n ? (int) std::min(n, 64u) : 1;
+  // the CPUs this process may run on (not the whole host), at most 16: the
+  // host's share per GPU
+  cpu_set_t set;
+  unsigned n = 0;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0)
+    n = (unsigned) CPU_COUNT(&set);
+  if (n == 0)
+    n = std::thread::hardware_concurrency();
+  return n ? (int) std::min(n, 16u) : 1;
 }
 
 namespace {
