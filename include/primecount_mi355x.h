@@ -180,7 +180,7 @@ int pcg_release(void);
 int pcg_pi(pcg_i128 x, pcg_i128* out);
 
 /* The other Gourdon terms with explicit parameters, as in
- * include/gourdon.hpp:26-29 (64-bit) and 36-39 (128-bit): A + C
+ * include/gourdon.hpp:24-27 (64-bit) and 34-37 (128-bit): A + C
  * (src/gourdon/AC.cpp), B (src/gourdon/B.cpp), Phi0 (src/gourdon/Phi0.cpp,
  * device path for 5 <= k <= 8) and Sigma (src/gourdon/Sigma.cpp). */
 int pcg_AC(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out);

@@ -18,7 +18,7 @@ struct PixStats {
 };
 
 /// One term with explicit parameters (src/gourdon/AC.cpp, B.cpp, Phi0.cpp,
-/// Sigma.cpp signatures of include/gourdon.hpp:26-29,36-39).  Throws pcg_error.
+/// Sigma.cpp signatures of include/gourdon.hpp:24-27,34-37).  Throws pcg_error.
 i128 gourdon_term(GourdonTerm t, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st);
 
 /// pi(x) with the reference's default parameters.  Throws pcg_error.

@@ -64,7 +64,20 @@ def test_library_exports_every_declared_symbol(pc):
 def test_cpp_header_compiles(tmp_path):
     src = tmp_path / "use.cpp"
     src.write_text('#include "primecount_mi355x.hpp"\n'
-                   'int main() { return (int) primecount_mi355x::D((int64_t) 1000, 10, 10, 2); }\n')
+                   'namespace pm = primecount_mi355x;\n'
+                   'int main() {\n'
+                   '  int64_t x = 1000; pm::int128_t X = 1000;\n'
+                   '  int64_t s = pm::D(x, 10, 10, 2) + pm::S2_hard(x, 10, 100, 2) + pm::pi(x)\n'
+                   '    + pm::nth_prime(x) + pm::AC(x, 10, 10, 2) + pm::B(x, 10)\n'
+                   '    + pm::Phi0(x, 10, 10, 2) + pm::Sigma(x, 10) + pm::S1(x, 10, 5)\n'
+                   '    + pm::S2_trivial(x, 10, 100, 5) + pm::S2_easy(x, 10, 100, 5)\n'
+                   '    + pm::P2(x, 10, 4) + pm::pi_deleglise_rivat(x, 1);\n'
+                   '  pm::int128_t t = pm::D(X, 10, 10, 2) + pm::pi(X) + pm::AC(X, 10, 10, 2)\n'
+                   '    + pm::B(X, 10) + pm::Phi0(X, 10, 10, 2) + pm::Sigma(X, 10)\n'
+                   '    + pm::S1(X, 10, 5) + pm::S2_trivial(X, 10, 100, 5) + pm::S2_easy(X, 10, 100, 5)\n'
+                   '    + pm::P2(X, 10, 4) + pm::pi_deleglise_rivat(X) + pm::S2_hard(X, 10, 100, 2);\n'
+                   '  return (int) (s + (int64_t) t);\n'
+                   '}\n')
     subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(REPO, "include"),
                     str(src)], check=True)
 
