@@ -86,3 +86,18 @@ def test_dr_term_errors(pc):
         pc.S2_easy(10 ** 12, 20000, 10 ** 12 // 20000, 9)
     with pytest.raises(pc.PrimecountError, match="too large"):
         pc.S2_trivial(2 ** 80, 1000, 2 ** 60, 8)
+
+
+def test_pi_128bit_x_both_formulas(pc):
+    """x above 2^64: the 128-bit paths of every term of both formulas.
+    pi(1e20) is the README value; at seeded x in (2^64, 4e19) the two
+    algorithms (different leaf sets, different tables) must agree."""
+    want = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}[10 ** 20]
+    log("pi(1e20), both formulas")
+    assert pc.pi_deleglise_rivat(10 ** 20) == want
+    assert pc.pi(10 ** 20) == want
+    rng = random.Random(0x128)
+    for _ in range(2):
+        x = rng.randrange(2 ** 64 + 1, 4 * 10 ** 19)
+        log("pi(%d): Gourdon vs Deleglise-Rivat" % x)
+        assert pc.pi_deleglise_rivat(x) == pc.pi(x), x
