@@ -309,7 +309,7 @@ def main():
         "end_to_end_s_first_call": round(first["seconds_total"], 3),
         "work": {k: st[k] for k in ("leaves", "hits", "factor_entries", "pairs", "windows",
                                     "leaf_primes", "leaf_steps", "dense_primes", "dense_steps",
-                                    "group_steps", "prefix_builds")},
+                                    "group_steps", "prefix_builds", "queue_steps")},
         "roofline": roofline,
         "roofline_lds": roofline_lds,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),

@@ -159,6 +159,8 @@ typedef struct pcg_stats {
   uint64_t group_steps;      /* grouped cross-off steps                       */
   uint64_t prefix_builds;    /* word-pair prefix rebuilds (a prime with leaves
                                 after first-time removals)                    */
+  uint64_t queue_steps;      /* steps counting queued factor-table leaves
+                                (up to 64 leaves each)                        */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

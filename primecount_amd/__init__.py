@@ -63,6 +63,7 @@ class Stats(ctypes.Structure):
         ("dense_steps", ctypes.c_uint64),
         ("group_steps", ctypes.c_uint64),
         ("prefix_builds", ctypes.c_uint64),
+        ("queue_steps", ctypes.c_uint64),
     ]
 
     def as_dict(self):

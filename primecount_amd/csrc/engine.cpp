@@ -788,7 +788,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
 
   std::vector<i128> col_out(ncols_total);
   std::vector<uint64_t> sacc(2 * nq);
-  uint64_t counters[11] = {};
+  uint64_t counters[12] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(sacc.data(), D.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, D.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
@@ -834,6 +834,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->dense_steps = counters[8];
     st->group_steps = counters[9];
     st->prefix_builds = counters[10];
+    st->queue_steps = counters[11];
   }
   return result;
 }
@@ -901,6 +902,7 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.dense_steps += st.dense_steps;
     acc.group_steps += st.group_steps;
     acc.prefix_builds += st.prefix_builds;
+    acc.queue_steps += st.queue_steps;
   }
   acc.seconds_total = now() - t0;
   if (stats) *stats = acc;
