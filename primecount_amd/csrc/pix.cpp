@@ -1,4 +1,6 @@
-// pix.cpp -- host orchestration of Gourdon's pi(x) on the device.
+// pix.cpp -- host orchestration of Gourdon's pi(x) on the device, of the
+// Deleglise-Rivat terms S1, S2_trivial, S2_easy and P2 (with S2_hard, the
+// Deleglise-Rivat pi(x)) and of nth_prime.
 //
 // The easy-leaf sums of A and C (src/gourdon/AC.cpp) are cut into one
 // descriptor per sieving prime (or per pair of primes for the m = q r leaves
@@ -331,14 +333,16 @@ void ac_lists_part(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   }
 }
 
-/// Descriptor lists of A and C, built by up to 16 host threads for large x.
-LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P)
+/// Runs part(t, T, lists) for t = 0 .. T - 1 on T host threads (up to 16;
+/// one when nb, the number of sieving primes, is small) and concatenates the
+/// parts' lists: the order of the terms does not matter to the sums.
+template <typename Part>
+LeafLists threaded_lists(uint64_t nb, Part&& part)
 {
-  const uint64_t nb = P.pi(iroot<3>((u128) x));
   const uint64_t T = nb < 4096 ? 1 : (uint64_t) std::min(16, default_threads());
   std::vector<LeafLists> parts(T);
   if (T == 1) {
-    ac_lists_part(x, y, z, k, x_star, P, 0, 1, parts[0]);
+    part(0, 1, parts[0]);
     return std::move(parts[0]);
   }
   std::vector<std::exception_ptr> err(T);
@@ -346,7 +350,7 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   for (uint64_t t = 0; t < T; t++)
     th.emplace_back([&, t] {
       try {
-        ac_lists_part(x, y, z, k, x_star, P, t, T, parts[t]);
+        part(t, T, parts[t]);
       } catch (...) {
         err[t] = std::current_exception();
       }
@@ -376,6 +380,14 @@ LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
     L.correction += q.correction;
   }
   return L;
+}
+
+/// Descriptor lists of A and C, built by up to 16 host threads for large x.
+LeafLists ac_lists(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const PrimeTables& P)
+{
+  return threaded_lists(P.pi(iroot<3>((u128) x)), [&](uint64_t t, uint64_t T, LeafLists& out) {
+    ac_lists_part(x, y, z, k, x_star, P, t, T, out);
+  });
 }
 
 /// The A/C leaves as the device takes them: the plus list as tile
@@ -796,9 +808,9 @@ namespace {
 /// (at x = 1e19: 3.2e10 leaves instead of 2.0e11); the reflection that
 /// src/gourdon/AC.cpp:271-320 applies to the clustered leaves of C2.  The -b + 2 parts and the
 /// pi(A), pi(B) products go to the correction.
-LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P)
+void s2_easy_part(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P, uint64_t t,
+                  uint64_t T, LeafLists& L)
 {
-  LeafLists L;
   const u128 ux = (u128) x;
   const uint64_t uy = (uint64_t) y, uz = (uint64_t) z;
   auto pi = [&](uint64_t n) -> uint64_t {
@@ -807,7 +819,7 @@ LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTabl
   };
   const uint64_t pi_sqrty = pi(isqrt((u128) uy));
   const uint64_t pi_x13 = pi(iroot<3>(ux));
-  for (uint64_t b = umax((uint64_t) c, pi_sqrty) + 1; b <= pi_x13; b++) {
+  for (uint64_t b = umax((uint64_t) c, pi_sqrty) + 1 + t; b <= pi_x13; b += T) {
     const uint64_t prime = P.primes[b];
     const u128 xp128 = ux / prime;
     if (xp128 >> 64) throw pcg_error("S2_easy: x / p does not fit in 64 bits");
@@ -835,7 +847,13 @@ LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTabl
       L.correction += (i128) (i - pi_min_sparse) * (2 - (i128) b);
     }
   }
-  return L;
+}
+
+LeafLists s2_easy_lists(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P)
+{
+  return threaded_lists(P.pi(iroot<3>((u128) x)), [&](uint64_t t, uint64_t T, LeafLists& out) {
+    s2_easy_part(x, y, z, c, P, t, T, out);
+  });
 }
 
 /// S2_trivial (src/deleglise-rivat/S2_trivial.cpp:30-77), on the host: for
