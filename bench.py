@@ -9,7 +9,10 @@ value on every step.  The prime, pi and factor tables are built on the first
 path: planning, the sieve kernel, the carry combine and the read-back.
 
 Prints ONE JSON line.  `roofline` is measured on the sieve kernel with HIP
-events on the engine's stream; `cpu_baseline` times the reference build
+events on the engine's stream (HBM); `roofline_lds` and `roofline_valu` put
+the same kernel time against LDS bandwidth and against VALU issue, the bound
+that holds (instructions per launch from the committed SQ-counter profile);
+`cpu_baseline` times the reference build
 (oracle/_ref/primecount, when present) on this host; `pi` reports the full
 pi(x) (all Gourdon terms on the device, and beside it the Deleglise-Rivat
 formula, every term on the device) outside the timed D steps.
@@ -137,6 +140,20 @@ def traffic_from_profile(x):
     with open(path) as f:
         for line in f:
             m = re.match(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", line.strip())
+            if m:
+                return float(m.group(1))
+    return None
+
+
+def valu_from_profile(x):
+    """Wave64 VALU instructions per sieve-kernel launch from the committed
+    rocprofv3 --pmc summary of the SQ counters (SQ_INSTS_VALU)."""
+    path = os.path.join(REPO, "profiles", "hsl_sieve_kernel_D_1e%d_sq.md" % round(math.log10(x)))
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            m = re.match(r"^valu_instructions_per_launch:\s*([0-9.eE+]+)", line.strip())
             if m:
                 return float(m.group(1))
     return None
@@ -283,6 +300,19 @@ def main():
         "frac": round(lds_bytes / k_avg / 1e9 / lds_peak_used, 6),
         "bytes_per_launch": lds_bytes,
     }
+    # the bound that holds: VALU issue.  A wave64 VALU instruction takes a
+    # SIMD16 4 cycles; MI355X has 1,024 SIMDs at up to 2.4 GHz.
+    valu = valu_from_profile(x)
+    valu_peak = 1024 * 2.4e9 / 4          # wave instructions per second
+    roofline_valu = None if valu is None else {
+        "bound": "valu",
+        "achieved": round(valu / k_avg / 1e12, 4),
+        "peak": round(valu_peak / 1e12, 4),
+        "unit": "T wave64 VALU instructions/s",
+        "frac": round(valu / k_avg / valu_peak, 4),
+        "instructions_per_launch": valu,
+        "source": "SQ_INSTS_VALU of the committed rocprofv3 summary, over this run's kernel time",
+    }
     out = {
         "metric": "D(x) secs at x=1e19",
         "value": round(sec_per_step, 6),
@@ -312,6 +342,7 @@ def main():
                                     "group_steps", "prefix_builds", "queue_steps")},
         "roofline": roofline,
         "roofline_lds": roofline_lds,
+        "roofline_valu": roofline_valu,
         "cpu_baseline": None if a.no_cpu else cpu_baseline(x, a.cpu_timeout),
     }
     if not a.no_pi:
