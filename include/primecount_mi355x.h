@@ -178,7 +178,8 @@ int pcg_release(void);
  * engine, the others over a prime-count table resident in HBM).  Replaces
  * pi(x) / primecount_pi (include/primecount.hpp:57,74, include/primecount.h:
  * 51, 69 -> src/gourdon/pi_gourdon.cpp:22-155).  x <= 10^8 is answered from
- * the prime-count table directly.  Domain: x / y < 2^50. */
+ * the prime-count table directly.  Domain: x / y < 2^49 (the prime-count
+ * table covers [0, x / y] in HBM: 7.5 GB at x = 1e19, 98 GB at 1e21). */
 int pcg_pi(pcg_i128 x, pcg_i128* out);
 
 /* The other Gourdon terms with explicit parameters, as in
@@ -203,7 +204,7 @@ int pcg_pi_deleglise_rivat(pcg_i128 x, pcg_i128* out);
  * S2_easy (include/S.hpp:22-24, 29-31; src/S1.cpp,
  * src/deleglise-rivat/S2_trivial.cpp, S2_easy.cpp; S1 needs 5 <= c <= 8) and
  * P2 (include/primecount-internal.hpp:43, 61; src/P2.cpp; pi(y) is computed
- * here).  Domain: x <= y * 2^62, y < 2^32, z < 2^50, x / y < 2^50. */
+ * here).  Domain: x <= y * 2^62, y < 2^32, z < 2^50, x / y < 2^49. */
 int pcg_S1(pcg_i128 x, int64_t y, int64_t c, pcg_i128* out);
 int pcg_S2_trivial(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);
 int pcg_S2_easy(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);

@@ -54,8 +54,8 @@ uint64_t table_need(i128 x, int64_t y, int64_t x_star)
   n = std::max<u128>(n, isqrt((u128) x));
   n = std::max<u128>(n, (u128) x / ((u128) x_star * (u128) y));
   n = std::max<u128>(n, isqrt((u128) x / (u128) x_star));
-  if (n >= ((u128) 1 << 50))
-    throw pcg_error("pi(x): x / y must be < 2^50 (prime-count table limit)");
+  if (n >= ((u128) 1 << 49))
+    throw pcg_error("pi(x): x / y must be < 2^49 (prime-count table limit)");
   return (uint64_t) n + 1;
 }
 
@@ -410,7 +410,8 @@ struct AcWork {
   }
 };
 
-AcWork ac_work(LeafLists&& L, const PrimeTables& P)
+/// max_quotient bounds every leaf quotient (the prime-count table limit).
+AcWork ac_work(LeafLists&& L, const PrimeTables& P, uint64_t max_quotient)
 {
   AcWork W;
   const size_t n = L.plus.size();
@@ -423,8 +424,9 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P)
     const uint64_t cnt = (i + 1 < n ? L.plus[i + 1].off : L.n_plus) - L.plus[i].off;
     top = std::max<uint64_t>(top, P.primes[L.plus[i].i_lo + cnt - 1]);
   }
-  if (top >= kPixTileMaxPrime) {
-    // the tiled kernel's 32-bit residual needs primes below 2^30
+  if (top >= kPixTileMaxPrime || max_quotient >= kPixTileMaxQuotient) {
+    // the tiled kernel's 32-bit residual needs primes below 2^30, its
+    // quotient guard quotients below 2^41; the flat kernel is exact below 2^49
     W.plus = std::move(L.plus);
     W.n_plus = L.n_plus;
     return W;
@@ -514,7 +516,7 @@ i128 ac_value(const AcWork& W, const uint64_t* h)
 i128 run_ac(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, const EngineView& v,
             uint64_t* leaves)
 {
-  const AcWork W = ac_work(ac_lists(x, y, z, k, x_star, *v.host), *v.host);
+  const AcWork W = ac_work(ac_lists(x, y, z, k, x_star, *v.host), *v.host, v.pix_n);
   *leaves = W.leaves;
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
@@ -670,7 +672,8 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
   size_t bytes = 0;
   for (size_t j = 0; j < n; j++) {
     const GourdonParams& gp = gps[idx[j]];
-    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host);
+    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host,
+                   v.pix_n);
     bytes += L[j].bytes();
     s.ac_leaves += L[j].leaves;
   }
@@ -925,7 +928,7 @@ i128 run_s1(i128 x, int64_t y, int64_t c, const EngineView& v)
 
 i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
 {
-  const AcWork W = ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host);
+  const AcWork W = ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host, v.pix_n);
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
@@ -953,8 +956,8 @@ uint64_t dr_table_need(i128 x, int64_t y)
   u128 n = (u128) x / (u128) y;
   n = std::max<u128>(n, isqrt((u128) x));
   n = std::max<u128>(n, (u128) y);
-  if (n >= ((u128) 1 << 50))
-    throw pcg_error("pi(x): x / y must be < 2^50 (prime-count table limit)");
+  if (n >= ((u128) 1 << 49))
+    throw pcg_error("pi(x): x / y must be < 2^49 (prime-count table limit)");
   return (uint64_t) n + 1;
 }
 

@@ -786,7 +786,7 @@ hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp
 {
   if (nitems == 0 || ngrp == 0) return hipSuccess;
   // the quotient guard v 2^-46 stays far below 1/2 for quotients below 2^41
-  if (max_quotient >= (1ull << 41)) return hipErrorInvalidValue;
+  if (max_quotient >= kPixTileMaxQuotient) return hipErrorInvalidValue;
   // contiguous item ranges, up to 64 waves per CU
   const uint64_t max_waves = (uint64_t) cus * 64;
   const uint64_t per_wave = (nitems + max_waves - 1) / max_waves;

@@ -59,6 +59,7 @@ static_assert(sizeof(PixTileDesc) == 32 && sizeof(PixTileGroup) == 32, "32-byte 
 
 constexpr uint32_t kPixTileGroup = 8;
 constexpr uint32_t kPixTileMaxPrime = 1u << 30;
+constexpr uint64_t kPixTileMaxQuotient = 1ull << 41;
 constexpr uint32_t kPixTileChunk = 16;
 
 constexpr uint32_t kPixSegWords = 4096;       // words per sieve segment (983,040 numbers)
