@@ -232,7 +232,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: the hard-special-leaf engine runs on a single MI355X "
-              "(multi-GPU sharding is described in DESIGN.md, not built); use --gpus 1",
+              "(sharding by segment range across GPUs is a design, not built); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
 

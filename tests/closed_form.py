@@ -9,7 +9,7 @@ phi(n, a) by Legendre's recursion.  Tiny inputs only.
   with m coprime to 2*3*5*7*11, z/p_b < m <= min(x/p_b^3, z), mu(m) != 0,
   lpf(m) > p_b, mpf(m) <= y;  q prime, p_b < q <= min(x/p_b^3, y);
   every leaf value n = x/(p_b m) lies below the sieve limit x/z.
-  (limits as src/gourdon/D.cpp:122-133, 279-290.)
+  (Limits as src/gourdon/D.cpp:122-133, 279-290.)
 
   S2_hard(x,y,z,c) = sum_{b=c+1}^{pi(sqrt y)} sum_{m} -mu(m) phi(x/(p_b m), b-1)
                    + sum_{b=pi(sqrt y)+1}^{pi(min(sqrt z, y))} sum_{q} phi(x/(p_b q), b-1)
