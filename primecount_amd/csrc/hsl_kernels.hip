@@ -81,10 +81,14 @@ __device__ __forceinline__ uint32_t leaf_stop(uint64_t xlo, uint64_t low, uint32
   return t < 0 ? s - 1 : ((uint64_t) t >= m ? s + 1 : s);
 }
 
-/// min(floor(xp / d), cap) for cap < 2^48.
-__device__ __forceinline__ uint64_t div_cap(double xd, uint64_t xlo, uint64_t d, uint64_t cap)
+/// min(floor(xp / d), cap) for cap < 2^48, with inv_d = 1 / d rounded to
+/// double.  xd * inv_d carries a relative error below 3 * 2^-53, so below the
+/// cap (quotient < 2^48 + 2) it is within 0.1 of xp / d and the residual
+/// fixes the last unit; above it the comparison with cap + 2 is safe.
+__device__ __forceinline__ uint64_t div_cap(double xd, uint64_t xlo, uint64_t d, double inv_d,
+                                            uint64_t cap)
 {
-  double qd = xd / (double) d;
+  double qd = xd * inv_d;
   if (qd >= (double) cap + 2.0)
     return cap;
   uint64_t q = (uint64_t) qd;
@@ -374,6 +378,10 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     const uint64_t high = min(low + SEG, qd.limit);
     const uint32_t seg_len = (uint32_t) (high - low);
     const uint64_t low1 = low > 0 ? low : 1;
+    // reciprocals of the segment bounds for the leaf-range quotients
+    const double inv_low1 = 1.0 / (double) low1;
+    const double inv_high = 1.0 / (double) high;
+    const double low_d = (double) low, last_d = (double) (high - 1);   // exact: < 2^53
 
     // numbers coprime to 30 in [low, high)
     for (uint32_t w = lane; w < W; w += kBlock) {
@@ -429,8 +437,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         for (int k = 0; k < NQ; k++) {
           if (bb < stop_b[k]) {
             const BInfo bi = binfo[k][bb - qd.b_first];
-            const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, bi.A);
-            uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, bi.C);
+            const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, inv_low1, bi.A);
+            uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, inv_high, bi.C);
             m_lo = m_lo > bi.B ? m_lo : bi.B;
             if (bi.type == 1) {
               brk[k] = bi.p >= m_hi;
@@ -450,8 +458,14 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         }
         r_p = p;
         // cofactors q of p with p*q in (low, high): q_lo .. q_lo + nq - 1
-        const uint64_t q_lo = div_exact((double) low, low, p) + 1;
-        const uint64_t q_hi = div_exact((double) (high - 1), high - 1, p);
+        // 1 / p from the hardware reciprocal and two Newton steps (relative
+        // error near 2^-52); the quotients are below 2^48, so the products
+        // are within 0.2 of low / p and (high - 1) / p
+        double inv_p = __builtin_amdgcn_rcp((double) p);
+        inv_p = fma(inv_p, fma(-(double) p, inv_p, 1.0), inv_p);
+        inv_p = fma(inv_p, fma(-(double) p, inv_p, 1.0), inv_p);
+        const uint64_t q_lo = div_exact_r(low_d, low, p, inv_p) + 1;
+        const uint64_t q_hi = div_exact_r(last_d, high - 1, p, inv_p);
         r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
         r_qm = (uint32_t) (q_lo % 30);
         r_off0 = (uint32_t) (p * q_lo - low);

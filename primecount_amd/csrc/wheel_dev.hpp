@@ -71,4 +71,18 @@ __device__ __forceinline__ uint64_t div_exact(double xd, uint64_t xlo, uint64_t 
   return q;
 }
 
+/// floor(n / d) for n < 2^53 (nd = n, exact) and a quotient below 2^48,
+/// with inv_d within a relative 2^-51 of 1 / d: the product is within 0.25
+/// of n / d and the residual, exact because |n - q d| < 2 d, fixes it.
+__device__ __forceinline__ uint64_t div_exact_r(double nd, uint64_t n, uint64_t d, double inv_d)
+{
+  uint64_t q = (uint64_t) (nd * inv_d);
+  int64_t r = (int64_t) (n - q * d);
+  if (r < 0)
+    q--;
+  else if ((uint64_t) r >= d)
+    q++;
+  return q;
+}
+
 } // namespace pcg
