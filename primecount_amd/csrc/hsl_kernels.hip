@@ -101,21 +101,24 @@ __device__ __forceinline__ uint64_t div_cap(double xd, uint64_t xlo, uint64_t d,
 }
 
 /// to_index of src/BaseFactorTable.hpp:36-42: index of the largest number
-/// coprime to 2310 that is <= n (n >= 1).
-__device__ __forceinline__ uint64_t ft_to_index(uint64_t n)
+/// coprime to 2310 that is <= n (n >= 1), for n < 2^32 -- every leaf-range
+/// bound is (m <= z < 2^32 for D, m <= y < 2^32 for S2_hard) -- with 32-bit
+/// arithmetic.
+__device__ __forceinline__ uint32_t ft_to_index32(uint32_t n)
 {
-  uint64_t q = n / 2310;
-  return 480 * q + (int64_t) c_cop_index[n - q * 2310];
+  const uint32_t q = n / 2310;
+  return 480 * q + (uint32_t) c_cop_index[n - q * 2310];
 }
 
-/// pi(n) for n within the prime table.
-__device__ __forceinline__ uint64_t pi_lookup(const uint64_t* __restrict__ bits,
-                                              const uint32_t* __restrict__ cnt, uint64_t n)
+/// pi(n) for n < 2^32 within the prime table (src/PiTable.hpp:46-57), with
+/// 32-bit index arithmetic.
+__device__ __forceinline__ uint64_t pi_lookup32(const uint64_t* __restrict__ bits,
+                                                const uint32_t* __restrict__ cnt, uint32_t n)
 {
   if (n < 7)
     return n < 2 ? 0 : (n < 3 ? 1 : (n < 5 ? 2 : 3));
-  uint64_t blk = n / 240;
-  uint32_t r = (uint32_t) (n - blk * 240);
+  const uint32_t blk = n / 240;
+  const uint32_t r = n - blk * 240;
   return 3 + (uint64_t) cnt[blk] + (uint64_t) __popcll(bits[blk] & le_mask(r));
 }
 
@@ -440,17 +443,19 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             const uint64_t m_hi = div_cap(bi.xp_d, bi.xp_lo, low1, inv_low1, bi.A);
             uint64_t m_lo = div_cap(bi.xp_d, bi.xp_lo, high, inv_high, bi.C);
             m_lo = m_lo > bi.B ? m_lo : bi.B;
+            // m_lo, m_hi <= max(A, B, C) <= z (D) or y: below 2^32
             if (bi.type == 1) {
               brk[k] = bi.p >= m_hi;
               if (!brk[k]) {
-                r_hi[k] = (uint32_t) ft_to_index(m_hi);
-                r_lo[k] = m_lo < m_hi ? (uint32_t) ft_to_index(m_lo) : r_hi[k];
+                r_hi[k] = ft_to_index32((uint32_t) m_hi);
+                r_lo[k] = m_lo < m_hi ? ft_to_index32((uint32_t) m_lo) : r_hi[k];
               }
             } else {
-              r_hi[k] = (uint32_t) pi_lookup(pi_bits, pi_cnt, m_hi);
+              r_hi[k] = (uint32_t) pi_lookup32(pi_bits, pi_cnt, (uint32_t) m_hi);
               brk[k] = r_hi[k] <= bb;
               if (!brk[k])
-                r_lo[k] = m_lo < m_hi ? (uint32_t) pi_lookup(pi_bits, pi_cnt, m_lo) : r_hi[k];
+                r_lo[k] = m_lo < m_hi ? (uint32_t) pi_lookup32(pi_bits, pi_cnt, (uint32_t) m_lo)
+                                      : r_hi[k];
             }
             r_T[k] = bi.T | (bi.type == 1 ? 0x80000000u : 0u);
             r_xlo[k] = bi.xp_lo;
@@ -613,7 +618,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             if (!ordered && delta < pp.z) {             // q = q_lo + delta < p
               const uint32_t q = pp.x - pp.z + delta;
               if (q > 1)
-                attr = (uint32_t) pi_lookup(pi_bits, pi_cnt, q) - b;
+                attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
             }
             atomicAdd(&w_rem[attr], 1u);
           }
