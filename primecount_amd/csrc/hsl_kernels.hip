@@ -315,16 +315,16 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
-  uint16_t* cop = reinterpret_cast<uint16_t*>(pre2 + W / 2);  // 480 u16
-  uint8_t* lut = reinterpret_cast<uint8_t*>(cop + 480);       // 240 u8
+  uint8_t* lut = reinterpret_cast<uint8_t*>(pre2 + W / 2);    // 240 u8
+  // LDS per wave bounds the waves per CU, and the kernel runs faster with
+  // more of them, so what fits elsewhere lives elsewhere: the 480 numbers
+  // coprime to 2310 are read from constant memory (c_cop) and the leaf signs
+  // of the window's primes stay in registers (lane j holds prime j's).
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
-  __shared__ int w_m[NQ][64];                                 // per member and prime: leaf signs
   __shared__ uint4 wp[64];                                    // per window prime: cross-off data
   __shared__ uint32_t lq[128];                                // leaf queue: factor index | sign << 31
 
   const uint32_t lane = threadIdx.x;
-  for (uint32_t i = lane; i < 480; i += kBlock)
-    cop[i] = c_cop[i];
   for (uint32_t r = lane; r < 240; r += kBlock)
     lut[r] = (uint8_t) (8 * (r / 30) + le_count(r % 30));
 
@@ -530,9 +530,10 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       prm.w = r_h;
       wp[lane] = prm;
       w_rem[lane] = 0;
+      int w_m[NQ];                     // lane j: the leaf-sign sum M of prime j, per member
 #pragma unroll
       for (int k = 0; k < NQ; k++)
-        w_m[k][lane] = 0;
+        w_m[k] = 0;
       wave_lds_sync();
 
       // Cross off primes [r0, r1) of the window, in two parts:
@@ -692,7 +693,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
                 const uint32_t e = lq[(qt + lane) & 127];
                 const uint32_t idx = e & 0x7fffffffu;
                 const uint32_t qq = idx / 480;
-                const uint32_t m = 2310 * qq + cop[idx - qq * 480];
+                const uint32_t m = 2310 * qq + c_cop[idx - qq * 480];
                 const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
                 S[k] += (e >> 31) ? (long long) cnt : -(long long) cnt;
               }
@@ -734,8 +735,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
                 S[k] += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
             }
           }
-          if (lane == 0)
-            w_m[k][j] = M;
+          w_m[k] = lane == j ? M : w_m[k];
         }
       }
       flush(run, nwin);
@@ -751,7 +751,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         for (int k = 0; k < NQ; k++) {
           if (live[k] && b + lane < stop_b[k] && lane < nwin_m[k]) {
             TC[k][b + lane - qd.b_first] = C;
-            TM[k][b + lane - qd.b_first] = w_m[k][lane];
+            TM[k][b + lane - qd.b_first] = w_m[k];
           }
         }
         total -= readlane_u32(incl, 63);

@@ -129,7 +129,6 @@ inline uint32_t hsl_lds_bytes(uint32_t W)
 {
   return 8 * W            // segment bitmap
        + 2 * W            // word-pair prefix counts (32-bit, W / 2 pairs)
-       + 480 * 2          // numbers coprime to 2310 in [0, 2310)
        + 240;             // sieve bits of the offsets <= r in a word, r < 240
 }
 
