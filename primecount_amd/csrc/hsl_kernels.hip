@@ -318,10 +318,10 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   uint8_t* lut = reinterpret_cast<uint8_t*>(pre2 + W / 2);    // 240 u8
   // LDS per wave bounds the waves per CU, and the kernel runs faster with
   // more of them, so what fits elsewhere lives elsewhere: the 480 numbers
-  // coprime to 2310 are read from constant memory (c_cop) and the leaf signs
-  // of the window's primes stay in registers (lane j holds prime j's).
+  // coprime to 2310 are read from constant memory (c_cop), the leaf signs
+  // of the window's primes stay in registers (lane j holds prime j's) and the
+  // grouped cross-offs fetch their prime's data from its lane (ds_bpermute).
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
-  __shared__ uint4 wp[64];                                    // per window prime: cross-off data
   __shared__ uint32_t lq[128];                                // leaf queue: factor index | sign << 31
 
   const uint32_t lane = threadIdx.x;
@@ -528,7 +528,6 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       const uint32_t grp = lane >> lg;
       const uint32_t ngrp = kBlock >> lg;
       prm.w = r_h;
-      wp[lane] = prm;
       w_rem[lane] = 0;
       int w_m[NQ];                     // lane j: the leaf-sign sum M of prime j, per member
 #pragma unroll
@@ -591,7 +590,13 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
           u_gsteps++;
           const uint32_t jo = jb + grp;
-          const uint4 pp = wp[jo < r1 ? jo : r0];
+          // the cross-off data of prime jo, read from lane jo (all lanes active)
+          const int src = (int) ((jo < r1 ? jo : r0) << 2);
+          uint4 pp;
+          pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
+          pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
+          pp.z = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z);
+          pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
           const uint32_t kk = lane & (G - 1);
           const bool has = jo < r1 && kk < pp.w;
           const uint32_t idx = (pp.y >> 25) + kk;
