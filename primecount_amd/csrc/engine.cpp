@@ -339,22 +339,22 @@ void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t
 /// Segment size.  One wave owns a segment; smaller segments mean more waves
 /// in flight and a finer-grained work queue, larger ones fewer (segment, b)
 /// slots per number.  Measured on the MI355X for D with the default
-/// parameters (kernel + combine, ms; the sieve limit is x / z):
+/// parameters (one call, ms; the sieve limit is x / z; 10 W + 1,008 bytes of
+/// LDS per wave: 28 waves per CU up to W = 384, 26 at 512, 21 at 640):
 ///
-///   x      limit    W=128   256    384    512
-///   1e15   5.8e8     4.2    6.7   10.0   13.0
-///   1e16   2.2e9     6.6   11.4   15.9   20.3
-///   1e17   8.4e9    16.5   18.9   26.1   34.9
-///   1e18   3.1e10   66.7   47.2   50.4   65.1
-///   2e18   4.6e10  107.9   75.1   66.5   80.2
-///   4e18   6.8e10  158.9  111.2   99.0  100.0
-///   1e19   1.1e11  285.8  196.9  175.2  171.1
+///   x      limit    W=128   256    384    512    640    768
+///   1e17   8.4e9    16.3   16.2   21.9   27.5   34.3   39.7
+///   1e18   3.1e10   63.6   45.6   44.9   54.4   66.2   74.0
+///   1e19   1.1e11  266.9  185.8  156.2  139.4  137.6  155.0
+///   3e19   2.1e11                        274.3  268.9  278.1
+///   1e20   4.8e11                        572.0  563.0  579.1
 ///
 /// Small segments win while a few heavy first segments would otherwise
 /// finish last; in a batch the heavy segments of many queries overlap and
-/// larger segments win, at the same total: batches of 16, 64 and 256 of the
-/// reference x values (1e12 .. 1e18) ran fastest at W = 384, 512, 512.  So
-/// the rule takes the sum of the sieve limits of the launch.
+/// larger segments win, at the same total: the 256 reference x values
+/// (1e12 .. 1e18) in one launch took 1.14, 0.94, 0.83, 0.76, 0.76, 0.80 s at
+/// W = 256, 384, 512, 640, 768, 896.  So the rule takes the sum of the sieve
+/// limits of the launch.
 uint32_t auto_segment_words(uint64_t limit)
 {
   uint32_t cfg = engine_config().segment_words;
@@ -362,7 +362,8 @@ uint32_t auto_segment_words(uint64_t limit)
   if (limit < 15000000000ull) return 128;
   if (limit < 35000000000ull) return 256;
   if (limit < 80000000000ull) return 384;
-  return 512;
+  if (limit < 100000000000ull) return 512;
+  return 640;
 }
 
 /// Table requirements of a group of queries.
