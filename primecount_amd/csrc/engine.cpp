@@ -349,21 +349,29 @@ void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t
 ///   3e19   2.1e11                        274.3  268.9  278.1
 ///   1e20   4.8e11                        572.0  563.0  579.1
 ///
+/// S2_hard (sieve limit z) turns to 640 later:
+///
+///   x      limit    W=256   384    512    640
+///   1e18   3.4e10   57.2   63.9   76.2   87.7
+///   1e19   1.3e11  241.2  196.5  178.6  205.8
+///   1e20   4.7e11 1037.9  838.5  731.8  711.3
+///   1e21   1.8e12                3212   3097
+///
 /// Small segments win while a few heavy first segments would otherwise
 /// finish last; in a batch the heavy segments of many queries overlap and
 /// larger segments win, at the same total: the 256 reference x values
 /// (1e12 .. 1e18) in one launch took 1.14, 0.94, 0.83, 0.76, 0.76, 0.80 s at
 /// W = 256, 384, 512, 640, 768, 896.  So the rule takes the sum of the sieve
 /// limits of the launch.
-uint32_t auto_segment_words(uint64_t limit)
+uint32_t auto_segment_words(uint64_t limit, LeafSum kind)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
   if (limit < 15000000000ull) return 128;
   if (limit < 35000000000ull) return 256;
   if (limit < 80000000000ull) return 384;
-  if (limit < 100000000000ull) return 512;
-  return 640;
+  const uint64_t to640 = kind == LeafSum::D ? 100000000000ull : 300000000000ull;
+  return limit < to640 ? 512 : 640;
 }
 
 /// Table requirements of a group of queries.
@@ -489,7 +497,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     if (!plans[i].empty) sum_limit += plans[i].qd.limit;
   }
   int bpc = 0;
-  uint32_t W = auto_segment_words(sum_limit);
+  uint32_t W = auto_segment_words(sum_limit, T.kind);
   if (W < 128 || W > 4096 || (W % 128) != 0)
     throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
