@@ -101,3 +101,27 @@ def test_pi_128bit_x_both_formulas(pc):
         x = rng.randrange(2 ** 64 + 1, 4 * 10 ** 19)
         log("pi(%d): Gourdon vs Deleglise-Rivat" % x)
         assert pc.pi_deleglise_rivat(x) == pc.pi(x), x
+
+
+def test_pi_deleglise_rivat_any_alpha(pc, oracle):
+    """The split of the leaves between S1, S2_trivial, S2_easy and S2_hard
+    moves with alpha (y = alpha x^(1/3)); pi(x) does not.  Alpha 1 puts
+    y at x^(1/3) (no trivial and no easy leaves), 30 near x^(1/6)."""
+    known = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    try:
+        for alpha in (1.0, 2.5, 7.0, 30.0):
+            pc.set_alpha(alpha)
+            for x in (10 ** 12, 10 ** 15):
+                log("pi_deleglise_rivat(%d), alpha %g, params %s" % (x, alpha, pc.dr_params(x)))
+                assert pc.pi_deleglise_rivat(x) == known[x], (x, alpha)
+            if alpha < 2:
+                continue                 # the oracle's tables would take seconds here
+            x = 987654321987
+            p = pc.dr_params(x)
+            for name, fn in (("S1", lambda: pc.S1(x, p["y"], p["c"])),
+                             ("S2_trivial", lambda: pc.S2_trivial(x, p["y"], p["z"], p["c"])),
+                             ("S2_easy", lambda: pc.S2_easy(x, p["y"], p["z"], p["c"])),
+                             ("P2", lambda: pc.P2(x, p["y"]))):
+                assert fn() == oracle.dr_term(name, x, p["y"], p["z"], p["c"]), (name, alpha)
+    finally:
+        pc.set_alpha(0)
