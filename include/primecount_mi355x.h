@@ -127,10 +127,17 @@ int pcg_set_shared_segments(int on);
 int pcg_factor_table_check(int kind, int64_t y, int64_t limit, uint64_t* mismatches);
 
 /* (segment, prime) slots per kernel pass (0 = default, 2^31).  C and M take
- * 8 bytes of HBM per slot; larger queries run in several passes whose carry is
+ * 4 bytes of HBM per slot; larger queries run in several passes whose carry is
  * kept on the device.  The pass size never changes results.  No counterpart in
  * the reference (its threads carry phi through their own segments). */
 int pcg_set_pass_slots(uint64_t slots);
+
+/* Bits of C in the 32-bit slot word (C | M << bits).  0 (default) takes the
+ * fewest the segment size allows (C <= 64 words); more bits leave fewer for
+ * M, so that more slots escape to the per-pass escape table.  14..28.  Never
+ * changes results; it exists to exercise the escape path.  No counterpart
+ * in the reference. */
+int pcg_set_slot_bits(uint32_t cbits);
 
 /* Host threads used by the table builders (0 = all hardware threads). */
 int pcg_set_host_threads(int threads);
@@ -217,6 +224,15 @@ int pcg_P2(pcg_i128 x, int64_t y, pcg_i128* out);
  * -> src/nth_prime.cpp:95-145). */
 int pcg_nth_prime(int64_t n, int64_t* out);
 
+/* Plausibility check of a pi(x) value: PCG_OK if |pix - Li(x)| <
+ * sqrt(x) log(x) / (8 pi) (Schoenfeld's bound; always for x < 2657), else
+ * PCG_E_INTERNAL with a message.  pcg_pi and pcg_pi_deleglise_rivat run it on
+ * every result and fail with PCG_E_INTERNAL instead of returning a value
+ * outside the bound.  Host arithmetic only (no device needed).
+ * Restates: verify_pix  src/util.cpp:397-420 (called at
+ * src/gourdon/pi_gourdon.cpp:89,160), which aborts instead. */
+int pcg_check_pi(pcg_i128 x, pcg_i128 pix);
+
 typedef struct pcg_pi_stats {
   double seconds_total;      /* wall time of the last pcg_pi call             */
   double seconds_table;      /* tables, including the prime-count table       */
@@ -267,7 +283,7 @@ typedef struct pcg_ctx pcg_ctx;
 
 /* Creates a serving context.  y_max, z_max > 0 build the D tables for default
  * queries up to that size now (else on first use); hbm_budget bounds the
- * per-pass slot arrays in bytes (0 = default 16 GiB); max_batch bounds the
+ * per-pass slot arrays in bytes (0 = default 8 GiB); max_batch bounds the
  * queries of one launch (0 = 1024). */
 int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t max_batch,
                    pcg_ctx** ctx);

@@ -32,7 +32,16 @@ _lib = None
 
 
 class PrimecountError(RuntimeError):
-    """Raised when the engine reports an error (mirrors primecount_error)."""
+    """Raised when the engine reports an error (mirrors primecount_error).
+    `status` is the negative PCG_E_* code of the failed call (E_INVALID,
+    E_DEVICE, E_NOMEM or E_INTERNAL), or None for errors raised here."""
+
+    def __init__(self, msg, status=None):
+        super().__init__(msg)
+        self.status = status
+
+
+E_INVALID, E_DEVICE, E_NOMEM, E_INTERNAL = -1, -2, -3, -4
 
 
 class _I128(ctypes.Structure):
@@ -135,12 +144,14 @@ def load():
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
+        "pcg_set_slot_bits": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_factor_table_check": ([ctypes.c_int, I64, I64, ctypes.POINTER(ctypes.c_uint64)],
                                    ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
         "pcg_release": ([], ctypes.c_int),
         "pcg_pi": ([_I128, PI128], ctypes.c_int),
         "pcg_last_pi_stats": ([ctypes.POINTER(PiStats)], ctypes.c_int),
+        "pcg_check_pi": ([_I128, _I128], ctypes.c_int),
         "pcg_nth_prime": ([I64, P64], ctypes.c_int),
         "pcg_AC": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
@@ -184,7 +195,7 @@ def _from128(r):
 def _check(rc):
     if rc != 0:
         msg = load().pcg_last_error().decode(errors="replace")
-        raise PrimecountError(msg or "error %d" % rc)
+        raise PrimecountError(msg or "error %d" % rc, rc)
 
 
 def _fits64(x):
@@ -250,6 +261,13 @@ def pi(x):
     out = _I128()
     _check(load().pcg_pi(_i128(x), ctypes.byref(out)))
     return _from128(out)
+
+
+def check_pi(x, pix):
+    """The reference's result check (verify_pix, src/util.cpp:397-420): raises
+    PrimecountError (status E_INTERNAL) when |pix - Li(x)| >= sqrt(x) log(x) /
+    (8 pi).  Host arithmetic only."""
+    _check(load().pcg_check_pi(_i128(x), _i128(pix)))
 
 
 def nth_prime(n):
@@ -358,6 +376,11 @@ def factor_table_check(kind, y, limit):
 
 def set_shared_segments(on):
     _check(load().pcg_set_shared_segments(1 if on else 0))
+
+
+def set_slot_bits(bits):
+    """Bits of C in a slot word (0 = automatic); more bits force escapes."""
+    _check(load().pcg_set_slot_bits(int(bits)))
 
 
 def set_pass_slots(n):

@@ -37,6 +37,7 @@ SOURCES = [
     ("params.cpp", ["-ffp-contract=off"], False),
     ("capi.cpp", [], False),
     ("serve.cpp", [], False),
+    ("verify.cpp", [], False),
 ]
 
 

@@ -48,10 +48,7 @@ int guarded(F&& f)
     return PCG_E_NOMEM;
   } catch (const pcg_error& e) {
     t_error = e.what();
-    std::string m = t_error;
-    if (m.find("HIP error") != std::string::npos || m.find("device") != std::string::npos)
-      return PCG_E_DEVICE;
-    return PCG_E_INVALID;
+    return e.status();
   } catch (const std::exception& e) {
     t_error = e.what();
     return PCG_E_INTERNAL;
@@ -95,7 +92,16 @@ HslQuery default_query(LeafSum kind, i128 x)
 
 extern "C" {
 
+static_assert(kErrInvalid == PCG_E_INVALID && kErrDevice == PCG_E_DEVICE &&
+                  kErrNoMem == PCG_E_NOMEM && kErrInternal == PCG_E_INTERNAL,
+              "pcg_error status codes are the C ABI's");
+
 const char* pcg_last_error(void) { return t_error.c_str(); }
+
+int pcg_check_pi(pcg_i128 x, pcg_i128 pix)
+{
+  return guarded([&] { verify_pix("pi", from_abi(x), from_abi(pix)); });
+}
 
 const char* pcg_version(void) { return "0.1.0 (gfx950)"; }
 
@@ -287,13 +293,13 @@ int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t m
     *ctx = nullptr;
     if (y_max < 0 || z_max < 0) throw pcg_error("y_max and z_max must be >= 0");
     if (!device_available())
-      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU");
+      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU", kErrDevice);
     // default parameters have y == z: one shared D table covers them all
     if (y_max > 0 || z_max > 0) {
       const int64_t m = std::max(y_max, z_max);
       hsl_reserve(LeafSum::D, m, m);
     }
-    const uint64_t slots = hbm_budget ? std::max<uint64_t>(hbm_budget / 8, 64) : 0;
+    const uint64_t slots = hbm_budget ? std::max<uint64_t>(hbm_budget / 4, 64) : 0;
     *ctx = reinterpret_cast<pcg_ctx*>(new Server(slots, max_batch));
   });
 }
@@ -428,6 +434,15 @@ int pcg_set_pass_slots(uint64_t slots)
     if (slots != 0 && slots < 64)
       throw pcg_error("pass slots must be 0 or >= 64");
     engine_config().max_slots_per_pass = slots;
+  });
+}
+
+int pcg_set_slot_bits(uint32_t cbits)
+{
+  return guarded([&] {
+    if (cbits != 0 && (cbits < 14 || cbits > 28))
+      throw pcg_error("slot bits must be 0 or in [14, 28]");
+    engine_config().slot_cbits = cbits;
   });
 }
 

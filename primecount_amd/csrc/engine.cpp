@@ -30,7 +30,8 @@ namespace pcg {
     hipError_t e_ = (expr);                                                        \
     if (e_ != hipSuccess)                                                          \
       throw pcg_error(std::string("HIP error '") + hipGetErrorString(e_) +         \
-                      "' in " #expr);                                              \
+                      "' in " #expr,                                               \
+                      e_ == hipErrorOutOfMemory ? kErrNoMem : kErrDevice);         \
   } while (0)
 
 EngineConfig& engine_config()
@@ -59,7 +60,8 @@ struct DevBuf {
       hipError_t e = hipMalloc(&p, bytes);
       if (e != hipSuccess) {
         (void) hipGetLastError();
-        throw pcg_error("device allocation of " + std::to_string(bytes) + " bytes failed");
+        throw pcg_error("device allocation of " + std::to_string(bytes) + " bytes failed",
+                        kErrNoMem);
       }
       cap = bytes;
     }
@@ -87,7 +89,8 @@ struct HostBuf {
       hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
       if (e != hipSuccess) {
         (void) hipGetLastError();
-        throw pcg_error("pinned host allocation of " + std::to_string(bytes) + " bytes failed");
+        throw pcg_error("pinned host allocation of " + std::to_string(bytes) + " bytes failed",
+                        kErrNoMem);
       }
       cap = bytes;
     }
@@ -109,7 +112,8 @@ struct Device {
   std::vector<hipEvent_t> ev;         // timing events, 3 per pass
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
   DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
-  DevBuf tc, tm, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans, groups;
+  DevBuf tc, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans, groups;
+  DevBuf esc_keys, esc_vals, esc_ctl;
   DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
   uint64_t pix_n = 0;
   DevBuf extra[8];                    // scratch of the other pi(x) terms
@@ -131,13 +135,14 @@ struct Device {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
       (void) hipGetLastError();
-      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU");
+      throw pcg_error("no HIP device visible: the MI355X engine needs a gfx950 GPU", kErrDevice);
     }
     PCG_HIP(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     PCG_HIP(hipGetDeviceProperties(&prop, dev));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-      throw pcg_error(std::string("unsupported device ") + prop.gcnArchName + " (need gfx950)");
+      throw pcg_error(std::string("unsupported device ") + prop.gcnArchName + " (need gfx950)",
+                      kErrDevice);
     cus = prop.multiProcessorCount;
     PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : staged) PCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -147,9 +152,9 @@ struct Device {
 
   void release()
   {
-    for (DevBuf* b : {&queries, &items, &binfo, &primes, &factor, &pi_bits, &pi_cnt, &tc, &tm,
+    for (DevBuf* b : {&queries, &items, &binfo, &primes, &factor, &pi_bits, &pi_cnt, &tc,
                       &sacc, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry, &reach,
-                      &sfx, &spans, &groups})
+                      &sfx, &spans, &groups, &esc_keys, &esc_vals, &esc_ctl})
       b->release();
     for (HostBuf* b : {&stage[0], &stage[1]})
       b->release();
@@ -180,6 +185,8 @@ struct QueryPlan {
   std::vector<BInfo> binfo;
   std::vector<uint32_t> reach;     // per b: segments that reach b (non-increasing)
   std::vector<uint64_t> sfx;       // suffix sums of reach
+  std::vector<uint32_t> esc_n;     // per b: the first esc_n segments may hold |M| too large
+                                   // for the slot word (SlotFormat); no later one can
   uint32_t nseg = 0;               // segments that reach at least b_first
 
   /// b values segment s reaches: #{b : reach_b > s}
@@ -199,7 +206,7 @@ struct QueryPlan {
 
 uint64_t imin(uint64_t a, uint64_t b) { return a < b ? a : b; }
 
-/// Slot budget of one pass (C and M take 8 bytes per slot).
+/// Slot budget of one pass (a slot word takes 4 bytes).
 constexpr uint64_t kMaxSlotsPerPass = (uint64_t) 1 << 31;
 
 /// Prime-table range a query needs.
@@ -264,7 +271,7 @@ QueryPlan plan_query(const HslQuery& q, const PrimeTables& P)
   }
   if (limit == 0 || b_first > b_last) return qp;
   if (b_last >= P.primes.size())
-    throw pcg_error("internal: prime table too small");
+    throw pcg_error("internal: prime table too small", kErrInternal);
 
   qp.empty = false;
   qp.qd.limit = limit;
@@ -317,7 +324,7 @@ QueryPlan plan_query(const HslQuery& q, const PrimeTables& P)
 /// stops): segment s reaches b iff max(1, s * 240 W) <= x / p_b^2.  reach_b
 /// counts those segments; it does not increase with b, and segments beyond
 /// reach_{b_first} hold no leaves and are dropped.
-void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W)
+void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t W, uint32_t cbits)
 {
   const uint64_t SEG = 240ull * W;
   const uint64_t nseg_all = (qp.qd.limit + SEG - 1) / SEG;
@@ -334,6 +341,32 @@ void plan_reach(QueryPlan& qp, const HslQuery& q, const PrimeTables& P, uint32_t
   for (uint32_t k = nb; k-- > 0;)
     qp.sfx[k] = qp.sfx[k + 1] + qp.reach[k];
   qp.nseg = qp.reach[0];
+  // Slot (s, b) holds at most floor(xp / low1) - floor(xp / high) leaves
+  // (the integers m of its leaf range), so |M| <= that.  For s >= 1 this is
+  // below xp / (SEG s (s + 1)) + 1, which reaches thr only while
+  // s (s + 1) <= K = xp / (SEG (thr - 1)); segment 0 always counts.
+  const double thr = (double) (1u << (31 - cbits));
+  qp.esc_n.assign(nb, 0);
+  for (uint32_t b = b_first; b <= b_last; b++) {
+    const long double xp = (long double) ((u128) q.x / P.primes[b]);
+    const long double K = xp / ((long double) SEG * (thr - 1));
+    const long double sm = (sqrtl(1 + 4 * K) - 1) / 2;     // largest s with s (s + 1) <= K
+    const uint64_t n = sm < 4e9L ? (uint64_t) sm + 2 : UINT32_MAX;   // + 1 segment of margin
+    qp.esc_n[b - b_first] = (uint32_t) std::min<uint64_t>(n, qp.reach[b - b_first]);
+  }
+}
+
+/// Upper bound of the slots of a span [s0, s1) of a query whose M may not
+/// fit the slot word.
+uint64_t escape_bound(const QueryPlan& qp, uint32_t s0, uint32_t s1)
+{
+  uint64_t n = 0;
+  for (size_t br = 0; br < qp.esc_n.size(); br++) {
+    const uint32_t hi = std::min(s1, qp.esc_n[br]);
+    if (hi <= s0) break;                       // esc_n does not increase with b
+    n += hi - s0;
+  }
+  return n;
 }
 
 /// Segment size.  One wave owns a segment; smaller segments mean more waves
@@ -430,7 +463,7 @@ TableSet build_tables(const TableNeed& n)
   if ((uint64_t) T.ft_limit >= (1ull << 32))
     throw pcg_error("factor table limit must be below 2^32");
   if ((uint64_t) isqrt((u128) T.ft_limit) > T.P.max)
-    throw pcg_error("internal: factor table needs primes up to its square root");
+    throw pcg_error("internal: factor table needs primes up to its square root", kErrInternal);
   return T;
 }
 
@@ -500,6 +533,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   uint32_t W = auto_segment_words(sum_limit, T.kind);
   if (W < 128 || W > 4096 || (W % 128) != 0)
     throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
+  const uint32_t cbits = std::max(slot_format(W).cbits, engine_config().slot_cbits);
+  if (cbits > 28) throw pcg_error("slot bits must be at most 28");
   PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
   bpc = std::max(bpc, 1);
   const uint64_t slots_wg = (uint64_t) D.cus * bpc;
@@ -510,7 +545,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   std::vector<size_t> live;
   for (size_t i = 0; i < qs.size(); i++) {
     if (plans[i].empty) continue;
-    plan_reach(plans[i], qs[i], T.P, W);
+    plan_reach(plans[i], qs[i], T.P, W, cbits);
     if (plans[i].nseg > 0) live.push_back(i);
   }
   if (live.empty()) return result;
@@ -661,9 +696,25 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   D.items.ensure((size_t) max_pass_items * sizeof(SegItem));
   D.cols.ensure(cols_bytes);
   D.tc.ensure(max_pass_slots * 4);
-  D.tm.ensure(max_pass_slots * 4);
   D.queue.ensure(npass * 4);
   PCG_HIP(hsl_launch_fill_zero(D.queue.p, npass * 4, s));
+  // escape tables: per pass at least twice the bound of its escaping slots
+  std::vector<uint32_t> esc_cap(npass);
+  uint32_t max_cap = 0;
+  for (size_t pi = 0; pi < npass; pi++) {
+    uint64_t bound = 0;
+    for (const PassSpan& sp : passes[pi].spans)
+      bound += escape_bound(plans[slot_query[sp.query]], sp.s0, sp.s1);
+    uint64_t cap = 1024;
+    while (cap < 2 * bound + 2) cap *= 2;
+    if (cap > (1ull << 31)) throw pcg_error("internal: escape table too large", kErrInternal);
+    esc_cap[pi] = (uint32_t) cap;
+    max_cap = std::max(max_cap, (uint32_t) cap);
+  }
+  D.esc_keys.ensure((size_t) max_cap * 8);
+  D.esc_vals.ensure((size_t) max_cap * 4);
+  D.esc_ctl.ensure(npass * 8);
+  PCG_HIP(hsl_launch_fill_zero(D.esc_ctl.p, npass * 8, s));
   for (auto& hb : D.stage) hb.ensure(spans_bytes + cols_bytes);
 
   // Shared segments: segment s of every query that covers it in full is
@@ -765,8 +816,14 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     a.factor = D.factor.p;
     a.pi_bits = (const uint64_t*) D.pi_bits.p;
     a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
-    a.T_C = (uint32_t*) D.tc.p;
-    a.T_M = (int32_t*) D.tm.p;
+    a.T_S = (uint32_t*) D.tc.p;
+    a.cbits = cbits;
+    a.esc.keys = (unsigned long long*) D.esc_keys.p;
+    a.esc.vals = (int32_t*) D.esc_vals.p;
+    a.esc.ctl = (unsigned int*) D.esc_ctl.p + 2 * pi;
+    a.esc.cap = esc_cap[pi];
+    // the previous pass's combine has read its table (stream order)
+    PCG_HIP(hsl_launch_fill_zero(D.esc_keys.p, (size_t) esc_cap[pi] * 8, s));
     a.S_acc = (unsigned long long*) D.sacc.p;
     a.counters = (unsigned long long*) D.counters.p;
     a.queue = (unsigned int*) D.queue.p + pi;
@@ -789,8 +846,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
     PCG_HIP(hipEventRecord(D.event(3 * pi + 1), s));
     PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const SegItem*) D.items.p,
-                               (const uint32_t*) D.tc.p, (const int32_t*) D.tm.p, slice_len,
-                               nslices, (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p,
+                               (const uint32_t*) D.tc.p, a.esc, cbits, slice_len, nslices,
+                               (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p,
                                (long long*) D.carry.p, D.out.p, s));
     PCG_HIP(hipEventRecord(D.event(3 * pi + 2), s));
   }
@@ -801,7 +858,14 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(sacc.data(), D.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, D.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+  std::vector<uint32_t> esc_ctl(2 * npass);
+  PCG_HIP(hipMemcpyAsync(esc_ctl.data(), D.esc_ctl.p, npass * 8, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
+  for (size_t pi = 0; pi < npass; pi++)
+    if (esc_ctl[2 * pi + 1] != 0)
+      throw pcg_error("internal: escape table overflow (" + std::to_string(esc_ctl[2 * pi]) +
+                          " escaping slots, capacity " + std::to_string(esc_cap[pi]) + ")",
+                      kErrInternal);
   for (size_t pi = 0; pi < npass; pi++) {
     float mk = 0, mc = 0;
     PCG_HIP(hipEventElapsedTime(&mk, D.ev[3 * pi], D.ev[3 * pi + 1]));
@@ -1000,7 +1064,7 @@ std::unique_lock<std::mutex> engine_lock()
 
 void* engine_buffer(int slot, size_t bytes)
 {
-  if (slot < 0 || slot >= 8) throw pcg_error("internal: engine buffer slot");
+  if (slot < 0 || slot >= 8) throw pcg_error("internal: engine buffer slot", kErrInternal);
   return g_dev.extra[slot].ensure(bytes);
 }
 
@@ -1015,7 +1079,7 @@ static void ensure_pix_table(uint64_t n)
   n = std::max<uint64_t>(n, 1u << 20);
   const TableSet& T = g_tables;
   const uint64_t r = isqrt(n);
-  if (r > T.P.max) throw pcg_error("internal: prime table too small for the prime-count table");
+  if (r > T.P.max) throw pcg_error("internal: prime table too small for the prime-count table", kErrInternal);
   const uint64_t nwords = n / 240 + 1;
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
   D.pix_n = 0;

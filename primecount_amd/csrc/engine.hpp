@@ -27,6 +27,8 @@ struct EngineConfig {
   int host_threads = 0;         // 0 = all hardware threads
   uint64_t max_slots_per_pass = 0;   // 0 = default (2^31 slots, 16 GiB)
   bool share_segments = true;   // queries of a launch share the sieve of a segment
+  uint32_t slot_cbits = 0;      // 0 = automatic; else bits of C in a slot word (at least
+                                // what the segment size needs): fewer bits for M, more escapes
 };
 
 EngineConfig& engine_config();

@@ -34,6 +34,7 @@
 #include <algorithm>
 
 #include "hsl_kernels.hpp"
+#include "quotient_fixup.hpp"
 #include "wheel_dev.hpp"
 
 namespace pcg {
@@ -66,19 +67,14 @@ __device__ __forceinline__ void wave_lds_sync()
 }
 
 /// floor(xp / m) - low for a leaf whose quotient lies in [low, low + 2^20)
-/// (the leaf ranges guarantee low <= xp / m < high) and m < 2^32.
-/// r = xp - low * m is exact modulo 2^64 from the low word of xp because
-/// 0 <= r < m * 2^20 < 2^52; r / m is estimated in double with the hardware
-/// reciprocal (any relative error below 2^-21 keeps the estimate within 1/2
-/// of r / m < 2^20, so its truncation is within one of the floor), and the
-/// integer residual r - s * m fixes the last unit.
+/// (the leaf ranges guarantee low <= xp / m < high) and m < 2^32: r / m is
+/// estimated in double with the hardware reciprocal, and the integer
+/// residual corrects the estimate (quotient_fixup: a loop per direction,
+/// exact whatever the reciprocal's error; with a relative error below 2^-21
+/// the estimate is within one unit and each loop runs at most once).
 __device__ __forceinline__ uint32_t leaf_stop(uint64_t xlo, uint64_t low, uint32_t m)
 {
-  const uint64_t r = xlo - low * m;
-  const double md = (double) m;
-  uint32_t s = (uint32_t) ((double) r * __builtin_amdgcn_rcp(md));
-  const int64_t t = (int64_t) (r - (uint64_t) s * m);
-  return t < 0 ? s - 1 : ((uint64_t) t >= m ? s + 1 : s);
+  return leaf_stop_from_rcp(xlo, low, m, __builtin_amdgcn_rcp((double) m));
 }
 
 /// min(floor(xp / d), cap) for cap < 2^48, with inv_d = 1 / d rounded to
@@ -287,6 +283,60 @@ __device__ __forceinline__ uint32_t wave_total_u32(uint32_t v)
   return readlane_u32(wave_incl_scan(v), 63);
 }
 
+__device__ __forceinline__ uint32_t esc_hash(unsigned long long key, uint32_t cap)
+{
+  return (uint32_t) ((key * 0x9E3779B97F4A7C15ull) >> 32) & (cap - 1);
+}
+
+/// Records M of slot `off` in the escape table (called by the lanes whose M
+/// does not fit the slot word).  At most cap / 2 entries are inserted, so
+/// the probe always reaches an empty entry; a further insert only raises
+/// the overflow flag, which the host reports as an error.
+__device__ __forceinline__ void esc_insert(EscTable t, uint64_t off, int32_t M)
+{
+  const unsigned int n = atomicAdd(&t.ctl[0], 1u);
+  if (n >= t.cap / 2) {
+    atomicOr(&t.ctl[1], 1u);
+    return;
+  }
+  const unsigned long long key = (unsigned long long) off + 1;
+  uint32_t h = esc_hash(key, t.cap);
+  for (;;) {
+    if (atomicCAS(&t.keys[h], 0ull, key) == 0ull) {
+      t.vals[h] = M;
+      return;
+    }
+    h = (h + 1) & (t.cap - 1);
+  }
+}
+
+/// M of an escaped slot; 0 if it is missing (only after an overflow, which
+/// the host turns into an error).  The probe ends at the first empty entry.
+__device__ __forceinline__ int32_t esc_lookup(const EscTable& t, uint64_t off)
+{
+  const unsigned long long key = (unsigned long long) off + 1;
+  uint32_t h = esc_hash(key, t.cap);
+  for (uint32_t i = 0; i < t.cap; i++) {
+    const unsigned long long k = t.keys[h];
+    if (k == key) return t.vals[h];
+    if (k == 0ull) return 0;
+    h = (h + 1) & (t.cap - 1);
+  }
+  return 0;
+}
+
+/// Slot word of (C, M): C | M << cbits, or C | the escape marker (bit 31)
+/// with M recorded in the escape table.
+__device__ __forceinline__ uint32_t pack_slot(uint32_t C, int32_t M, uint32_t cbits, EscTable esc,
+                                              uint64_t off)
+{
+  const int32_t lim = 1 << (31 - cbits);          // |M| < lim fits the signed field
+  if (M > -lim && M < lim)
+    return C | ((uint32_t) M << cbits);
+  esc_insert(esc, off, M);
+  return C | 0x80000000u;
+}
+
 /// One wave owns one segment at a time: the epochs need no workgroup
 /// barriers (a wave's LDS operations execute in order), and the per-prime
 /// bookkeeping runs once per segment and prime.
@@ -308,9 +358,9 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     const SegGroup* __restrict__ groups, const BInfo* __restrict__ binfo_all,
     const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
     const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,
-    uint32_t* __restrict__ T_C, int32_t* __restrict__ T_M, unsigned long long* __restrict__ S_acc,
+    uint32_t* __restrict__ T_S, EscTable esc, unsigned long long* __restrict__ S_acc,
     unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,
-    uint32_t nitems, uint32_t W)
+    uint32_t nitems, uint32_t W, uint32_t cbits)
 {
   extern __shared__ __align__(16) uint32_t lds[];
   uint32_t* bm = lds;                                         // 2W u32
@@ -358,8 +408,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     }
     const QueryDesc qd = queries[its[0].query];     // b_first, pre_last and limit are shared
     const BInfo* binfo[NQ];
-    uint32_t* TC[NQ];
-    int* TM[NQ];
+    uint32_t* TS[NQ];
     uint32_t b_stop_m[NQ], stop_b[NQ];
     bool live[NQ];
     long long S[NQ];
@@ -368,8 +417,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     for (int k = 0; k < NQ; k++) {
       const QueryDesc q = queries[its[k].query];
       binfo[k] = binfo_all + q.binfo_off;
-      TC[k] = T_C + its[k].t_off;
-      TM[k] = T_M + its[k].t_off;
+      TS[k] = T_S + its[k].t_off;
       b_stop_m[k] = qd.b_first + its[k].nbs - 1;
       live[k] = k < (int) nmem;
       stop_b[k] = live[k] ? b_stop_m[k] + 1 : 0u;
@@ -755,8 +803,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
           if (live[k] && b + lane < stop_b[k] && lane < nwin_m[k]) {
-            TC[k][b + lane - qd.b_first] = C;
-            TM[k][b + lane - qd.b_first] = w_m[k];
+            const uint32_t rel = b + lane - qd.b_first;
+            TS[k][rel] = pack_slot(C, w_m[k], cbits, esc, its[k].t_off + rel);
           }
         }
         total -= readlane_u32(incl, 63);
@@ -775,10 +823,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       if (!live[k])
         continue;
       // slots the member never reached hold zero
-      for (uint32_t bq = stop_b[k] + lane; bq <= b_stop_m[k]; bq += kBlock) {
-        TC[k][bq - qd.b_first] = 0;
-        TM[k][bq - qd.b_first] = 0;
-      }
+      for (uint32_t bq = stop_b[k] + lane; bq <= b_stop_m[k]; bq += kBlock)
+        TS[k][bq - qd.b_first] = 0;
       const long long tot = wave_sum_i64(S[k]);
       if (lane == 0 && tot != 0) {
         // 128-bit accumulation from two 64-bit atomics: the add whose low word
@@ -815,8 +861,8 @@ static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st
 {
   hipLaunchKernelGGL((hsl_sieve_kernel<FT, NQ>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
                      a.queries, a.items, a.groups, a.binfo, a.primes,
-                     reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_C, a.T_M,
-                     a.S_acc, a.counters, a.queue, a.nitems, a.W);
+                     reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
+                     a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
   return hipGetLastError();
 }
 
@@ -884,10 +930,11 @@ __global__ void hsl_build_items(const QueryDesc* __restrict__ queries,
 /// M(s, b) * (counts of the earlier segments inside the slice).
 __global__ void hsl_combine_slices(const CombineCol* __restrict__ cols, uint32_t ncols,
                                    const SegItem* __restrict__ items,
-                                   const uint32_t* __restrict__ T_C, const int* __restrict__ T_M,
+                                   const uint32_t* __restrict__ T_S, EscTable esc, uint32_t cbits,
                                    uint32_t slice_len, uint32_t nslices,
                                    long long* sC, long long* sM, __int128* sMP)
 {
+  const uint32_t cmask = (1u << cbits) - 1;
   uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   uint32_t sl = blockIdx.y;
   if (c >= ncols)
@@ -901,27 +948,31 @@ __global__ void hsl_combine_slices(const CombineCol* __restrict__ cols, uint32_t
   // 8 items per step: the 16 loads of a step are independent and in flight
   // together
   for (; i + 8 <= i1; i += 8) {
-    uint32_t tc[8];
-    int tm[8];
+    uint32_t v[8];
+    uint64_t off[8];
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      const uint64_t off = items[cc.item_first + i + u].t_off + cc.b_rel;
-      tc[u] = T_C[off];
-      tm[u] = T_M[off];
+      off[u] = items[cc.item_first + i + u].t_off + cc.b_rel;
+      v[u] = T_S[off[u]];
     }
 #pragma unroll
     for (int u = 0; u < 8; u++) {
-      MP += (__int128) tm[u] * P;
-      P += tc[u];
-      Ms += tm[u];
+      int tm = (int) v[u] >> cbits;
+      if ((v[u] & ~cmask) == 0x80000000u)         // escape marker
+        tm = esc_lookup(esc, off[u]);
+      MP += (__int128) tm * P;
+      P += v[u] & cmask;
+      Ms += tm;
     }
   }
   for (; i < i1; i++) {
     const uint64_t off = items[cc.item_first + i].t_off + cc.b_rel;
-    const long long tc = T_C[off];
-    const long long tm = T_M[off];
+    const uint32_t v = T_S[off];
+    int tm = (int) v >> cbits;
+    if ((v & ~cmask) == 0x80000000u)
+      tm = esc_lookup(esc, off);
     MP += (__int128) tm * P;
-    P += tc;
+    P += v & cmask;
     Ms += tm;
   }
   size_t o = (size_t) sl * ncols + c;
@@ -1131,14 +1182,14 @@ hipError_t hsl_launch_build_items(const QueryDesc* queries, const PassSpan* span
 }
 
 hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const SegItem* items,
-                              const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
-                              uint32_t nslices, long long* sC, long long* sM, void* sMP,
-                              long long* carry, void* out, hipStream_t st)
+                              const uint32_t* T_S, const EscTable& esc, uint32_t cbits,
+                              uint32_t slice_len, uint32_t nslices, long long* sC, long long* sM,
+                              void* sMP, long long* carry, void* out, hipStream_t st)
 {
   if (ncols == 0) return hipSuccess;
   dim3 g1((ncols + 255) / 256, nslices);
-  hipLaunchKernelGGL(hsl_combine_slices, g1, dim3(256), 0, st, cols, ncols, items, T_C, T_M,
-                     slice_len, nslices, sC, sM, (__int128*) sMP);
+  hipLaunchKernelGGL(hsl_combine_slices, g1, dim3(256), 0, st, cols, ncols, items, T_S, esc,
+                     cbits, slice_len, nslices, sC, sM, (__int128*) sMP);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(hsl_combine_final, dim3((ncols + 255) / 256), dim3(256), 0, st, cols, ncols,

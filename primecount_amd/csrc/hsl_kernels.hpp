@@ -11,7 +11,8 @@
 //             or a prime m (type 2, prime loop); it contributes
 //             sign * phi(x / (p_b * m), b - 1)
 //   slot      one (segment, b) pair: C = unsieved numbers of the segment
-//             before p_b is crossed off, M = sum of the leaf signs
+//             before p_b is crossed off, M = sum of the leaf signs, packed
+//             into one 32-bit word (SlotFormat)
 #pragma once
 
 #include <stdint.h>
@@ -92,6 +93,34 @@ struct alignas(16) SegGroup {
   uint32_t pad[3];
 };
 
+/// One 32-bit word per slot: C in the low `cbits` bits (C <= 64 W, so
+/// cbits = 16 for W < 1024, up to 19 for W = 4096) and M as a signed
+/// (32 - cbits)-bit field above it.  |M| >= 2^(31 - cbits) does not fit: the
+/// field then holds its most negative value (the escape marker, bit 31 alone)
+/// and M goes to the escape table.
+struct SlotFormat {
+  uint32_t cbits;
+};
+
+inline SlotFormat slot_format(uint32_t W)
+{
+  uint32_t cb = 1;
+  while ((1ull << cb) <= 64ull * W) cb++;     // C <= 64 W < 2^cb
+  return SlotFormat{cb};
+}
+
+/// Escape table of one pass: open addressing over `cap` entries (a power of
+/// two), key = slot offset + 1 (0 = empty), value = M.  The host sizes cap
+/// to at least twice an upper bound of the escaping slots (a slot's |M| is at
+/// most the number of m in its leaf range), so inserts always find room;
+/// count > cap / 2 would set the overflow flag instead of looping.
+struct EscTable {
+  unsigned long long* keys;
+  int32_t* vals;
+  unsigned int* ctl;            // [0] entries inserted, [1] overflow flag
+  uint32_t cap;
+};
+
 struct HslArgs {
   const QueryDesc* queries;
   const SegItem* items;
@@ -101,8 +130,9 @@ struct HslArgs {
   const void* factor;           // uint16_t or uint32_t factor table
   const uint64_t* pi_bits;      // prime bits per 240 numbers (primes >= 7)
   const uint32_t* pi_cnt;       // primes >= 7 below 240 * i
-  uint32_t* T_C;                // per slot: unsieved count before p_b is crossed off
-  int32_t* T_M;                 // per slot: sum of leaf signs
+  uint32_t* T_S;                // per slot: C | M << cbits (SlotFormat)
+  EscTable esc;                 // M of the slots whose |M| does not fit
+  uint32_t cbits;
   unsigned long long* S_acc;    // per query: 128-bit sum of sign * in-segment count (lo, hi)
   unsigned long long* counters; // leaves, hits, factor entries, slots
   unsigned int* queue;          // next work item
@@ -149,8 +179,8 @@ hipError_t hsl_launch_build_items(const QueryDesc* queries, const PassSpan* span
                                   const uint32_t* reach, const uint64_t* sfx, SegItem* items,
                                   uint32_t nitems, hipStream_t st);
 hipError_t hsl_launch_combine(const CombineCol* cols, uint32_t ncols, const SegItem* items,
-                              const uint32_t* T_C, const int32_t* T_M, uint32_t slice_len,
-                              uint32_t nslices, long long* sC, long long* sM, void* sMP,
-                              long long* carry, void* out, hipStream_t st);
+                              const uint32_t* T_S, const EscTable& esc, uint32_t cbits,
+                              uint32_t slice_len, uint32_t nslices, long long* sC, long long* sM,
+                              void* sMP, long long* carry, void* out, hipStream_t st);
 
 } // namespace pcg

@@ -12,11 +12,26 @@ namespace pcg {
 typedef __int128 i128;
 typedef unsigned __int128 u128;
 
-/// Error type thrown by the C++ layer; the C ABI turns it into a status code.
-/// Mirrors primecount_error (include/primecount.hpp:26-32).
+/// Status codes carried by pcg_error; the values are the PCG_E_* codes of the
+/// C ABI (include/primecount_mi355x.h).
+enum : int {
+  kErrInvalid = -1,    // argument outside the supported domain
+  kErrDevice = -2,     // HIP runtime / device error
+  kErrNoMem = -3,      // device or host allocation failed
+  kErrInternal = -4,   // broken internal invariant
+};
+
+/// Error type thrown by the C++ layer; the C ABI returns its status().
+/// Mirrors primecount_error (include/primecount.hpp:26-32), with the kind of
+/// failure kept as a code instead of being guessed from the message.
 class pcg_error : public std::runtime_error {
 public:
-  explicit pcg_error(const std::string& msg) : std::runtime_error(msg) {}
+  explicit pcg_error(const std::string& msg, int status = kErrInvalid)
+      : std::runtime_error(msg), status_(status) {}
+  int status() const { return status_; }
+
+private:
+  int status_;
 };
 
 /// floor(sqrt(n)) for n < 2^127, exact.

@@ -34,7 +34,8 @@ namespace pcg {
     hipError_t e_ = (expr);                                                        \
     if (e_ != hipSuccess)                                                          \
       throw pcg_error(std::string("HIP error '") + hipGetErrorString(e_) +         \
-                      "' in " #expr);                                              \
+                      "' in " #expr,                                               \
+                      e_ == hipErrorOutOfMemory ? kErrNoMem : kErrDevice);         \
   } while (0)
 
 namespace {
@@ -85,7 +86,7 @@ void acc_read(Acc& A, hipStream_t s)
 std::vector<uint64_t> pi_points(const EngineView& v, const std::vector<uint64_t>& ns)
 {
   for (uint64_t n : ns)
-    if (n > v.pix_n) throw pcg_error("internal: prime-count table too small");
+    if (n > v.pix_n) throw pcg_error("internal: prime-count table too small", kErrInternal);
   uint64_t* dn = (uint64_t*) engine_buffer(1, ns.size() * 8);
   uint64_t* dout = (uint64_t*) engine_buffer(2, ns.size() * 8);
   PIX_HIP(hipMemcpyAsync(dn, ns.data(), ns.size() * 8, hipMemcpyHostToDevice, v.stream));
@@ -142,7 +143,7 @@ void launch_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v, unsign
   const uint64_t sqrt_xy = isqrt((u128) x / (u128) y);
   const uint32_t b_lo = (uint32_t) v.host->pi((uint64_t) x_star) + 1;
   const uint32_t b_hi = (uint32_t) v.host->pi(x13) + 1;
-  if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma");
+  if (b_hi > v.nprimes) throw pcg_error("internal: prime table too small for Sigma", kErrInternal);
   PIX_HIP(pix_launch_sigma((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, sqrt_xy, v.primes,
                            b_lo, b_hi, v.pix_tab, acc, v.stream));
 }
@@ -196,7 +197,7 @@ void launch_phi0(i128 x, int64_t z, int64_t k, const EngineView& v, const PhiTin
 {
   static const uint32_t small[8] = {2, 3, 5, 7, 11, 13, 17, 19};
   const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) z) + 1;
-  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for Phi0");
+  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for Phi0", kErrInternal);
   PIX_HIP(pix_launch_phi0((uint64_t) x, (uint64_t) ((u128) x >> 64), v.factor, v.factor32, false,
                           nidx, (uint32_t) k, small[k - 1], dtbl, t.pp, t.phi_pp, acc, v.stream));
 }
@@ -270,7 +271,7 @@ void ac_lists_part(i128 x, int64_t y, int64_t z, int64_t k, int64_t x_star, cons
   const u128 xhigh = ux / sqrtx;                 // one segment [0, isqrt(x))
   const uint64_t uy = (uint64_t) y, uz = (uint64_t) z;
   auto pi = [&](uint64_t n) -> uint64_t {
-    if (n > P.max) throw pcg_error("internal: prime table too small for A/C");
+    if (n > P.max) throw pcg_error("internal: prime table too small for A/C", kErrInternal);
     return P.pi(n);
   };
   auto xp_of = [&](uint64_t p) -> uint64_t {
@@ -444,7 +445,7 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P, uint64_t max_quotient)
     t.pad = 0;
     // the tiled kernel leaves out the pi(v) < 3 corrections: every quotient
     // of the plus list is at least p >= 7
-    if (d.xp / P.primes[t.i_hi] < 7) throw pcg_error("internal: A/C quotient below 7");
+    if (d.xp / P.primes[t.i_hi] < 7) throw pcg_error("internal: A/C quotient below 7", kErrInternal);
     // weight-2 leaves: primes[i] > q1
     uint64_t two = 0;
     if (d.q1 < P.max) {
@@ -647,7 +648,7 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
   std::vector<uint64_t> pts(n);
   for (size_t j = 0; j < n; j++) pts[j] = isqrt((u128) xs[idx[j]]);
   for (uint64_t p : pts)
-    if (p > v.pix_n) throw pcg_error("internal: prime-count table too small");
+    if (p > v.pix_n) throw pcg_error("internal: prime-count table too small", kErrInternal);
   PIX_HIP(hipMemcpyAsync(dn, pts.data(), n * 8, hipMemcpyHostToDevice, v.stream));
   PIX_HIP(pix_launch_pi_points(dn, dpts, (uint32_t) n, v.pix_tab, v.stream));
 
@@ -714,7 +715,7 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
     const uint64_t x13 = iroot<3>((u128) x);
     const uint64_t sqrt_xy = isqrt((u128) x / (u128) gp.y);
     const uint64_t top = umax(umax((uint64_t) gp.y, x13), umax(sqrt_xy, (uint64_t) gp.x_star));
-    if (top > P.max) throw pcg_error("internal: prime table too small for Sigma");
+    if (top > P.max) throw pcg_error("internal: prime table too small for Sigma", kErrInternal);
     const i128 sigma = sigma_value(P.pi((uint64_t) gp.y), P.pi(x13), P.pi(sqrt_xy),
                                    P.pi((uint64_t) gp.x_star), (i128) pis[j], hj);
     const i128 phi0 = phi_tiny_host(x, phi0_table(gp.k)) + acc_value(hj + 6);
@@ -781,6 +782,9 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st, ui
     s.table_limit = std::max<uint64_t>((uint64_t) s.table_limit, (uint64_t) v.pix_n);
     pi_group(max_slots, xs, g, gps, v, result, s);
   }
+  // the reference checks every pi(x) against Schoenfeld's bound
+  // (src/gourdon/pi_gourdon.cpp:89,160)
+  for (size_t i = 0; i < xs.size(); i++) verify_pix("pi_gourdon", xs[i], result[i]);
   s.seconds_total = now_s() - t0;
   if (st) *st = s;
   return result;
@@ -817,7 +821,7 @@ void s2_easy_part(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables& P,
   const u128 ux = (u128) x;
   const uint64_t uy = (uint64_t) y, uz = (uint64_t) z;
   auto pi = [&](uint64_t n) -> uint64_t {
-    if (n > P.max) throw pcg_error("internal: prime table too small for S2_easy");
+    if (n > P.max) throw pcg_error("internal: prime table too small for S2_easy", kErrInternal);
     return P.pi(n);
   };
   const uint64_t pi_sqrty = pi(isqrt((u128) uy));
@@ -866,7 +870,7 @@ i128 s2_trivial_host(i128 x, int64_t y, int64_t z, int64_t c, const PrimeTables&
 {
   if (y < 2) return 0;
   const uint64_t uy = (uint64_t) y;
-  if (uy > P.max) throw pcg_error("internal: prime table too small for S2_trivial");
+  if (uy > P.max) throw pcg_error("internal: prime table too small for S2_trivial", kErrInternal);
   const i128 pi_y = (i128) P.pi(uy);
   const uint64_t sqrtz = isqrt((u128) z);
   const uint64_t prime_c = P.primes[(size_t) c];
@@ -916,7 +920,7 @@ i128 run_s1(i128 x, int64_t y, int64_t c, const EngineView& v)
   const PhiTiny& t = phi_tiny_table((uint32_t) c);
   const uint32_t* dtbl = phi_tiny_device(t, v);
   const uint64_t nidx = (uint64_t) ft_to_index((uint64_t) y) + 1;
-  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for S1");
+  if (nidx > v.factor_n) throw pcg_error("internal: factor table too small for S1", kErrInternal);
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
@@ -969,7 +973,7 @@ i128 dr_term(DRTerm term, i128 x, int64_t y, int64_t z, int64_t c)
   auto lock = engine_lock();
   const EngineView v =
       engine_view(y, z, dr_table_need(x, y), LeafSum::S2_hard, (uint64_t) y + 1);
-  if ((uint64_t) y > v.host->max) throw pcg_error("internal: prime table below y");
+  if ((uint64_t) y > v.host->max) throw pcg_error("internal: prime table below y", kErrInternal);
   switch (term) {
     case DRTerm::S1: return run_s1(x, y, c, v);
     case DRTerm::S2_trivial: return s2_trivial_host(x, y, z, c, *v.host);
@@ -988,7 +992,7 @@ i128 pi_deleglise_rivat(i128 x)
   auto lock = engine_lock();
   const EngineView v = engine_view(p.y, p.z, dr_table_need(x, p.y), LeafSum::S2_hard,
                                      (uint64_t) p.y + 1);
-  if ((uint64_t) p.y > v.host->max) throw pcg_error("internal: prime table below y");
+  if ((uint64_t) p.y > v.host->max) throw pcg_error("internal: prime table below y", kErrInternal);
   i128 r = run_s1(x, p.y, p.c, v);
   r += s2_trivial_host(x, p.y, p.z, p.c, *v.host);
   r += run_s2_easy(x, p.y, p.z, p.c, v);
@@ -1001,6 +1005,8 @@ i128 pi_deleglise_rivat(i128 x)
   q.z = p.z;
   q.k = p.c;
   r += hsl_compute_batch_locked({q}, nullptr)[0];
+  // src/deleglise-rivat/pi_deleglise_rivat.cpp:92,138
+  verify_pix("pi_deleglise_rivat", x, r);
   return r;
 }
 
@@ -1086,7 +1092,7 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
       uint64_t total = 0;
       for (uint64_t b : bc) total += b;
       if (total < k) {
-        if (down && lo == 2) throw pcg_error("internal: nth_prime interval");
+        if (down && lo == 2) throw pcg_error("internal: nth_prime interval", kErrInternal);
         delta *= 2;
         continue;
       }
@@ -1124,7 +1130,7 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
           }
         }
       }
-      if (!res) throw pcg_error("internal: nth_prime selection");
+      if (!res) throw pcg_error("internal: nth_prime selection", kErrInternal);
       break;
     }
     result[big_i[j]] = res;

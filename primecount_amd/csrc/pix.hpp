@@ -50,4 +50,15 @@ uint64_t nth_prime(uint64_t n, PixStats* st);
 std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,
                                       uint64_t max_slots = 0);
 
+/// Li(x) = li(x) - li(2), the offset logarithmic integral (long double).
+long double offset_log_integral(long double x);
+
+/// Whether pi(x) = pix lies within Schoenfeld's bound |pi(x) - Li(x)| <
+/// sqrt(x) log(x) / (8 pi) (always true for x < 2657), the check the
+/// reference runs after every pi(x) (verify_pix, src/util.cpp:397-420).
+bool pix_plausible(i128 x, i128 pix);
+
+/// Throws pcg_error (kErrInternal) naming the function if pix fails the check.
+void verify_pix(const char* name, i128 x, i128 pix);
+
 } // namespace pcg

@@ -23,12 +23,7 @@ double now_s()
 int status_of(const std::exception& e)
 {
   if (dynamic_cast<const std::bad_alloc*>(&e)) return PCG_E_NOMEM;
-  if (dynamic_cast<const pcg_error*>(&e)) {
-    const std::string m = e.what();
-    if (m.find("HIP error") != std::string::npos || m.find("device") != std::string::npos)
-      return PCG_E_DEVICE;
-    return PCG_E_INVALID;
-  }
+  if (const pcg_error* pe = dynamic_cast<const pcg_error*>(&e)) return pe->status();
   return PCG_E_INTERNAL;
 }
 

@@ -228,3 +228,31 @@ def test_device_factor_table(pc, kind, y, limit):
     log("factor table %s y=%d limit=%d" % (kind, y, limit))
     k = pc.KIND_D if kind == "D" else pc.KIND_S2_HARD
     assert pc.factor_table_check(k, y, limit) == 0
+
+
+@pytest.mark.parametrize("bits", [20, 24, 28])
+def test_slot_escape_invariance(pc, bits):
+    # C and M share one 32-bit slot word; a slot whose M does not fit the
+    # field above C goes to the pass's escape table.  More bits for C leave
+    # fewer for M (28: |M| < 8), so that many slots escape: D, S2_hard,
+    # a batch and a many-pass run give the same sums (an escape table too
+    # small for them would fail the call, never return a wrong sum).
+    rows = {int(v["x"]): v for v in golden("configs.json")["D"]}
+    s2 = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}
+    kat = [[int(v) for v in r] for r in golden("kat_D.json")["rows"]][::5]
+    fx = [r for r in golden("batch_D.json")["rows"] if int(r["x"]) <= 10 ** 15][:16]
+    try:
+        pc.set_slot_bits(bits)
+        log("slot bits %d: D default 1e13, 1e15, 1e17, S2_hard 1e15" % bits)
+        for e in (13, 15, 17):
+            assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+        assert pc.S2_hard_default(10 ** 15) == int(s2[10 ** 15]["value"])
+        for x, y, z, k, want in kat:
+            assert pc.D(x, y, z, k) == want, (x, y, z, k)
+        log("slot bits %d: batch of %d" % (bits, len(fx)))
+        assert pc.D_batch([int(r["x"]) for r in fx]) == [int(r["D"]) for r in fx]
+        pc.set_pass_slots(1 << 16)
+        assert pc.D_default(10 ** 15) == int(rows[10 ** 15]["value"])
+    finally:
+        pc.set_pass_slots(0)
+        pc.set_slot_bits(0)

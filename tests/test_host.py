@@ -2,18 +2,23 @@
 
 - the table builders against the arithmetic definitions (a C++ harness
   compiled here from primecount_amd/csrc/tables.cpp);
-- the error analysis of the kernel's leaf quotient (FP64 reciprocal estimate
-  plus one integer fix-up);
+- the kernel's leaf quotient (FP64 reciprocal estimate plus integer fix-up
+  loops) under reciprocal errors far above the hardware's;
+- the reference's pi(x) result check (Schoenfeld's bound) and the typed
+  status codes;
 - the shared library loads and exports every function that
   include/primecount_mi355x.h declares; the C++ overload header compiles;
 - without a device the engine refuses to run (no silent CPU fallback).
 """
 import ctypes
+import math
 import os
 import re
 import subprocess
 
 import pytest
+
+from tests.conftest import golden
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -28,8 +33,9 @@ def test_table_builders_match_definitions(tmp_path):
 
 
 def test_leaf_quotient_estimate_is_exact(tmp_path):
-    # the sieve kernel's FP64-reciprocal quotient with one integer fix-up,
-    # under a reciprocal with a relative error up to 2^-22 (the hardware's is
+    # the sieve kernel's FP64-reciprocal quotient with its integer fix-up
+    # loops (the kernel's own code, quotient_fixup.hpp), under reciprocals
+    # with relative errors up to 2^-18, 2^-10 and 2^-4 (the hardware's is
     # near 2^-24)
     exe = tmp_path / "leaf_stop_check"
     subprocess.run(["g++", "-O2", "-std=c++17", "-o", str(exe),
@@ -100,3 +106,75 @@ def test_argument_errors(pc):
     with pytest.raises(pc.PrimecountError):
         pc.set_host_threads(-1)
     pc.set_segment_words(0)
+
+
+# li(10^n) - pi(10^n) for n = 10 .. 22: the published table of the error of
+# the logarithmic integral (e.g. OEIS A057752); with Li(x) = li(x) - li(2)
+# the difference Li(x) - pi(x) is one less, rounded down.
+LI_MINUS_PI = {10: 3104, 11: 11588, 12: 38263, 13: 108971, 14: 314890, 15: 1052619,
+               16: 3214632, 17: 7956589, 18: 21949555, 19: 99877775, 20: 222744644,
+               21: 597394254, 22: 1932355208}
+
+
+def schoenfeld_bound(x):
+    return math.sqrt(x) * math.log(x) / (8 * math.pi)
+
+
+def test_result_check_bound(pc):
+    """pcg_check_pi (the reference's verify_pix, src/util.cpp:397-420):
+    the README values pass, and values just inside and just outside
+    |pix - Li(x)| < sqrt(x) log(x) / (8 pi) fall on the two sides, which pins
+    both Li(x) (to a few units, against the published li(x) - pi(x)) and the
+    bound."""
+    pis = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    for n, d in LI_MINUS_PI.items():
+        x = 10 ** n
+        pc.check_pi(x, pis[x])
+        Li = pis[x] + d - 1                       # li(2) = 1.045...
+        B = schoenfeld_bound(x)
+        # Li(x) is summed in long double: within a unit up to 1e20, within
+        # x / 2^60 (a few hundred) above
+        margin = 4 + (x >> 60)
+        for side in (1, -1):
+            edge = Li + side * int(B)
+            pc.check_pi(x, edge - side * margin)
+            with pytest.raises(pc.PrimecountError, match="result check failed") as e:
+                pc.check_pi(x, edge + side * margin)
+            assert e.value.status == pc.E_INTERNAL
+    # below 2657 nothing is checked (as the reference)
+    pc.check_pi(2656, 10 ** 6)
+
+
+def test_result_check_catches_a_wrong_term(pc):
+    """pi(x) assembled from the reference's printed Gourdon terms passes; the
+    same sum with one term off by more than the bound fails, and a term off by
+    less than it is (as in the reference) not detected."""
+    for r in golden("gourdon_terms.json")["rows"]:
+        x = int(r["x"])
+        t = {k: int(r[k]) for k in ("AC", "B", "Phi0", "Sigma", "D")}
+        pix = t["AC"] - t["B"] + t["Phi0"] + t["Sigma"] + t["D"]
+        assert pix == int(r["pi"])
+        pc.check_pi(x, pix)
+        B = int(schoenfeld_bound(x))
+        for term in ("D", "AC"):
+            wrong = dict(t)
+            wrong[term] += 3 * B
+            bad = wrong["AC"] - wrong["B"] + wrong["Phi0"] + wrong["Sigma"] + wrong["D"]
+            with pytest.raises(pc.PrimecountError, match="result check failed"):
+                pc.check_pi(x, bad)
+        pc.check_pi(x, pix + B // 100)
+
+
+def test_status_codes_are_typed(pc):
+    """Argument errors return PCG_E_INVALID, even when the message speaks of
+    the device; without a GPU the engine returns PCG_E_DEVICE."""
+    with pytest.raises(pc.PrimecountError) as e:
+        pc.set_segment_words(1000)
+    assert e.value.status == pc.E_INVALID
+    with pytest.raises(pc.PrimecountError) as e:
+        pc.nth_prime(0)
+    assert e.value.status == pc.E_INVALID
+    if not pc.device_available():
+        with pytest.raises(pc.PrimecountError) as e:
+            pc.D(10 ** 12, 76500, 76500, 8)
+        assert e.value.status == pc.E_DEVICE
