@@ -73,6 +73,19 @@ int pcg_S2_hard_i64(int64_t x, int64_t y, int64_t z, int64_t c, int64_t* out);
  *           (src/deleglise-rivat/S2_hard.cpp:252-293). */
 int pcg_S2_hard_i128(pcg_i128 x, int64_t y, int64_t z, int64_t c, pcg_i128* out);
 
+/* Gourdon's D(x, y, z, k) for x <= 2^63 - 1, as pcg_D_i64, without
+ * allocating: no hipMalloc, hipHostMalloc, hipFree or other allocating
+ * device routine.  Every table and work buffer of the call lives in a
+ * static device array of 64 MiB per GPU (held until the process ends), one
+ * kernel pass over all segments; a query that needs more (D(x) with the
+ * default parameters fits up to about x = 1e15) fails with PCG_E_INVALID.
+ * The array keeps the last call's tables (rebuilt when a query needs larger
+ * ones); calls are serialised by the library's lock, so they never overlap
+ * on one GPU within a process.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (HIP graph capture, tight
+ * serving loops, harnesses that forbid allocation). */
+int pcg_D_i64_noalloc(int64_t x, int64_t y, int64_t z, int64_t k, int64_t* out);
+
 /* D(x) with primecount's default (y, z, k): the CLI's `primecount x --D`.
  * Replaces: maxint_t D(maxint_t x, int threads)  src/app/main.cpp:205-246. */
 int pcg_D(pcg_i128 x, pcg_i128* out);
@@ -168,6 +181,9 @@ typedef struct pcg_stats {
                                 after first-time removals)                    */
   uint64_t queue_steps;      /* steps counting queued factor-table leaves
                                 (up to 64 leaves each)                        */
+  uint64_t device_allocations; /* device and pinned host allocations the library
+                                has made in this process so far (at the end of
+                                the call)                                     */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

@@ -73,6 +73,7 @@ class Stats(ctypes.Structure):
         ("group_steps", ctypes.c_uint64),
         ("prefix_builds", ctypes.c_uint64),
         ("queue_steps", ctypes.c_uint64),
+        ("device_allocations", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -130,6 +131,7 @@ def load():
         "pcg_device_available": ([], ctypes.c_int),
         "pcg_D_i64": ([I64, I64, I64, I64, P64], ctypes.c_int),
         "pcg_D_i128": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
+        "pcg_D_i64_noalloc": ([I64, I64, I64, I64, P64], ctypes.c_int),
         "pcg_S2_hard_i64": ([I64, I64, I64, I64, P64], ctypes.c_int),
         "pcg_S2_hard_i128": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_D": ([_I128, PI128], ctypes.c_int),
@@ -210,6 +212,15 @@ def D(x, y, z, k, threads=None):
         _check(load().pcg_D_i64(x, int(y), int(z), int(k), ctypes.byref(out)))
         return out.value
     return D128(x, y, z, k)
+
+
+def D_noalloc(x, y, z, k):
+    """D(x, y, z, k) for x < 2^63 through the no-allocation entry point
+    (pcg_D_i64_noalloc): all device memory from the library's fixed scratch
+    array; queries too large for it raise PrimecountError (E_INVALID)."""
+    out = ctypes.c_int64()
+    _check(load().pcg_D_i64_noalloc(int(x), int(y), int(z), int(k), ctypes.byref(out)))
+    return out.value
 
 
 def D128(x, y, z, k, threads=None):

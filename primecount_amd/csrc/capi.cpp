@@ -115,6 +115,22 @@ int pcg_D_i64(int64_t x, int64_t y, int64_t z, int64_t k, int64_t* out)
   });
 }
 
+int pcg_D_i64_noalloc(int64_t x, int64_t y, int64_t z, int64_t k, int64_t* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    HslQuery q;
+    q.kind = LeafSum::D;
+    q.x = x;
+    q.y = y;
+    q.z = z;
+    q.k = k;
+    pcg_stats st{};
+    *out = (int64_t) hsl_compute_noalloc(q, &st);
+    t_stats = st;
+  });
+}
+
 int pcg_D_i128(pcg_i128 x, int64_t y, int64_t z, int64_t k, pcg_i128* out)
 {
   return guarded([&] {

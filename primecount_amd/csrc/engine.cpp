@@ -18,6 +18,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
 #include <mutex>
@@ -47,6 +48,10 @@ double now()
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+/// Device and pinned host allocations made by the library in this process
+/// (reported in pcg_stats; the no-allocation entry point leaves it unchanged).
+std::atomic<uint64_t> g_allocations{0};
+
 struct DevBuf {
   void* p = nullptr;
   size_t cap = 0;
@@ -58,6 +63,7 @@ struct DevBuf {
       p = nullptr;
       cap = 0;
       hipError_t e = hipMalloc(&p, bytes);
+      g_allocations++;
       if (e != hipSuccess) {
         (void) hipGetLastError();
         throw pcg_error("device allocation of " + std::to_string(bytes) + " bytes failed",
@@ -87,6 +93,7 @@ struct HostBuf {
       p = nullptr;
       cap = 0;
       hipError_t e = hipHostMalloc(&p, bytes, hipHostMallocDefault);
+      g_allocations++;
       if (e != hipSuccess) {
         (void) hipGetLastError();
         throw pcg_error("pinned host allocation of " + std::to_string(bytes) + " bytes failed",
@@ -115,6 +122,7 @@ struct Device {
   DevBuf tc, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans, groups;
   DevBuf esc_keys, esc_vals, esc_ctl;
   DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
+  char* scratch = nullptr;            // the device's scratch array (no-allocation entry point)
   uint64_t pix_n = 0;
   DevBuf extra[8];                    // scratch of the other pi(x) terms
   HostBuf stage[2];                   // per pass: spans, columns
@@ -147,6 +155,9 @@ struct Device {
     PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     for (auto& e : staged) PCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     PCG_HIP(hsl_upload_constants());
+    void* sp = nullptr;
+    PCG_HIP(hsl_scratch_address(dev, &sp));
+    scratch = (char*) sp;
     ready = true;
   }
 
@@ -908,6 +919,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->group_steps = counters[9];
     st->prefix_builds = counters[10];
     st->queue_steps = counters[11];
+    st->device_allocations = g_allocations.load();
   }
   return result;
 }
@@ -930,6 +942,215 @@ std::vector<std::vector<size_t>> group_queries(const std::vector<HslQuery>& qs)
 }
 
 } // namespace
+
+namespace {
+
+/// Tables of the no-allocation entry point, at the start of the device's
+/// scratch array; kept while they cover the next query.
+struct ScratchTables {
+  bool valid = false;
+  TableSet T;
+  size_t primes = 0, pi_bits = 0, pi_cnt = 0, factor = 0;   // byte offsets
+  size_t end = 0;
+};
+ScratchTables g_scratch_tables;
+
+/// Bump placement of buffers in the scratch array, 256-byte aligned.
+struct ScratchLayout {
+  size_t used;
+  size_t take(size_t bytes)
+  {
+    const size_t off = (used + 255) & ~(size_t) 255;
+    used = off + std::max<size_t>(bytes, 16);
+    return off;
+  }
+};
+
+std::string mib(size_t bytes)
+{
+  return std::to_string((bytes + (1 << 20) - 1) >> 20) + " MiB";
+}
+
+} // namespace
+
+/// D(x, y, z, k) for one query with every table and work buffer in the
+/// device's scratch array: one pass over all segments, no device or pinned
+/// host allocation.  Calls are serialised by the engine lock (they share the
+/// array, which keeps the last call's tables; everything a call needs zeroed
+/// is cleared by fill kernels).  Queries whose buffers exceed the array are
+/// rejected (kErrInvalid).
+i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
+{
+  std::lock_guard<std::mutex> lock(g_mutex);
+  Device& D = g_dev;
+  D.init();
+  const double t0 = now();
+  if (q.kind != LeafSum::D) throw pcg_error("internal: no-allocation path is for D", kErrInternal);
+  validate(q);
+  if (q.x < 1) return 0;
+  hipStream_t s = D.stream;
+  char* const base = D.scratch;
+  ScratchTables& ST = g_scratch_tables;
+  const TableNeed need = table_need({q});
+  if (!(ST.valid && covers(ST.T, need))) {
+    ST.valid = false;
+    TableSet T = build_tables(need);
+    ScratchLayout L{0};
+    const size_t o_primes = L.take(T.P.primes.size() * 4);
+    const size_t o_bits = L.take(T.P.pi_bits.size() * 8);
+    const size_t o_cnt = L.take(T.P.pi_cnt.size() * 4);
+    const size_t o_fac = L.take(T.factor_n * (T.factor32 ? 4 : 2));
+    if (L.used > kHslScratchBytes)
+      throw pcg_error("the no-allocation entry point holds " + mib(kHslScratchBytes) +
+                      " of device scratch; the tables of this query need " + mib(L.used));
+    PCG_HIP(hipMemcpyAsync(base + o_primes, T.P.primes.data(), T.P.primes.size() * 4,
+                           hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(base + o_bits, T.P.pi_bits.data(), T.P.pi_bits.size() * 8,
+                           hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(base + o_cnt, T.P.pi_cnt.data(), T.P.pi_cnt.size() * 4,
+                           hipMemcpyHostToDevice, s));
+    PCG_HIP(hsl_launch_factor_table(base + o_fac, T.factor32, true, (uint64_t) T.ft_limit,
+                                    (uint64_t) T.ft_y, (const uint32_t*) (base + o_primes),
+                                    (uint32_t) T.P.primes.size(), s));
+    PCG_HIP(hipStreamSynchronize(s));
+    ST.T = std::move(T);
+    ST.primes = o_primes;
+    ST.pi_bits = o_bits;
+    ST.pi_cnt = o_cnt;
+    ST.factor = o_fac;
+    ST.end = L.used;
+    ST.valid = true;
+  }
+  const TableSet& T = ST.T;
+  const double t_tab = now();
+
+  QueryPlan qp = plan_query(q, T.P);
+  if (qp.empty) return 0;
+  const uint32_t W = auto_segment_words(qp.qd.limit + 1, T.kind);
+  if (W < 128 || W > 4096 || (W % 128) != 0)
+    throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
+  const uint32_t cbits = std::max(slot_format(W).cbits, engine_config().slot_cbits);
+  if (cbits > 28) throw pcg_error("slot bits must be at most 28");
+  plan_reach(qp, q, T.P, W, cbits);
+  if (qp.nseg == 0) return 0;
+
+  // one pass: every segment of the query, one column per b
+  const uint64_t slots = qp.F(qp.nseg);
+  const uint32_t nitems = qp.nseg;
+  const uint32_t ncols = qp.nbs(0);
+  const uint32_t max_n = qp.reach[0];
+  const uint32_t slice_len = std::max<uint32_t>(1, (max_n + 127) / 128);
+  const uint32_t nslices = (max_n + slice_len - 1) / slice_len;
+  const uint64_t bound = escape_bound(qp, 0, qp.nseg);
+  uint64_t cap = 1024;
+  while (cap < 2 * bound + 2) cap *= 2;
+  const size_t nb = qp.binfo.size();
+  ScratchLayout L{ST.end};
+  const size_t o_qd = L.take(sizeof(QueryDesc)), o_binfo = L.take(nb * sizeof(BInfo));
+  const size_t o_reach = L.take(nb * 4), o_sfx = L.take((nb + 1) * 8);
+  const size_t o_span = L.take(sizeof(PassSpan)), o_items = L.take((size_t) nitems * sizeof(SegItem));
+  const size_t o_cols = L.take((size_t) ncols * sizeof(CombineCol));
+  const size_t o_slots = L.take(slots * 4);
+  const size_t o_ekeys = L.take(cap * 8), o_evals = L.take(cap * 4), o_ectl = L.take(8);
+  const size_t part = (size_t) nslices * ncols;
+  const size_t o_sc = L.take(part * 8), o_sm = L.take(part * 8), o_smp = L.take(part * 16);
+  const size_t o_carry = L.take((size_t) ncols * 8), o_out = L.take((size_t) ncols * 16);
+  const size_t o_sacc = L.take(16), o_cnt = L.take(128), o_queue = L.take(4);
+  if (L.used > kHslScratchBytes || slots > ((uint64_t) 1 << 32))
+    throw pcg_error("the no-allocation entry point holds " + mib(kHslScratchBytes) +
+                    " of device scratch; D(x, y, z, k) at x = " + to_string128(q.x) + " needs " +
+                    mib(L.used) + " (" + std::to_string(slots) + " slots)");
+
+  QueryDesc qd = qp.qd;
+  qd.binfo_off = 0;
+  qd.reach_off = 0;
+  qd.sfx_off = 0;
+  PassSpan sp{};
+  sp.query = 0;
+  sp.s0 = 0;
+  sp.s1 = qp.nseg;
+  sp.item_base = 0;
+  sp.t_base = 0;
+  std::vector<CombineCol> cols(ncols);
+  for (uint32_t br = 0; br < ncols; br++) cols[br] = CombineCol{br, 0, qp.reach[br], br};
+  PCG_HIP(hipMemcpyAsync(base + o_qd, &qd, sizeof(qd), hipMemcpyHostToDevice, s));
+  PCG_HIP(hipMemcpyAsync(base + o_binfo, qp.binfo.data(), nb * sizeof(BInfo), hipMemcpyHostToDevice, s));
+  PCG_HIP(hipMemcpyAsync(base + o_reach, qp.reach.data(), nb * 4, hipMemcpyHostToDevice, s));
+  PCG_HIP(hipMemcpyAsync(base + o_sfx, qp.sfx.data(), (nb + 1) * 8, hipMemcpyHostToDevice, s));
+  PCG_HIP(hipMemcpyAsync(base + o_span, &sp, sizeof(sp), hipMemcpyHostToDevice, s));
+  PCG_HIP(hipMemcpyAsync(base + o_cols, cols.data(), (size_t) ncols * sizeof(CombineCol),
+                         hipMemcpyHostToDevice, s));
+  // the array keeps the previous call's contents: clear what starts at zero
+  PCG_HIP(hsl_launch_fill_zero(base + o_ekeys, cap * 8, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_ectl, 8, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_carry, (size_t) ncols * 8, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_out, (size_t) ncols * 16, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_sacc, 16, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_cnt, 128, s));
+  PCG_HIP(hsl_launch_fill_zero(base + o_queue, 4, s));
+  PCG_HIP(hsl_launch_build_items((const QueryDesc*) (base + o_qd), (const PassSpan*) (base + o_span),
+                                 1, (const uint32_t*) (base + o_reach),
+                                 (const uint64_t*) (base + o_sfx), (SegItem*) (base + o_items),
+                                 nitems, s));
+  int bpc = 0;
+  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
+  bpc = std::max(bpc, 1);
+  HslArgs a{};
+  a.queries = (const QueryDesc*) (base + o_qd);
+  a.items = (const SegItem*) (base + o_items);
+  a.groups = nullptr;
+  a.binfo = (const BInfo*) (base + o_binfo);
+  a.primes = (const uint32_t*) (base + ST.primes);
+  a.factor = base + ST.factor;
+  a.pi_bits = (const uint64_t*) (base + ST.pi_bits);
+  a.pi_cnt = (const uint32_t*) (base + ST.pi_cnt);
+  a.T_S = (uint32_t*) (base + o_slots);
+  a.cbits = cbits;
+  a.esc.keys = (unsigned long long*) (base + o_ekeys);
+  a.esc.vals = (int32_t*) (base + o_evals);
+  a.esc.ctl = (unsigned int*) (base + o_ectl);
+  a.esc.cap = (uint32_t) cap;
+  a.S_acc = (unsigned long long*) (base + o_sacc);
+  a.counters = (unsigned long long*) (base + o_cnt);
+  a.queue = (unsigned int*) (base + o_queue);
+  a.nitems = nitems;
+  a.W = W;
+  const uint32_t grid = (uint32_t) std::min<uint64_t>((uint64_t) D.cus * bpc, nitems);
+  PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
+  PCG_HIP(hsl_launch_combine((const CombineCol*) (base + o_cols), ncols, a.items, a.T_S, a.esc,
+                             cbits, slice_len, nslices, (long long*) (base + o_sc),
+                             (long long*) (base + o_sm), base + o_smp,
+                             (long long*) (base + o_carry), base + o_out, s));
+  std::vector<i128> col_out(ncols);
+  uint64_t sacc[2] = {}, counters[12] = {};
+  uint32_t ectl[2] = {};
+  PCG_HIP(hipMemcpyAsync(col_out.data(), base + o_out, (size_t) ncols * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(sacc, base + o_sacc, 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(counters, base + o_cnt, sizeof(counters), hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(ectl, base + o_ectl, 8, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipStreamSynchronize(s));
+  if (ectl[1] != 0)
+    throw pcg_error("internal: escape table overflow", kErrInternal);
+  i128 r = (i128) (((u128) sacc[1] << 64) | sacc[0]);
+  for (const i128& v : col_out) r += v;
+  if (stats) {
+    *stats = pcg_stats{};
+    stats->seconds_total = now() - t0;
+    stats->seconds_tables = t_tab - t0;
+    stats->segments = nitems;
+    stats->chunks = nitems;
+    stats->segment_words = W;
+    stats->sieve_limit = qp.qd.limit;
+    stats->b_first = qp.qd.b_first;
+    stats->b_last = qp.qd.b_last;
+    stats->leaves = counters[0];
+    stats->hits = counters[1];
+    stats->factor_entries = counters[2];
+    stats->pairs = counters[3];
+    stats->device_allocations = g_allocations.load();
+  }
+  return r;
+}
 
 i128 hsl_compute(const HslQuery& q, pcg_stats* stats)
 {
@@ -977,6 +1198,7 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.prefix_builds += st.prefix_builds;
     acc.queue_steps += st.queue_steps;
   }
+  acc.device_allocations = g_allocations.load();
   acc.seconds_total = now() - t0;
   if (stats) *stats = acc;
   return out;

@@ -36,6 +36,11 @@ EngineConfig& engine_config();
 /// Computes one query on the device; fills `stats`.  Throws pcg_error.
 i128 hsl_compute(const HslQuery& q, pcg_stats* stats);
 
+/// Computes one D query with all its device memory in the scratch array of
+/// the device (no device or pinned host allocation).  Throws pcg_error
+/// (kErrInvalid) when the query does not fit the array.
+i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats);
+
 /// Computes many D/S2_hard queries with default or explicit parameters in one
 /// persistent launch per table set.  max_slots overrides the pass slot budget
 /// (0 = engine setting).  Throws pcg_error.

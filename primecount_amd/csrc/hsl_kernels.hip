@@ -42,6 +42,13 @@ namespace pcg {
 __constant__ int16_t c_cop_index[2310];   // largest index of a coprime <= r
 __constant__ uint16_t c_cop[480];         // coprimes to 2310 in [0, 2310)
 
+/// Device memory of the no-allocation entry point (pcg_D_i64_noalloc): its
+/// tables and work buffers, kHslScratchBytes per device for the life of the
+/// process.  Host code never passes `scratch` itself (in host code the name is
+/// a stand-in, not a device address); it uses the address that
+/// hipGetSymbolAddress returns for the device (hsl_scratch_address).
+static __device__ __align__(256) unsigned char scratch[kHslScratchBytes];
+
 namespace {
 
 
@@ -1124,6 +1131,21 @@ hipError_t hsl_launch_factor_table(void* out, bool factor32, bool d_table, uint6
                          limit, y, primes, nprimes);
   }
   return hipGetLastError();
+}
+
+hipError_t hsl_scratch_address(int device, void** ptr)
+{
+  // one address per device: each GPU holds its own copy of the array
+  static void* addr[kHslMaxDevices] = {};
+  if (device < 0 || device >= kHslMaxDevices) return hipErrorInvalidDevice;
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[device] = p;
+  }
+  *ptr = addr[device];
+  return hipSuccess;
 }
 
 hipError_t hsl_upload_constants()

@@ -162,8 +162,17 @@ inline uint32_t hsl_lds_bytes(uint32_t W)
        + 240;             // sieve bits of the offsets <= r in a word, r < 240
 }
 
+/// Bytes of the device scratch array of the no-allocation entry point: it
+/// holds every table and work buffer of one D(x) call (D(1e15) with the
+/// default parameters needs about 30 MB).
+constexpr size_t kHslScratchBytes = (size_t) 64 << 20;
+constexpr int kHslMaxDevices = 64;
+
 // Host-side launch wrappers (defined in hsl_kernels.hip).
 hipError_t hsl_upload_constants();
+/// Device address of the scratch array on `device` (the current device),
+/// looked up with hipGetSymbolAddress on the first call for that device.
+hipError_t hsl_scratch_address(int device, void** ptr);
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
 hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
