@@ -169,12 +169,49 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
   return v;
 }
 
+/// build_pair_prefix for PER = W / 128 pairs per lane known at compile time:
+/// the lane's PER 128-bit loads are issued together, then counted.
+template <int PER>
+__device__ __forceinline__ void build_pair_prefix_n(const uint32_t* bm, uint32_t* pre2,
+                                                    uint32_t lane)
+{
+  const uint4* w4 = reinterpret_cast<const uint4*>(bm) + lane * PER;
+  uint4 q[PER];
+#pragma unroll
+  for (int i = 0; i < PER; i++)
+    q[i] = w4[i];
+  uint32_t c[PER];
+  uint32_t tot = 0;
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    c[i] = __popc(q[i].x) + __popc(q[i].y) + __popc(q[i].z) + __popc(q[i].w);
+    tot += c[i];
+  }
+  uint32_t run = wave_incl_scan(tot) - tot;
+  uint32_t* out = pre2 + lane * PER;
+#pragma unroll
+  for (int i = 0; i < PER; i++) {
+    out[i] = run;
+    run += c[i];
+  }
+}
+
 /// Prefix counts of the segment at word-pair granularity: pre2[g] = unsieved
 /// numbers in words [0, 2 g).  Lane l takes W / 128 consecutive pairs (one
-/// 128-bit LDS load each), then a wave scan gives every lane its base.
+/// 128-bit LDS load each), then a wave scan gives every lane its base.  The
+/// segment sizes up to 640 words take a copy specialised for their pair
+/// count (wave-uniform switch).
 __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* pre2, uint32_t W,
                                                   uint32_t lane)
 {
+  switch (W >> 7) {
+    case 1: build_pair_prefix_n<1>(bm, pre2, lane); return;
+    case 2: build_pair_prefix_n<2>(bm, pre2, lane); return;
+    case 3: build_pair_prefix_n<3>(bm, pre2, lane); return;
+    case 4: build_pair_prefix_n<4>(bm, pre2, lane); return;
+    case 5: build_pair_prefix_n<5>(bm, pre2, lane); return;
+    default: break;
+  }
   const uint32_t per = W >> 7;
   const uint4* w4 = reinterpret_cast<const uint4*>(bm) + lane * per;
   uint32_t* out = pre2 + lane * per;
@@ -386,13 +423,27 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
     lut[r] = (uint8_t) (8 * (r / 30) + le_count(r % 30));
 
   const uint64_t SEG = 240ull * W;
-  unsigned long long n_slots = 0, u_hits = 0;
-  unsigned long long u_leaves = 0, u_entries = 0;   // wave-uniform
+  // Wave-uniform work counts, 32-bit so that they take one SGPR each; they
+  // are added to the 64-bit totals whenever one passes 2^30 (checked after
+  // each work item, which adds far less than 2^31) and at the end.
+  uint32_t n_slots = 0, u_hits = 0, u_leaves = 0, u_entries = 0;
   // wave-level step counts: windows, primes with leaves, leaf-loop steps,
   // one-at-a-time cross-off primes and steps, grouped cross-off steps,
   // pair-prefix builds
-  unsigned long long u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0,
-                     u_gsteps = 0, u_pbuilds = 0, u_qsteps = 0;
+  uint32_t u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0, u_gsteps = 0,
+           u_pbuilds = 0, u_qsteps = 0;
+  auto flush_counters = [&]() {
+    if (lane == 0) {
+      const uint32_t v[12] = {u_leaves, u_hits, u_entries, n_slots, u_win, u_lprimes,
+                              u_lsteps, u_dprimes, u_dsteps, u_gsteps, u_pbuilds, u_qsteps};
+#pragma unroll
+      for (int i = 0; i < 12; i++)
+        if (v[i])
+          atomicAdd(&counters[i], (unsigned long long) v[i]);
+    }
+    n_slots = u_hits = u_leaves = u_entries = 0;
+    u_win = u_lprimes = u_lsteps = u_dprimes = u_dsteps = u_gsteps = u_pbuilds = u_qsteps = 0;
+  };
 
   for (;;) {
     uint32_t qi = 0;
@@ -480,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
       uint32_t r_lo[NQ], r_hi[NQ], r_T[NQ];
       uint64_t r_xlo[NQ];
       bool brk[NQ];
-      uint32_t r_nq = 0, r_qm = 0, r_off0 = 0, r_h = 0, r_p = 0;
+      uint32_t r_nq = 0, r_h = 0, r_p = 0;
       bool par = false;
       uint4 prm = make_uint4(0, 0, 0, 0);
 #pragma unroll
@@ -527,8 +578,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         const uint64_t q_lo = div_exact_r(low_d, low, p, inv_p) + 1;
         const uint64_t q_hi = div_exact_r(last_d, high - 1, p, inv_p);
         r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
-        r_qm = (uint32_t) (q_lo % 30);
-        r_off0 = (uint32_t) (p * q_lo - low);
+        const uint32_t qm = (uint32_t) (q_lo % 30);
+        const uint32_t off0 = (uint32_t) (p * q_lo - low);   // < 2^20 when nq > 0
         // Every cofactor is below p^2: a number of the segment that is still
         // unsieved when p is crossed off and is a multiple of p is p * q with
         // q = 1 or q prime, and its least prime factor is min(p, q).
@@ -536,13 +587,16 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         if (r_nq > 0 && p >= 7) {
           // hits: cofactors coprime to 30; c0 = wheel residues below q_lo % 30
           // (2, 3 and 5 are not in the wheel and have no hits)
-          const uint32_t c0 = r_qm ? le_count(r_qm - 1) : 0u;
-          const uint32_t v = r_qm + r_nq - 1;
+          const uint32_t c0 = qm ? le_count(qm - 1) : 0u;
+          const uint32_t v = qm + r_nq - 1;
           const uint32_t vb = v / 30;
           r_h = 8 * vb + le_count(v - 30 * vb) - c0;
-          prm.y = r_off0 | (r_qm << 20) | (c0 << 25);   // off0 < 2^20 when nq > 0
+          // off0 = 30 b0 + cc0: the multiple p (q_lo + delta) lies in byte
+          // b0 + a delta + (cc0 + rp delta) / 30 of the segment (p = 30 a + rp)
+          const uint32_t b0 = off0 / 30;
+          prm.y = b0 | ((off0 - 30 * b0) << 16) | (qm << 21) | (c0 << 26);
         }
-        prm.x = p;
+        prm.x = (p / 30) | ((p % 30) << 21);            // p < 2^25: a < 2^21
         prm.z = q_lo < p ? (uint32_t) (p - q_lo) : 0u;
       }
       // per member: its window end and its primes with factor entries
@@ -612,27 +666,41 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           u_dprimes++;
           uint32_t removed = 0;                         // wave-uniform
           if (nq > 0 && p >= 7) {
-            const uint32_t qm = readlane_u32(r_qm, j);
-            const uint32_t off0 = readlane_u32(r_off0, j);
+            const uint32_t px = readlane_u32(prm.x, j);
+            const uint32_t py = readlane_u32(prm.y, j);
+            const uint32_t a = px & 0x1FFFFFu, rp = px >> 21;
+            const uint32_t qm = (py >> 21) & 31u;
             const uint32_t cls = lane & 7;
-            const uint32_t r = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
-            const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;
+            const uint32_t r = wheel_residue(cls);
+            const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;   // < 30
             const uint32_t n = delta < nq ? (nq - 1 - delta) / 30 + 1 : 0u;   // class count
-            const uint32_t off = off0 + p * delta;
-            const uint32_t byte0 = off / 30;
-            const uint32_t bib = wheel_bit(off - byte0 * 30);
+            // byte and bit of the class's first multiple p (q_lo + delta):
+            // t = cc0 + rp delta < 871, so t / 30 = (t 2185) >> 16
+            const uint32_t t = ((py >> 16) & 31u) + __umul24(rp, delta);
+            const uint32_t tq = (t * 2185u) >> 16;
             const uint32_t j0 = lane >> 3;
+            // lane j0 takes multiples j0, j0 + 8, ... of its class: p bytes
+            // apart per multiple, so 8 p bytes (2 p words) per step, and the
+            // bit within the word does not change
+            const uint32_t B0 = (py & 0xFFFFu) + __umul24(a, delta) + tq + j0 * p;
+            const uint32_t mask = 1u << (((B0 & 3) << 3) | wheel_bit(t - 30 * tq));
+            const uint32_t nmine = n > j0 ? (n - j0 + 7) >> 3 : 0u;
             const uint32_t nmax = (nq + 29) / 30;       // >= every class count
-            u_dsteps += (nmax + 7) / 8;
-            for (uint32_t t0 = 0; t0 < nmax; t0 += 8) {
-              const uint32_t t = t0 + j0;
-              bool first = false;
-              if (t < n) {
-                const uint32_t B = byte0 + t * p;
-                const uint32_t mask = 1u << (((B & 3) << 3) | bib);
-                first = (atomicAnd(&bm[B >> 2], ~mask) & mask) != 0;
-              }
-              removed += (uint32_t) __popcll(__ballot(first));
+            const uint32_t nsteps = (nmax + 7) / 8;
+            u_dsteps += nsteps;
+            const uint32_t wstep = 2 * p;
+            uint32_t wi = B0 >> 2;
+            // two multiples per lane per step: both atomics are in flight
+            // together (one prime's removals may be counted in any order)
+            for (uint32_t u = 0; u < nsteps; u += 2) {
+              uint32_t o0 = 0, o1 = 0;
+              if (u < nmine)
+                o0 = atomicAnd(&bm[wi], ~mask);
+              if (u + 1 < nmine)
+                o1 = atomicAnd(&bm[wi + wstep], ~mask);
+              removed += (uint32_t) __popcll(__ballot((o0 & mask) != 0)) +
+                         (uint32_t) __popcll(__ballot((o1 & mask) != 0));
+              wi += 2 * wstep;
             }
           }
           dirty |= removed != 0;
@@ -654,13 +722,16 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
           const uint32_t kk = lane & (G - 1);
           const bool has = jo < r1 && kk < pp.w;
-          const uint32_t idx = (pp.y >> 25) + kk;
-          const uint32_t delta = 30 * (idx >> 3) +
-                                 (uint32_t) ((kWheelBytes >> (8 * (idx & 7))) & 0xff) -
-                                 ((pp.y >> 20) & 31);
-          const uint32_t off = (pp.y & 0xFFFFFu) + pp.x * delta;
-          const uint32_t B = off / 30;
-          const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(off - B * 30));
+          // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
+          // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
+          // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
+          const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
+          const uint32_t idx = (pp.y >> 26) + kk;
+          const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
+          const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
+          const uint32_t tq = (t * 2185u) >> 16;
+          const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
+          const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(t - 30 * tq));
           uint32_t old = 0;
           if (!ordered) {
             if (has)
@@ -677,7 +748,7 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           if (fresh) {
             uint32_t attr = jo;
             if (!ordered && delta < pp.z) {             // q = q_lo + delta < p
-              const uint32_t q = pp.x - pp.z + delta;
+              const uint32_t q = 30 * a + rp - pp.z + delta;
               if (q > 1)
                 attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
             }
@@ -844,23 +915,13 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
           atomicAdd(&acc[1], inc);
       }
     }
+    if ((n_slots | u_hits | u_leaves | u_entries | u_win | u_lprimes | u_lsteps | u_dprimes |
+         u_dsteps | u_gsteps | u_pbuilds | u_qsteps) >= (1u << 30))
+      flush_counters();
   }
 
   // the u_* counters are updated in wave-uniform control flow only
-  if (lane == 0) {
-    atomicAdd(&counters[0], u_leaves);
-    atomicAdd(&counters[1], u_hits);
-    atomicAdd(&counters[2], u_entries);
-    atomicAdd(&counters[3], n_slots);
-    atomicAdd(&counters[4], u_win);
-    atomicAdd(&counters[5], u_lprimes);
-    atomicAdd(&counters[6], u_lsteps);
-    atomicAdd(&counters[7], u_dprimes);
-    atomicAdd(&counters[8], u_dsteps);
-    atomicAdd(&counters[9], u_gsteps);
-    atomicAdd(&counters[10], u_pbuilds);
-    atomicAdd(&counters[11], u_qsteps);
-  }
+  flush_counters();
 }
 
 template <typename FT, int NQ>

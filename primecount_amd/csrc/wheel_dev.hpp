@@ -9,50 +9,38 @@
 
 namespace pcg {
 
-/// Nibble table: number of wheel residues {1,7,11,...,29} that are <= r.
-constexpr uint64_t le_nibbles(int first)
-{
-  uint64_t v = 0;
-  for (int r = first; r < first + 16 && r < 30; r++) {
-    const int w[8] = {1, 7, 11, 13, 17, 19, 23, 29};
-    int k = 0;
-    for (int i = 0; i < 8; i++)
-      k += w[i] <= r;
-    v |= (uint64_t) k << (4 * (r - first));
-  }
-  return v;
-}
-constexpr uint64_t kLeLo = le_nibbles(0);
-constexpr uint64_t kLeHi = le_nibbles(16);
+/// Bit r set for the residues r in [0, 30) coprime to 30.
+constexpr uint32_t kCop30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) |
+                            (1u << 17) | (1u << 19) | (1u << 23) | (1u << 29);
 
 /// Number of wheel residues {1, 7, 11, ..., 29} that are <= r (r < 30).
 __device__ __forceinline__ uint32_t le_count(uint32_t r)
 {
-  return (uint32_t) ((r < 16 ? (kLeLo >> (4 * r)) : (kLeHi >> (4 * (r - 16)))) & 15);
+  return (uint32_t) __popc(kCop30 & ((2u << r) - 1u));
 }
 
-/// Bits of a sieve word that stand for the offsets <= r (r < 240).
+/// Bits of a sieve word that stand for the offsets <= r (r < 240): the low
+/// 8 (r / 30) + le_count(r % 30) bits.
 __device__ __forceinline__ uint64_t le_mask(uint32_t r)
 {
-  uint32_t j = r / 30;
-  uint32_t rem = r - 30 * j;
-  uint32_t k = (uint32_t) ((rem < 16 ? (kLeLo >> (4 * rem)) : (kLeHi >> (4 * (rem - 16)))) & 15);
-  uint64_t below = (1ull << (8 * j)) - 1;
-  return below | ((((uint64_t) 1 << k) - 1) << (8 * j));
+  const uint32_t j = r / 30;
+  const uint32_t nb = 8 * j + le_count(r - 30 * j);
+  return nb >= 64 ? ~0ull : (1ull << nb) - 1;
 }
 
 /// Residues coprime to 30 in bit order, packed as bytes: 1, 7, 11, ..., 29.
 constexpr uint64_t kWheelBytes = 0x1D1713110D0B0701ull;
 
-/// Bit r set for the residues r in [0, 30) coprime to 30.
-constexpr uint32_t kCop30 = (1u << 1) | (1u << 7) | (1u << 11) | (1u << 13) |
-                            (1u << 17) | (1u << 19) | (1u << 23) | (1u << 29);
-
-/// Bit (0..7) of a wheel residue r in a sieve byte.
+/// Bit (0..7) of a wheel residue r in a sieve byte: the residues below r.
 __device__ __forceinline__ uint32_t wheel_bit(uint32_t r)
 {
-  uint32_t k = (uint32_t) ((r < 16 ? (kLeLo >> (4 * r)) : (kLeHi >> (4 * (r - 16)))) & 15);
-  return k - 1;
+  return (uint32_t) __popc(kCop30 & ((1u << r) - 1u));
+}
+
+/// The i-th wheel residue (i < 8): byte i of kWheelBytes.
+__device__ __forceinline__ uint32_t wheel_residue(uint32_t i)
+{
+  return (uint32_t) ((kWheelBytes >> (8 * i)) & 0xff);
 }
 
 /// floor(xp / d) for a quotient < 2^50 and d < 2^53.  xd is xp rounded to
