@@ -768,18 +768,26 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
         // hold no leaf (every entry <= T, a quarter of the primes with
         // entries at 1e19) are passed over, and the cross-offs stay pending
         uint32_t active = 0;
+        // the first step of each member's leaf range: its factor entries (or
+        // primes), loaded here so that the loads are in flight during the
+        // cross-offs and the prefix build
+        uint32_t first[NQ];
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
+          first[k] = 0;
           if (!((lmask_m[k] >> j) & 1))
             continue;
           const uint32_t lo = readlane_u32(r_lo[k], j);
           const uint32_t hi = readlane_u32(r_hi[k], j);
           const uint32_t Tt = readlane_u32(r_T[k], j);
+          const uint32_t idx = lo + 1 + lane;
           bool has = true;
-          if ((Tt >> 31) && hi - lo <= (uint32_t) kBlock) {
-            const uint32_t idx = lo + 1 + lane;
-            const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
-            has = __ballot(f > (Tt & 0x7fffffffu)) != 0;
+          if (Tt >> 31) {
+            first[k] = idx <= hi ? (uint32_t) factor[idx] : 0u;
+            if (hi - lo <= (uint32_t) kBlock)
+              has = __ballot(first[k] > (Tt & 0x7fffffffu)) != 0;
+          } else {
+            first[k] = idx <= hi ? primes[idx] : 0u;
           }
           if (has)
             active |= 1u << k;
@@ -812,19 +820,16 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             u_entries += hi - lo;
             u_lsteps += (hi - lo + kBlock - 1) / kBlock;
             // The leaves (about a third of the entries) are queued in LDS as
-            // factor index | sign << 31 and counted 64 at a time, so that the
-            // quotient and the count run on full waves (189 M counting steps
-            // instead of 427 M at x = 1e19).  Indexes are below 2^31 (the
-            // table covers numbers below 2^32).  qh, qt: head and tail of the
-            // queue (mod 128), wave-uniform.
+            // (m >> 1) | sign << 31 (m is odd and below 2^32) and counted 64
+            // at a time, so that the quotient and the count run on full waves
+            // (189 M counting steps instead of 427 M at x = 1e19).  qh, qt:
+            // head and tail of the queue (mod 128), wave-uniform.
             uint32_t qh = 0, qt = 0;
             auto drain = [&](uint32_t n) {
               wave_lds_sync();
               if (lane < n) {
                 const uint32_t e = lq[(qt + lane) & 127];
-                const uint32_t idx = e & 0x7fffffffu;
-                const uint32_t qq = idx / 480;
-                const uint32_t m = 2310 * qq + c_cop[idx - qq * 480];
+                const uint32_t m = (e << 1) | 1u;
                 const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
                 S[k] += (e >> 31) ? (long long) cnt : -(long long) cnt;
               }
@@ -832,10 +837,16 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               qt += n;
               u_qsteps++;
             };
+            // entries and coprime residues of the next step are loaded
+            // before this step is filtered, queued and counted
+            uint32_t f = first[k];
+            uint32_t cop = c_cop[(lo + 1 + lane) % 480];
 #pragma nounroll
             for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
               const uint32_t idx = base + lane;
-              const uint32_t f = idx <= hi ? (uint32_t) factor[idx] : 0u;
+              const uint32_t nidx = idx + kBlock;
+              const uint32_t f_next = nidx <= hi ? (uint32_t) factor[nidx] : 0u;
+              const uint32_t cop_next = c_cop[nidx % 480];
               const bool leaf = f > T;
               const bool pos = leaf && (f & 1);
               const unsigned long long lb = __ballot(leaf);
@@ -846,11 +857,14 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
               if (leaf) {
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
-                lq[(qh + rank) & 127] = idx | (pos ? 0x80000000u : 0u);
+                const uint32_t m = 2310 * (idx / 480) + cop;
+                lq[(qh + rank) & 127] = (m >> 1) | (pos ? 0x80000000u : 0u);
               }
               qh += nl;
               if (qh - qt >= 64)
                 drain(64);
+              f = f_next;
+              cop = cop_next;
             }
             if (qh != qt)
               drain(qh - qt);
@@ -859,11 +873,14 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
             u_leaves += hi - lo;
             u_lsteps += (hi - lo + kBlock - 1) / kBlock;
             M = (int) (hi - lo);
+            uint32_t q = first[k];
 #pragma nounroll
             for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
               const uint32_t idx = base + lane;
+              const uint32_t q_next = idx + kBlock <= hi ? primes[idx + kBlock] : 0u;
               if (idx <= hi)
-                S[k] += seg_count(bm, pre2, lut, leaf_stop(xlo, low, primes[idx]));
+                S[k] += seg_count(bm, pre2, lut, leaf_stop(xlo, low, q));
+              q = q_next;
             }
           }
           w_m[k] = lane == j ? M : w_m[k];
