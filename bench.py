@@ -49,19 +49,13 @@ def expected_D(x):
     return None, None
 
 
-def measured_lds_peak():
-    """LDS read bandwidth of this device in GB/s, from the probe kernel built
-    next to the engine (bank-conflict-free 128-bit LDS loads on every CU)."""
-    import ctypes
-    path = os.path.join(REPO, "primecount_amd", "libpcg_lds_probe.so")
-    if not os.path.exists(path):
+def measured_lds_peak(pc):
+    """LDS read bandwidth of this device in GB/s, from the engine library's
+    probe kernel (bank-conflict-free 128-bit LDS loads on every CU)."""
+    try:
+        return pc.lds_read_gbs()
+    except pc.PrimecountError:
         return None
-    lib = ctypes.CDLL(path)
-    lib.pcg_probe_lds_read_gbs.argtypes = [ctypes.POINTER(ctypes.c_double)]
-    v = ctypes.c_double()
-    if lib.pcg_probe_lds_read_gbs(ctypes.byref(v)) != 0:
-        return None
-    return v.value
 
 
 def measure_pi(pc, x):
@@ -129,6 +123,74 @@ def measure_serving(pc, n, rate):
     return {"workload": "first %d queries of the serving mix" % len(qs),
             "arrivals": "Poisson, %.3g/s (seed 0xC5C5)" % rate,
             "plain": plain, "served": served}
+
+
+def measure_batch(pc):
+    """Config 3: the 256 seeded D(x) queries of tests/golden/batch_D.json
+    (x = 10^(12 + 6u), values printed by the reference build) in one
+    persistent launch (pcg_D_batch), median of 3 after one warmup; beside it
+    the same queries one per launch."""
+    path = os.path.join(REPO, "tests", "golden", "batch_D.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        fx = json.load(f)
+    xs = [int(r["x"]) for r in fx["rows"]]
+    want = [int(r["D"]) for r in fx["rows"]]
+    ok = pc.D_batch(xs) == want
+    times, kern = [], []
+    for _ in range(3):
+        t = time.perf_counter()
+        got = pc.D_batch(xs)
+        times.append(time.perf_counter() - t)
+        kern.append(pc.last_stats()["seconds_kernel"])
+        ok = ok and got == want
+    times.sort()
+    lat = []
+    t0 = time.perf_counter()
+    for x, v in zip(xs, want):
+        t = time.perf_counter()
+        ok = ok and pc.D_default(x) == v
+        lat.append(time.perf_counter() - t)
+    plain_s = time.perf_counter() - t0
+    lat.sort()
+    return {"workload": "256 D(x) queries, x = 10^(12+6u) from SplitMix64 seed %s, one "
+                        "persistent launch" % fx.get("seed", "?"),
+            "queries": len(xs), "seconds": round(times[1], 4),
+            "queries_per_s": round(len(xs) / times[1], 1),
+            "kernel_s": round(sorted(kern)[1], 4), "bit_exact": ok,
+            "plain_one_per_launch": {"seconds": round(plain_s, 4),
+                                     "queries_per_s": round(len(xs) / plain_s, 1),
+                                     "p50_ms": round(lat[len(lat) // 2] * 1000, 3),
+                                     "p99_ms": round(lat[int(0.99 * len(lat))] * 1000, 3)}}
+
+
+def measure_int128(pc):
+    """Config 4: D(x) and S2_hard(x) at x = 1e21 (x > 2^64, the 128-bit path)
+    with the reference's default parameters, against the values the
+    reference build printed: the first call (which builds the tables) and
+    the second (tables resident)."""
+    path = os.path.join(REPO, "tests", "golden", "configs_big.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        big = json.load(f)
+    out = {}
+    for name, fn in (("D", pc.D_default), ("S2_hard", pc.S2_hard_default)):
+        r = big[name][0]
+        x, want = int(r["x"]), int(r["value"])
+        t = time.perf_counter()
+        v1 = fn(x)
+        first = time.perf_counter() - t
+        t = time.perf_counter()
+        v2 = fn(x)
+        second = time.perf_counter() - t
+        st = pc.last_stats()
+        out[name] = {"x": str(x), "value": str(v2), "bit_exact": v1 == want and v2 == want,
+                     "seconds_first_call": round(first, 3), "seconds": round(second, 3),
+                     "kernel_s": round(st["seconds_kernel"], 3),
+                     "segment_words": st["segment_words"]}
+    return out
 
 
 def traffic_from_profile(x):
@@ -223,9 +285,9 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=25.0,
                     help="Poisson arrival rate of the served run (queries/s)")
     ap.add_argument("--no-lds-probe", action="store_true",
-                    help="use the nominal LDS peak instead of running the LDS probe kernel "
-                         "(for runs under rocprofv3, where launching the probe library's "
-                         "kernel crashed the host process)")
+                    help="use the nominal LDS peak instead of running the LDS probe kernel")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 256-query batch and the 1e21 (128-bit) measurements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -245,11 +307,13 @@ def main():
     print("bench: D(%d) y=%d z=%d k=%d" % (x, p["y"], p["z"], p["k"]), file=sys.stderr, flush=True)
 
     t = time.time()
+    first = None
     for i in range(max(a.warmup, 1)):
         got = pc.D128(x, p["y"], p["z"], p["k"])
         if want is not None and got != want:
             raise SystemExit("warmup result %d != reference %d" % (got, want))
-    first = pc.last_stats()
+        if first is None:
+            first = pc.last_stats()       # the first call builds the tables
     print("bench: warmup done in %.2f s (tables %.2f s)" % (time.time() - t, first["seconds_tables"]),
           file=sys.stderr, flush=True)
 
@@ -288,7 +352,7 @@ def main():
         "frac": round(hbm_bytes / k_avg / 1e9 / HBM_PEAK_GBS, 6),
         "traffic": traffic,
     }
-    lds_peak = None if a.no_lds_probe else measured_lds_peak()
+    lds_peak = None if a.no_lds_probe else measured_lds_peak(pc)
     lds_peak_used = lds_peak if lds_peak else LDS_PEAK_GBS
     roofline_lds = {
         "bound": "lds",
@@ -335,6 +399,9 @@ def main():
         "kernel_ms": round(1000 * k_avg, 3),
         "combine_ms": round(1000 * sum(combine_s) / len(combine_s), 3),
         "host_plan_ms": round(1000 * sum(plan_s) / len(plan_s), 3),
+        "tables": "resident: the timed steps reuse the prime, prime-count and factor tables "
+                  "that the first warmup call built in HBM (the reference's D() builds them "
+                  "on every call); that first call is reported separately",
         "table_build_s_first_call": round(first["seconds_tables"], 3),
         "end_to_end_s_first_call": round(first["seconds_total"], 3),
         "work": {k: st[k] for k in ("leaves", "hits", "factor_entries", "pairs", "windows",
@@ -347,6 +414,9 @@ def main():
     }
     if not a.no_pi:
         out["pi"] = measure_pi(pc, x)
+    if not a.no_configs:
+        out["batch_256"] = measure_batch(pc)
+        out["int128_1e21"] = measure_int128(pc)
     if a.serving:
         out["serving"] = measure_serving(pc, a.serving, a.serving_rate)
     if a.calibrate_copy:

@@ -189,6 +189,12 @@ typedef struct pcg_stats {
 /* Statistics of the last D/S2_hard call on this thread. */
 int pcg_last_stats(pcg_stats* out);
 
+/* LDS read bandwidth of the current device in GB/s, measured by a probe
+ * kernel (every CU reads its LDS with bank-conflict-free 128-bit loads): the
+ * peak of the sieve kernel's LDS roofline.  No counterpart in the
+ * reference. */
+int pcg_lds_read_gbs(double* gbs);
+
 /* Release every device buffer the library holds, including the prime and
  * factor tables that stay resident in HBM between calls. */
 int pcg_release(void);

@@ -154,6 +154,7 @@ def load():
         "pcg_pi": ([_I128, PI128], ctypes.c_int),
         "pcg_last_pi_stats": ([ctypes.POINTER(PiStats)], ctypes.c_int),
         "pcg_check_pi": ([_I128, _I128], ctypes.c_int),
+        "pcg_lds_read_gbs": ([ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "pcg_nth_prime": ([I64, P64], ctypes.c_int),
         "pcg_AC": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
@@ -272,6 +273,13 @@ def pi(x):
     out = _I128()
     _check(load().pcg_pi(_i128(x), ctypes.byref(out)))
     return _from128(out)
+
+
+def lds_read_gbs():
+    """LDS read bandwidth of the device in GB/s (probe kernel)."""
+    v = ctypes.c_double()
+    _check(load().pcg_lds_read_gbs(ctypes.byref(v)))
+    return v.value
 
 
 def check_pi(x, pix):

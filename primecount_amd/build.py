@@ -15,7 +15,6 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libprimecount_mi355x.so")
-PROBE = os.path.join(PKG, "libpcg_lds_probe.so")
 
 ARCH = os.environ.get("PCG_OFFLOAD_ARCH", "gfx950")
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
@@ -38,6 +37,7 @@ SOURCES = [
     ("capi.cpp", [], False),
     ("serve.cpp", [], False),
     ("verify.cpp", [], False),
+    ("lds_probe.hip", [], True),
 ]
 
 
@@ -70,13 +70,6 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
     os.replace(tmp, LIB)
-    # LDS bandwidth probe used by bench.py for the LDS roofline peak
-    probe = _compile("lds_probe.hip", [], True)
-    cmd = [hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", PROBE + ".tmp", probe]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError("link failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
-    os.replace(PROBE + ".tmp", PROBE)
     if verbose:
         print("built", LIB)
     return LIB

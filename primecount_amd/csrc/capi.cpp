@@ -17,6 +17,10 @@
 
 using namespace pcg;
 
+namespace pcg {
+hipError_t lds_read_gbs(double* gbs);      // lds_probe.hip
+}
+
 namespace {
 
 thread_local std::string t_error;
@@ -97,6 +101,17 @@ static_assert(kErrInvalid == PCG_E_INVALID && kErrDevice == PCG_E_DEVICE &&
               "pcg_error status codes are the C ABI's");
 
 const char* pcg_last_error(void) { return t_error.c_str(); }
+
+int pcg_lds_read_gbs(double* gbs)
+{
+  return guarded([&] {
+    if (!gbs) throw pcg_error("gbs must not be NULL");
+    const hipError_t e = lds_read_gbs(gbs);
+    if (e != hipSuccess)
+      throw pcg_error(std::string("HIP error '") + hipGetErrorString(e) + "' in the LDS probe",
+                      e == hipErrorOutOfMemory ? kErrNoMem : kErrDevice);
+  });
+}
 
 int pcg_check_pi(pcg_i128 x, pcg_i128 pix)
 {

@@ -1,5 +1,5 @@
-// lds_probe.hip -- measures the LDS read bandwidth of the device (used by
-// bench.py for the LDS roofline peak; not part of the engine's C ABI).
+// lds_probe.hip -- measures the LDS read bandwidth of the device, the peak of
+// the LDS roofline of the sieve kernel (pcg_lds_read_gbs).
 //
 // Every workgroup (256 threads) fills 16 KB of LDS and then reads it back with
 // 128-bit loads in a bank-conflict-free pattern (consecutive lanes read
@@ -37,32 +37,46 @@ __global__ __launch_bounds__(kThreads) void lds_read_kernel(uint4* out, uint32_t
 
 } // namespace
 
-/// Runs the probe on the current device; returns 0 and the measured LDS read
-/// bandwidth in GB/s, or a negative value on a HIP error.
-extern "C" int pcg_probe_lds_read_gbs(double* gbs)
+namespace pcg {
+
+/// Runs the probe on the current device: the measured LDS read bandwidth in
+/// GB/s (bytes read by all workgroups / kernel time), or the HIP error of a
+/// failed call or launch.
+hipError_t lds_read_gbs(double* gbs)
 {
   int dev = 0, cus = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return -1;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return -1;
+  hipError_t err = hipGetDevice(&dev);
+  if (err == hipSuccess) err = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (err != hipSuccess) return err;
   const uint32_t blocks = (uint32_t) cus * 8;          // 8 workgroups (32 waves) per CU
   const uint32_t iters = 2048;
   uint4* out = nullptr;
-  if (hipMalloc(&out, (size_t) blocks * kThreads * sizeof(uint4)) != hipSuccess) return -2;
-  hipEvent_t e0, e1;
-  (void) hipEventCreate(&e0);
-  (void) hipEventCreate(&e1);
-  hipLaunchKernelGGL(lds_read_kernel, dim3(blocks), dim3(kThreads), 0, 0, out, 16u);   // warm-up
-  (void) hipEventRecord(e0, 0);
-  hipLaunchKernelGGL(lds_read_kernel, dim3(blocks), dim3(kThreads), 0, 0, out, iters);
-  (void) hipEventRecord(e1, 0);
-  hipError_t err = hipEventSynchronize(e1);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t st = nullptr;
   float ms = 0;
-  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
-  (void) hipEventDestroy(e0);
-  (void) hipEventDestroy(e1);
+  if ((err = hipMalloc(&out, (size_t) blocks * kThreads * sizeof(uint4))) != hipSuccess) return err;
+  if ((err = hipStreamCreateWithFlags(&st, hipStreamNonBlocking)) != hipSuccess) goto done;
+  if ((err = hipEventCreate(&e0)) != hipSuccess) goto done;
+  if ((err = hipEventCreate(&e1)) != hipSuccess) goto done;
+  hipLaunchKernelGGL(lds_read_kernel, dim3(blocks), dim3(kThreads), 0, st, out, 16u);  // warm-up
+  if ((err = hipGetLastError()) != hipSuccess) goto done;
+  if ((err = hipEventRecord(e0, st)) != hipSuccess) goto done;
+  hipLaunchKernelGGL(lds_read_kernel, dim3(blocks), dim3(kThreads), 0, st, out, iters);
+  if ((err = hipGetLastError()) != hipSuccess) goto done;
+  if ((err = hipEventRecord(e1, st)) != hipSuccess) goto done;
+  if ((err = hipEventSynchronize(e1)) != hipSuccess) goto done;
+  if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) goto done;
+  if (ms <= 0) {
+    err = hipErrorUnknown;
+    goto done;
+  }
+  *gbs = (double) blocks * kThreads * iters * (kWords / kThreads) * 16.0 / (ms * 1e-3) / 1e9;
+done:
+  if (e0) (void) hipEventDestroy(e0);
+  if (e1) (void) hipEventDestroy(e1);
+  if (st) (void) hipStreamDestroy(st);
   (void) hipFree(out);
-  if (err != hipSuccess || ms <= 0) return -3;
-  const double bytes = (double) blocks * kThreads * iters * (kWords / kThreads) * 16.0;
-  *gbs = bytes / (ms * 1e-3) / 1e9;
-  return 0;
+  return err;
 }
+
+} // namespace pcg
