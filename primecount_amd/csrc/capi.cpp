@@ -232,7 +232,17 @@ int pcg_nth_prime(int64_t n, int64_t* out)
   return guarded([&] {
     if (!out) throw pcg_error("out must not be NULL");
     if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
-    *out = (int64_t) nth_prime((uint64_t) n, nullptr);
+    PixStats st;
+    *out = (int64_t) nth_prime((uint64_t) n, &st);
+    t_pi_stats.seconds_total = st.seconds_total;
+    t_pi_stats.seconds_table = st.seconds_table;
+    t_pi_stats.seconds_sigma = st.seconds_sigma;
+    t_pi_stats.seconds_phi0 = st.seconds_phi0;
+    t_pi_stats.seconds_ac = st.seconds_ac;
+    t_pi_stats.seconds_b = st.seconds_b;
+    t_pi_stats.seconds_d = st.seconds_d;
+    t_pi_stats.ac_leaves = st.ac_leaves;
+    t_pi_stats.table_limit = st.table_limit;
   });
 }
 

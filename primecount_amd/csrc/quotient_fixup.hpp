@@ -29,16 +29,16 @@ constexpr uint32_t quotient_fixup(uint64_t r, uint32_t m, uint32_t s)
   return s;
 }
 
-/// floor(xp / m) - low for a leaf whose quotient lies in [low, low + 2^20)
-/// and m < 2^32, from rcp ~ 1 / m: r = xp - low * m is exact modulo 2^64
-/// from the low word of xp because 0 <= r < m * 2^20 < 2^52, and r * rcp is
-/// truncated to the estimate that quotient_fixup corrects.
 /// r as a double for r < 2^52, exactly: the bits of 2^52 + r, minus 2^52.
 constexpr double u52_to_double(uint64_t r)
 {
   return __builtin_bit_cast(double, r | 0x4330000000000000ull) - 4503599627370496.0;
 }
 
+/// floor(xp / m) - low for a leaf whose quotient lies in [low, low + 2^20)
+/// and m < 2^32, from rcp ~ 1 / m: r = xp - low * m is exact modulo 2^64
+/// from the low word of xp because 0 <= r < m * 2^20 < 2^52, and r * rcp is
+/// truncated to the estimate that quotient_fixup corrects.
 constexpr uint32_t leaf_stop_from_rcp(uint64_t xlo, uint64_t low, uint32_t m, double rcp)
 {
   const uint64_t r = xlo - low * m;

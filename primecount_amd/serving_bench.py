@@ -121,8 +121,41 @@ def run(sv, qs, times):
     return res, elapsed, bad, s0, s1
 
 
-def run_plain(qs):
-    """One call per query, back to back: (latencies, elapsed, mismatches)."""
+TERMS = ("table", "sigma", "phi0", "b", "ac", "d")
+
+
+def split_plain(qs, lat, terms):
+    """Where the plain path's time goes: per kind and decade of the argument,
+    the queries, their wall time and the device-stream time of each pi(x)
+    term (stream events between the term launches; A/C includes the host's
+    descriptor lists, D its planning; for nth_prime the terms are those of
+    pi at the starting point, the rest is the interval sieve)."""
+    rows = {}
+    for (k, a, _), w, t in zip(qs, lat, terms):
+        key = ("pi" if k == pc.KIND_PI else "nth" if k == pc.KIND_NTH_PRIME else "D",
+               int(math.log10(max(a, 1))))
+        r = rows.setdefault(key, {"queries": 0, "wall_s": 0.0, **{n: 0.0 for n in TERMS}})
+        r["queries"] += 1
+        r["wall_s"] += w
+        for n in TERMS:
+            r[n] += t.get(n, 0.0)
+    out = []
+    for (kind, dec), r in sorted(rows.items()):
+        other = r["wall_s"] - sum(r[n] for n in TERMS)
+        out.append({"kind": kind, "decade": dec, "queries": r["queries"],
+                    "wall_s": round(r["wall_s"], 4),
+                    **{n + "_s": round(r[n], 4) for n in TERMS},
+                    "other_s": round(other, 4)})
+    tot = {"wall_s": round(sum(x["wall_s"] for x in out), 3)}
+    for n in TERMS:
+        tot[n + "_s"] = round(sum(x[n + "_s"] for x in out), 3)
+    tot["other_s"] = round(sum(x["other_s"] for x in out), 3)
+    return {"rows": out, "total": tot}
+
+
+def run_plain(qs, terms=None):
+    """One call per query, back to back: (latencies, elapsed, mismatches).
+    With a list `terms`, each query's pi(x) term times are appended to it."""
     call = {pc.KIND_PI: pc.pi, pc.KIND_NTH_PRIME: pc.nth_prime, pc.KIND_D: pc.D_default}
     lat, bad = [], 0
     t_start = time.perf_counter()
@@ -131,6 +164,10 @@ def run_plain(qs):
         t = time.perf_counter()
         got = call[k](a)
         lat.append(time.perf_counter() - t)
+        if terms is not None:
+            st = pc.last_pi_stats() if k != pc.KIND_D else {}
+            terms.append({"table": st.get("seconds_table", 0.0),
+                          **{n: st.get("seconds_" + n, 0.0) for n in TERMS[1:]}})
         bad += got != v
         if time.perf_counter() - last > 30:
             print("plain: %d of %d" % (i + 1, len(qs)), file=sys.stderr, flush=True)
@@ -149,6 +186,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=0)
     ap.add_argument("--fixture", default=None, help="serving-mix answers (JSON or JSON lines)")
     ap.add_argument("--plain", action="store_true", help="one call per query, no serving context")
+    ap.add_argument("--split", action="store_true",
+                    help="with --plain: report the time per pi(x) term by kind and decade")
     a = ap.parse_args()
     rate = a.rate if a.rate is not None else (20.0 if a.workload == "mix" else 100.0)
 
@@ -175,7 +214,9 @@ def main():
         for k, x, _ in warm:
             call[k](x)
         warm_s = time.perf_counter() - t0
-        lat, elapsed, bad = run_plain(qs)
+        terms = [] if a.split else None
+        lat, elapsed, bad = run_plain(qs, terms)
+        split = split_plain(qs, lat, terms) if a.split else None
         lat.sort()
 
         def pctp(p):
@@ -188,6 +229,7 @@ def main():
             "latency_ms": {"p50": pctp(0.50), "p90": pctp(0.90), "p99": pctp(0.99),
                            "max": round(lat[-1] * 1000, 3)},
             "warmup_s": round(warm_s, 3),
+            **({"split": split} if split else {}),
         }), flush=True)
         return 0 if bad == 0 else 1
     sv = pc.Server(z_max=z_max, max_batch=a.max_batch)
