@@ -272,13 +272,22 @@ int pcg_last_pi_stats(pcg_pi_stats* out);
 /* ---- serving: continuous batching ---------------------------------------
  *
  * A context owns a worker thread.  pcg_submit queues queries from any thread
- * and returns at once; whenever the device is free the worker runs everything
- * queued (up to max_batch queries) as one batch -- D / S2_hard in one
- * persistent launch per table set, as pcg_D_batch; pi(x) and nth_prime with
- * the D parts of all of them in one launch -- so queries that arrive during a
- * launch join the next one.  pcg_poll hands out finished results in completion order.  No
- * counterpart in the reference, whose entry points compute one x per call
- * (D / S2_hard, include/gourdon.hpp:28,38, include/S.hpp:25,32). */
+ * and returns at once.  Light queries (pi(x) with x <= 1e16, nth_prime(n) with
+ * n <= 2e14, D / S2_hard with x <= 1e16: about 10 ms or less) run as soon as
+ * the device reaches a point where other work may start: at once when it is
+ * idle, otherwise at the next pass point of the running heavy batch (between
+ * its sieve passes of at most 2^28 slots, about 20 ms, and between the term
+ * launches of its queries), with work buffers of their own.  Heavy queries
+ * run in batches that share one launch sequence per kind -- D / S2_hard in
+ * one persistent launch per table set, as pcg_D_batch; pi(x) and nth_prime
+ * with the D parts of all of them in one launch -- taken in arrival order up
+ * to about 250 ms of estimated work (half the estimated backlog, up to 4 s,
+ * when more than 8 heavy queries wait).  Work is admitted by its estimated
+ * device memory (tables + scratch) against the context's budget; a query
+ * that can never fit is answered at once with PCG_E_NOMEM.  pcg_poll hands
+ * out finished results in completion order.  No counterpart in the
+ * reference, whose entry points compute one x per call (D / S2_hard,
+ * include/gourdon.hpp:28,38, include/S.hpp:25,32; pi, src/api.cpp:75-91). */
 
 #define PCG_KIND_D 0
 #define PCG_KIND_S2_HARD 1
@@ -305,13 +314,15 @@ typedef struct pcg_ctx pcg_ctx;
 
 /* Creates a serving context.  y_max, z_max > 0 build the D tables for default
  * queries up to that size now (else on first use); hbm_budget bounds the
- * per-pass slot arrays in bytes (0 = default 8 GiB); max_batch bounds the
- * queries of one launch (0 = 1024). */
+ * device memory the context's work may take, tables and scratch, in bytes
+ * (0 = 90% of the device's memory); max_batch bounds the queries of one
+ * launch (0 = 1024). */
 int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t max_batch,
                    pcg_ctx** ctx);
 
 /* Queues n queries; ids[i] receives the id of q[i].  A query outside the
- * engine's domain is answered at once with an error result. */
+ * engine's domain is answered at once with PCG_E_INVALID, one whose
+ * estimated device memory exceeds the context's budget with PCG_E_NOMEM. */
 int pcg_submit(pcg_ctx* ctx, const pcg_query* q, size_t n, uint64_t* ids);
 
 /* Moves up to max finished results into out; waits up to timeout_ms (< 0:
@@ -321,6 +332,10 @@ int pcg_poll(pcg_ctx* ctx, pcg_result* out, size_t max, int timeout_ms, size_t* 
 typedef struct pcg_ctx_stats {
   uint64_t submitted, completed, batches, max_batch;
   double busy_seconds;  /* worker time inside launches                         */
+  uint64_t light_batches, heavy_batches;
+  uint64_t interleaved;     /* light queries answered at pass points of a heavy
+                               batch                                          */
+  uint64_t rejected_nomem;  /* queries refused for their device memory        */
 } pcg_ctx_stats;
 
 int pcg_ctx_get_stats(pcg_ctx* ctx, pcg_ctx_stats* out);

@@ -93,7 +93,9 @@ class _Result(ctypes.Structure):
 class _CtxStats(ctypes.Structure):
     _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64),
                 ("batches", ctypes.c_uint64), ("max_batch", ctypes.c_uint64),
-                ("busy_seconds", ctypes.c_double)]
+                ("busy_seconds", ctypes.c_double), ("light_batches", ctypes.c_uint64),
+                ("heavy_batches", ctypes.c_uint64), ("interleaved", ctypes.c_uint64),
+                ("rejected_nomem", ctypes.c_uint64)]
 
 
 class PiStats(ctypes.Structure):
@@ -427,8 +429,10 @@ def release():
 
 class Server:
     """Continuous-batching serving context (pcg_ctx_*): submit queries from any
-    thread, poll finished results.  Queries arriving while a launch runs join
-    the next launch.
+    thread, poll finished results.  Light queries arriving while a heavy batch
+    runs are answered at its next pass point; heavy ones join the next heavy
+    batch.  hbm_budget bounds the device memory of the context's work
+    (0 = 90% of the device); a query that cannot fit gets status E_NOMEM.
 
         with Server(z_max=10**8) as sv:
             ids = sv.submit([(KIND_D, 10**15), (KIND_D, 10**16)])
@@ -456,7 +460,8 @@ class Server:
         return [ids[i] for i in range(n)]
 
     def poll(self, max_results=1024, timeout_ms=-1):
-        """Finished results: dicts with id, status, value, seconds, error."""
+        """Finished results: dicts with id, status (0 or a negative PCG_E_*
+        code), value and seconds (submit to result)."""
         out = (_Result * max(max_results, 1))()
         n = ctypes.c_size_t()
         _check(load().pcg_poll(self._ctx, out, max_results, int(timeout_ms), ctypes.byref(n)))

@@ -340,8 +340,7 @@ int pcg_ctx_create(int64_t y_max, int64_t z_max, uint64_t hbm_budget, uint32_t m
       const int64_t m = std::max(y_max, z_max);
       hsl_reserve(LeafSum::D, m, m);
     }
-    const uint64_t slots = hbm_budget ? std::max<uint64_t>(hbm_budget / 4, 64) : 0;
-    *ctx = reinterpret_cast<pcg_ctx*>(new Server(slots, max_batch));
+    *ctx = reinterpret_cast<pcg_ctx*>(new Server(hbm_budget, max_batch));
   });
 }
 
@@ -364,8 +363,10 @@ int pcg_submit(pcg_ctx* ctx, const pcg_query* q, size_t n, uint64_t* ids)
           sq.arg = x;
           if (sq.kind == ServeQuery::NTH_PRIME && (x < 1 || x > (i128) 216289611853439384LL))
             throw pcg_error("nth_prime(n): n must be in [1, 216289611853439384]");
-          if (sq.kind == ServeQuery::PI && x < 0)
-            throw pcg_error("pi(x): x must be >= 0");
+          // the domain checks of the device path, so that an out-of-domain
+          // query is answered at once instead of failing inside a batch
+          if (sq.kind == ServeQuery::PI)
+            pi_validate(x);
         } else if (q[i].kind == PCG_KIND_D || q[i].kind == PCG_KIND_S2_HARD) {
           const LeafSum kind = q[i].kind == PCG_KIND_D ? LeafSum::D : LeafSum::S2_hard;
           if (dflt) {
@@ -418,6 +419,10 @@ int pcg_ctx_get_stats(pcg_ctx* ctx, pcg_ctx_stats* out)
     out->completed = st.completed;
     out->batches = st.batches;
     out->max_batch = st.max_batch;
+    out->light_batches = st.light_batches;
+    out->heavy_batches = st.heavy_batches;
+    out->interleaved = st.interleaved;
+    out->rejected_nomem = st.rejected_nomem;
     out->busy_seconds = st.busy_seconds;
   });
 }

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <cstring>
 #include <mutex>
+#include <functional>
 #include <numeric>
 
 namespace pcg {
@@ -111,19 +112,20 @@ struct HostBuf {
   }
 };
 
-struct Device {
-  bool ready = false;
-  int dev = 0;
-  int cus = 0;
-  hipStream_t stream = nullptr;
+} // namespace
+
+/// Work buffers of one set of engine calls (engine_work_select): the
+/// per-query constants, work items, slot words, escape tables, columns and
+/// partial sums of the sieve passes, the scratch of the other pi(x) terms and
+/// their events.  Sets share the device's stream and resident tables.
+struct Work {
   std::vector<hipEvent_t> ev;         // timing events, 3 per pass
   hipEvent_t staged[2] = {};          // staging slot s may be overwritten
-  DevBuf queries, items, binfo, primes, factor, pi_bits, pi_cnt;
+  hipEvent_t term_ev[6] = {};         // pi(x) term boundaries (pix.cpp)
+  PhiUpload phi;                      // what extra[3] holds (pix.cpp)
+  DevBuf queries, items, binfo;
   DevBuf tc, sacc, counters, queue, cols, sc, sm, smp, out, carry, reach, sfx, spans, groups;
   DevBuf esc_keys, esc_vals, esc_ctl;
-  DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
-  char* scratch = nullptr;            // the device's scratch array (no-allocation entry point)
-  uint64_t pix_n = 0;
   DevBuf extra[8];                    // scratch of the other pi(x) terms
   HostBuf stage[2];                   // per pass: spans, columns
 
@@ -136,6 +138,49 @@ struct Device {
     }
     return ev[i];
   }
+
+  void init()
+  {
+    if (!staged[0])
+      for (auto& e : staged) PCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!term_ev[0])
+      for (auto& e : term_ev) PCG_HIP(hipEventCreate(&e));
+  }
+
+  void release()
+  {
+    for (DevBuf* b : {&queries, &items, &binfo, &tc, &sacc, &counters, &queue, &cols, &sc, &sm,
+                      &smp, &out, &carry, &reach, &sfx, &spans, &groups, &esc_keys, &esc_vals,
+                      &esc_ctl})
+      b->release();
+    for (HostBuf* b : {&stage[0], &stage[1]})
+      b->release();
+    for (DevBuf& b : extra)
+      b.release();
+    phi = PhiUpload();
+  }
+
+  ~Work()
+  {
+    release();
+    for (hipEvent_t e : ev) (void) hipEventDestroy(e);
+    for (hipEvent_t e : staged) if (e) (void) hipEventDestroy(e);
+    for (hipEvent_t e : term_ev) if (e) (void) hipEventDestroy(e);
+  }
+};
+
+namespace {
+
+/// Device-wide state: the stream, the resident tables, the scratch array.
+struct Device {
+  bool ready = false;
+  int dev = 0;
+  int cus = 0;
+  hipStream_t stream = nullptr;
+  DevBuf primes, factor, pi_bits, pi_cnt;
+  DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
+  char* scratch = nullptr;            // the device's scratch array (no-allocation entry point)
+  uint64_t pix_n = 0;
 
   void init()
   {
@@ -153,7 +198,6 @@ struct Device {
                       kErrDevice);
     cus = prop.multiProcessorCount;
     PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    for (auto& e : staged) PCG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     PCG_HIP(hsl_upload_constants());
     void* sp = nullptr;
     PCG_HIP(hsl_scratch_address(dev, &sp));
@@ -163,22 +207,37 @@ struct Device {
 
   void release()
   {
-    for (DevBuf* b : {&queries, &items, &binfo, &primes, &factor, &pi_bits, &pi_cnt, &tc,
-                      &sacc, &counters, &queue, &cols, &sc, &sm, &smp, &out, &carry, &reach,
-                      &sfx, &spans, &groups, &esc_keys, &esc_vals, &esc_ctl})
+    for (DevBuf* b : {&primes, &factor, &pi_bits, &pi_cnt, &pix_tab, &pix_seg})
       b->release();
-    for (HostBuf* b : {&stage[0], &stage[1]})
-      b->release();
-    for (DevBuf* b : {&pix_tab, &pix_seg})
-      b->release();
-    for (DevBuf& b : extra)
-      b.release();
     pix_n = 0;
   }
 };
 
-std::mutex g_mutex;
+std::recursive_mutex g_mutex;
 Device g_dev;
+
+// Work sets: the default one, the sets engine_work_create made (all released
+// by release_device), and the set the calling thread uses.
+Work g_work0;
+std::vector<Work*> g_works = {&g_work0};
+thread_local Work* t_work = nullptr;
+
+Work& cur_work()
+{
+  Work& w = t_work ? *t_work : g_work0;
+  w.init();
+  return w;
+}
+
+// Pass hook of the calling thread (PassHookScope) and the table freeze that
+// holds while it runs.
+thread_local const std::function<void()>* t_pass_hook = nullptr;
+thread_local int t_tables_frozen = 0;
+
+void throw_if_frozen()
+{
+  if (t_tables_frozen) throw tables_frozen_error();
+}
 
 /// Host tables shared by the queries of one launch.
 struct TableSet {
@@ -495,6 +554,7 @@ const TableSet& resident_tables(const std::vector<HslQuery>& qs, uint64_t pmin =
   need.pmax = std::max(need.pmax, pmin);
   if (g_tables_valid && covers(g_tables, need))
     return g_tables;
+  throw_if_frozen();
   g_tables_valid = false;
   g_tables = build_tables(need);
   Device& D = g_dev;
@@ -523,6 +583,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
 {
   Device& D = g_dev;
   D.init();
+  Work& Wk = cur_work();
   double t0 = now();
   for (const auto& q : qs) validate(q);
 
@@ -589,20 +650,20 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     nsegments += plans[i].nseg;
   }
 
-  upload(D.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
-  upload(D.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
-  upload(D.reach, reach.data(), reach.size() * 4);
-  upload(D.sfx, sfx.data(), sfx.size() * 8);
+  upload(Wk.queries, qdesc.data(), qdesc.size() * sizeof(QueryDesc));
+  upload(Wk.binfo, binfo.data(), binfo.size() * sizeof(BInfo));
+  upload(Wk.reach, reach.data(), reach.size() * 4);
+  upload(Wk.sfx, sfx.data(), sfx.size() * 8);
   const size_t nq = qdesc.size();
-  D.carry.ensure((size_t) ncols_total * 8);
-  D.out.ensure((size_t) ncols_total * 16);
-  D.sacc.ensure(nq * 16);
-  D.counters.ensure(128);
+  Wk.carry.ensure((size_t) ncols_total * 8);
+  Wk.out.ensure((size_t) ncols_total * 16);
+  Wk.sacc.ensure(nq * 16);
+  Wk.counters.ensure(128);
   hipStream_t s = D.stream;
-  PCG_HIP(hsl_launch_fill_zero(D.carry.p, (size_t) ncols_total * 8, s));
-  PCG_HIP(hsl_launch_fill_zero(D.out.p, (size_t) ncols_total * 16, s));
-  PCG_HIP(hsl_launch_fill_zero(D.sacc.p, nq * 16, s));
-  PCG_HIP(hsl_launch_fill_zero(D.counters.p, 128, s));
+  PCG_HIP(hsl_launch_fill_zero(Wk.carry.p, (size_t) ncols_total * 8, s));
+  PCG_HIP(hsl_launch_fill_zero(Wk.out.p, (size_t) ncols_total * 16, s));
+  PCG_HIP(hsl_launch_fill_zero(Wk.sacc.p, nq * 16, s));
+  PCG_HIP(hsl_launch_fill_zero(Wk.counters.p, 128, s));
 
   uint64_t max_slots = slots_override ? slots_override : engine_config().max_slots_per_pass;
   if (max_slots == 0) max_slots = kMaxSlotsPerPass;
@@ -703,12 +764,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   }
   const size_t spans_bytes = (size_t) max_pass_spans * sizeof(PassSpan);
   const size_t cols_bytes = (size_t) ncols_total * sizeof(CombineCol);
-  D.spans.ensure(spans_bytes);
-  D.items.ensure((size_t) max_pass_items * sizeof(SegItem));
-  D.cols.ensure(cols_bytes);
-  D.tc.ensure(max_pass_slots * 4);
-  D.queue.ensure(npass * 4);
-  PCG_HIP(hsl_launch_fill_zero(D.queue.p, npass * 4, s));
+  Wk.spans.ensure(spans_bytes);
+  Wk.items.ensure((size_t) max_pass_items * sizeof(SegItem));
+  Wk.cols.ensure(cols_bytes);
+  Wk.tc.ensure(max_pass_slots * 4);
+  Wk.queue.ensure(npass * 4);
+  PCG_HIP(hsl_launch_fill_zero(Wk.queue.p, npass * 4, s));
   // escape tables: per pass at least twice the bound of its escaping slots
   std::vector<uint32_t> esc_cap(npass);
   uint32_t max_cap = 0;
@@ -722,11 +783,11 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     esc_cap[pi] = (uint32_t) cap;
     max_cap = std::max(max_cap, (uint32_t) cap);
   }
-  D.esc_keys.ensure((size_t) max_cap * 8);
-  D.esc_vals.ensure((size_t) max_cap * 4);
-  D.esc_ctl.ensure(npass * 8);
-  PCG_HIP(hsl_launch_fill_zero(D.esc_ctl.p, npass * 8, s));
-  for (auto& hb : D.stage) hb.ensure(spans_bytes + cols_bytes);
+  Wk.esc_keys.ensure((size_t) max_cap * 8);
+  Wk.esc_vals.ensure((size_t) max_cap * 4);
+  Wk.esc_ctl.ensure(npass * 8);
+  PCG_HIP(hsl_launch_fill_zero(Wk.esc_ctl.p, npass * 8, s));
+  for (auto& hb : Wk.stage) hb.ensure(spans_bytes + cols_bytes);
 
   // Shared segments: segment s of every query that covers it in full is
   // sieved once for groups of up to gmax queries with the same first b; a
@@ -779,7 +840,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
       }
       max_groups = std::max(max_groups, groups.size());
     }
-    D.groups.ensure(max_groups * sizeof(SegGroup));
+    Wk.groups.ensure(max_groups * sizeof(SegGroup));
   }
   double t_plan = now() - t_tab, ms_kernel = 0, ms_combine = 0;
 
@@ -787,8 +848,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     double tp = now();
     const Pass& ps = passes[pi];
     const int slot = (int) (pi & 1);
-    if (pi >= 2) PCG_HIP(hipEventSynchronize(D.staged[slot]));
-    char* stage = (char*) D.stage[slot].p;
+    if (pi >= 2) PCG_HIP(hipEventSynchronize(Wk.staged[slot]));
+    char* stage = (char*) Wk.stage[slot].p;
     PassSpan* spans = (PassSpan*) stage;
     CombineCol* cols = (CombineCol*) (stage + spans_bytes);
     std::memcpy(spans, ps.spans.data(), ps.spans.size() * sizeof(PassSpan));
@@ -806,71 +867,74 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     uint32_t slice_len = std::max<uint32_t>(1, (max_n + 127) / 128);
     uint32_t nslices = (max_n + slice_len - 1) / slice_len;
 
-    PCG_HIP(hipMemcpyAsync(D.spans.p, spans, ps.spans.size() * sizeof(PassSpan), hipMemcpyHostToDevice, s));
-    PCG_HIP(hipMemcpyAsync(D.cols.p, cols, (size_t) ncols * sizeof(CombineCol), hipMemcpyHostToDevice, s));
-    PCG_HIP(hipEventRecord(D.staged[slot], s));
-    if ((size_t) nslices * ncols * 16 > D.smp.cap)
+    PCG_HIP(hipMemcpyAsync(Wk.spans.p, spans, ps.spans.size() * sizeof(PassSpan), hipMemcpyHostToDevice, s));
+    PCG_HIP(hipMemcpyAsync(Wk.cols.p, cols, (size_t) ncols * sizeof(CombineCol), hipMemcpyHostToDevice, s));
+    PCG_HIP(hipEventRecord(Wk.staged[slot], s));
+    if ((size_t) nslices * ncols * 16 > Wk.smp.cap)
       PCG_HIP(hipStreamSynchronize(s));          // earlier passes may still read them
-    D.sc.ensure((size_t) nslices * ncols * 8);
-    D.sm.ensure((size_t) nslices * ncols * 8);
-    D.smp.ensure((size_t) nslices * ncols * 16);
-    PCG_HIP(hsl_launch_build_items((const QueryDesc*) D.queries.p, (const PassSpan*) D.spans.p,
-                                   (uint32_t) ps.spans.size(), (const uint32_t*) D.reach.p,
-                                   (const uint64_t*) D.sfx.p, (SegItem*) D.items.p, ps.nitems, s));
+    Wk.sc.ensure((size_t) nslices * ncols * 8);
+    Wk.sm.ensure((size_t) nslices * ncols * 8);
+    Wk.smp.ensure((size_t) nslices * ncols * 16);
+    PCG_HIP(hsl_launch_build_items((const QueryDesc*) Wk.queries.p, (const PassSpan*) Wk.spans.p,
+                                   (uint32_t) ps.spans.size(), (const uint32_t*) Wk.reach.p,
+                                   (const uint64_t*) Wk.sfx.p, (SegItem*) Wk.items.p, ps.nitems, s));
     t_plan += now() - tp;
 
     HslArgs a{};
-    a.queries = (const QueryDesc*) D.queries.p;
-    a.items = (const SegItem*) D.items.p;
-    a.binfo = (const BInfo*) D.binfo.p;
+    a.queries = (const QueryDesc*) Wk.queries.p;
+    a.items = (const SegItem*) Wk.items.p;
+    a.binfo = (const BInfo*) Wk.binfo.p;
     a.primes = (const uint32_t*) D.primes.p;
     a.factor = D.factor.p;
     a.pi_bits = (const uint64_t*) D.pi_bits.p;
     a.pi_cnt = (const uint32_t*) D.pi_cnt.p;
-    a.T_S = (uint32_t*) D.tc.p;
+    a.T_S = (uint32_t*) Wk.tc.p;
     a.cbits = cbits;
-    a.esc.keys = (unsigned long long*) D.esc_keys.p;
-    a.esc.vals = (int32_t*) D.esc_vals.p;
-    a.esc.ctl = (unsigned int*) D.esc_ctl.p + 2 * pi;
+    a.esc.keys = (unsigned long long*) Wk.esc_keys.p;
+    a.esc.vals = (int32_t*) Wk.esc_vals.p;
+    a.esc.ctl = (unsigned int*) Wk.esc_ctl.p + 2 * pi;
     a.esc.cap = esc_cap[pi];
     // the previous pass's combine has read its table (stream order)
-    PCG_HIP(hsl_launch_fill_zero(D.esc_keys.p, (size_t) esc_cap[pi] * 8, s));
-    a.S_acc = (unsigned long long*) D.sacc.p;
-    a.counters = (unsigned long long*) D.counters.p;
-    a.queue = (unsigned int*) D.queue.p + pi;
+    PCG_HIP(hsl_launch_fill_zero(Wk.esc_keys.p, (size_t) esc_cap[pi] * 8, s));
+    a.S_acc = (unsigned long long*) Wk.sacc.p;
+    a.counters = (unsigned long long*) Wk.counters.p;
+    a.queue = (unsigned int*) Wk.queue.p + pi;
     a.nitems = ps.nitems;
     a.groups = nullptr;
     if (share) {
       // the previous pass's kernel may still read the groups buffer
       const std::vector<SegGroup>& groups = pass_groups[pi];
       if (pi > 0) PCG_HIP(hipStreamSynchronize(s));
-      PCG_HIP(hipMemcpyAsync(D.groups.p, groups.data(), groups.size() * sizeof(SegGroup),
+      PCG_HIP(hipMemcpyAsync(Wk.groups.p, groups.data(), groups.size() * sizeof(SegGroup),
                              hipMemcpyHostToDevice, s));
-      a.groups = (const SegGroup*) D.groups.p;
+      a.groups = (const SegGroup*) Wk.groups.p;
       a.nitems = (uint32_t) groups.size();
       a.group_max = gmax;
     }
     a.W = W;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);
 
-    PCG_HIP(hipEventRecord(D.event(3 * pi), s));
+    PCG_HIP(hipEventRecord(Wk.event(3 * pi), s));
     PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
-    PCG_HIP(hipEventRecord(D.event(3 * pi + 1), s));
-    PCG_HIP(hsl_launch_combine((const CombineCol*) D.cols.p, ncols, (const SegItem*) D.items.p,
-                               (const uint32_t*) D.tc.p, a.esc, cbits, slice_len, nslices,
-                               (long long*) D.sc.p, (long long*) D.sm.p, D.smp.p,
-                               (long long*) D.carry.p, D.out.p, s));
-    PCG_HIP(hipEventRecord(D.event(3 * pi + 2), s));
+    PCG_HIP(hipEventRecord(Wk.event(3 * pi + 1), s));
+    PCG_HIP(hsl_launch_combine((const CombineCol*) Wk.cols.p, ncols, (const SegItem*) Wk.items.p,
+                               (const uint32_t*) Wk.tc.p, a.esc, cbits, slice_len, nslices,
+                               (long long*) Wk.sc.p, (long long*) Wk.sm.p, Wk.smp.p,
+                               (long long*) Wk.carry.p, Wk.out.p, s));
+    PCG_HIP(hipEventRecord(Wk.event(3 * pi + 2), s));
+    // between passes: the calling thread's pass hook (other work on the
+    // same stream, after this pass) may run
+    if (pi + 1 < npass) engine_pass_point();
   }
 
   std::vector<i128> col_out(ncols_total);
   std::vector<uint64_t> sacc(2 * nq);
   uint64_t counters[12] = {};
-  PCG_HIP(hipMemcpyAsync(col_out.data(), D.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
-  PCG_HIP(hipMemcpyAsync(sacc.data(), D.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
-  PCG_HIP(hipMemcpyAsync(counters, D.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(col_out.data(), Wk.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(sacc.data(), Wk.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(counters, Wk.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
   std::vector<uint32_t> esc_ctl(2 * npass);
-  PCG_HIP(hipMemcpyAsync(esc_ctl.data(), D.esc_ctl.p, npass * 8, hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipMemcpyAsync(esc_ctl.data(), Wk.esc_ctl.p, npass * 8, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
   for (size_t pi = 0; pi < npass; pi++)
     if (esc_ctl[2 * pi + 1] != 0)
@@ -879,8 +943,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
                       kErrInternal);
   for (size_t pi = 0; pi < npass; pi++) {
     float mk = 0, mc = 0;
-    PCG_HIP(hipEventElapsedTime(&mk, D.ev[3 * pi], D.ev[3 * pi + 1]));
-    PCG_HIP(hipEventElapsedTime(&mc, D.ev[3 * pi + 1], D.ev[3 * pi + 2]));
+    PCG_HIP(hipEventElapsedTime(&mk, Wk.ev[3 * pi], Wk.ev[3 * pi + 1]));
+    PCG_HIP(hipEventElapsedTime(&mc, Wk.ev[3 * pi + 1], Wk.ev[3 * pi + 2]));
     ms_kernel += mk;
     ms_combine += mc;
   }
@@ -981,7 +1045,7 @@ std::string mib(size_t bytes)
 /// rejected (kErrInvalid).
 i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   Device& D = g_dev;
   D.init();
   const double t0 = now();
@@ -1154,7 +1218,7 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
 
 i128 hsl_compute(const HslQuery& q, pcg_stats* stats)
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   auto r = run_group({q}, stats);
   return r[0];
 }
@@ -1162,7 +1226,7 @@ i128 hsl_compute(const HslQuery& q, pcg_stats* stats)
 std::vector<i128> hsl_compute_batch(const std::vector<HslQuery>& qs, pcg_stats* stats,
                                     uint64_t max_slots)
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   return hsl_compute_batch_locked(qs, stats, max_slots);
 }
 
@@ -1206,7 +1270,7 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
 
 uint64_t factor_table_mismatches(LeafSum kind, int64_t y, int64_t limit)
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   g_dev.init();
   if (y < 1 || limit < 1 || (uint64_t) limit >= (1ull << 32))
     throw pcg_error("factor table check needs y >= 1 and 1 <= limit < 2^32");
@@ -1257,7 +1321,7 @@ void hsl_validate(const HslQuery& q)
 
 void hsl_reserve(LeafSum kind, int64_t y, int64_t z)
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   g_dev.init();
   HslQuery q;
   q.kind = kind;
@@ -1270,7 +1334,7 @@ void hsl_reserve(LeafSum kind, int64_t y, int64_t z)
 
 bool device_available()
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   try {
     g_dev.init();
     return true;
@@ -1279,15 +1343,76 @@ bool device_available()
   }
 }
 
-std::unique_lock<std::mutex> engine_lock()
+std::unique_lock<std::recursive_mutex> engine_lock()
 {
-  return std::unique_lock<std::mutex>(g_mutex);
+  return std::unique_lock<std::recursive_mutex>(g_mutex);
+}
+
+Work* engine_work_create()
+{
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
+  Work* w = new Work();
+  g_works.push_back(w);
+  return w;
+}
+
+void engine_work_destroy(Work* w)
+{
+  if (!w) return;
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
+  g_works.erase(std::remove(g_works.begin(), g_works.end(), w), g_works.end());
+  if (t_work == w) t_work = nullptr;
+  delete w;
+}
+
+Work* engine_work_select(Work* w)
+{
+  Work* prev = t_work;
+  t_work = w;
+  return prev;
+}
+
+PassHookScope::PassHookScope(std::function<void()> f) : f_(std::move(f)), prev_(t_pass_hook)
+{
+  t_pass_hook = &f_;
+}
+
+PassHookScope::~PassHookScope()
+{
+  t_pass_hook = prev_;
+}
+
+void engine_pass_point()
+{
+  if (!t_pass_hook) return;
+  const std::function<void()>* h = t_pass_hook;
+  t_pass_hook = nullptr;              // calls made by the hook see no hook
+  t_tables_frozen++;
+  try {
+    (*h)();
+  } catch (...) {
+    t_tables_frozen--;
+    t_pass_hook = h;
+    throw;
+  }
+  t_tables_frozen--;
+  t_pass_hook = h;
+}
+
+PhiUpload& engine_phi_upload()
+{
+  return cur_work().phi;
+}
+
+hipEvent_t* engine_term_events()
+{
+  return cur_work().term_ev;
 }
 
 void* engine_buffer(int slot, size_t bytes)
 {
   if (slot < 0 || slot >= 8) throw pcg_error("internal: engine buffer slot", kErrInternal);
-  return g_dev.extra[slot].ensure(bytes);
+  return cur_work().extra[slot].ensure(bytes);
 }
 
 /// Prime-count table over [0, n] built on the device: segmented sieve of
@@ -1298,6 +1423,7 @@ static void ensure_pix_table(uint64_t n)
 {
   Device& D = g_dev;
   if (D.pix_n >= n) return;
+  throw_if_frozen();
   n = std::max<uint64_t>(n, 1u << 20);
   const TableSet& T = g_tables;
   const uint64_t r = isqrt(n);
@@ -1354,9 +1480,10 @@ uint64_t engine_generation()
 
 void release_device()
 {
-  std::lock_guard<std::mutex> lock(g_mutex);
+  std::lock_guard<std::recursive_mutex> lock(g_mutex);
   g_generation++;
   g_dev.release();
+  for (Work* w : g_works) w->release();
   g_tables = TableSet();
   g_tables_valid = false;
 }

@@ -6,6 +6,7 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <functional>
 #include <mutex>
 #include <vector>
 
@@ -88,7 +89,63 @@ struct EngineView {
   uint64_t pix_n = 0;                   // it covers [0, pix_n]
 };
 
-std::unique_lock<std::mutex> engine_lock();
+/// The engine lock (recursive: engine calls made from a pass hook on the
+/// thread that holds it take it again).
+std::unique_lock<std::recursive_mutex> engine_lock();
+
+/// What a work set's phi table buffer (extra slot 3) holds (pix.cpp).
+struct PhiUpload {
+  const void* buf = nullptr;
+  uint32_t k = 0;
+  uint64_t gen = 0;
+};
+
+/// A set of work buffers (slot words, work items, partial sums, the scratch
+/// of the other pi(x) terms).  Engine calls on a thread use the set it
+/// selected, the default set otherwise; all sets share the device's stream
+/// and resident tables, so calls with different sets run one after another
+/// on the device, and a call made from a pass hook does not disturb the
+/// buffers of the batch it interrupts.
+struct Work;
+Work* engine_work_create();
+void engine_work_destroy(Work* w);
+/// Selects the calling thread's work set (nullptr: the default); returns the
+/// previous one.
+Work* engine_work_select(Work* w);
+
+/// While alive, the calling thread's batches call `f` at their pass points:
+/// between the kernel passes of a D / S2_hard launch and between the
+/// per-query term launches of a pi(x) batch, with the engine lock held, the
+/// work queued so far ahead of it on the stream, and the resident tables
+/// frozen (an engine call inside f that needs larger tables fails with
+/// kErrInternal instead of rebuilding them).  f runs on the same thread; calls
+/// it makes see no hook.
+class PassHookScope {
+public:
+  explicit PassHookScope(std::function<void()> f);
+  ~PassHookScope();
+  PassHookScope(const PassHookScope&) = delete;
+  PassHookScope& operator=(const PassHookScope&) = delete;
+
+private:
+  std::function<void()> f_;
+  const std::function<void()>* prev_;
+};
+
+/// Thrown by an engine call inside a pass hook that would need larger
+/// resident tables than the running batch holds.
+class tables_frozen_error : public pcg_error {
+public:
+  tables_frozen_error()
+      : pcg_error("internal: resident tables are in use by the running batch", kErrInternal) {}
+};
+
+/// A pass point: runs the calling thread's pass hook, if any.
+void engine_pass_point();
+
+/// The selected work set's phi table record and its 6 term events (pix.cpp).
+PhiUpload& engine_phi_upload();
+hipEvent_t* engine_term_events();
 
 /// Makes the D tables for (y, z) (y >= z shares one prefix-valid table), or
 /// the S2_hard tables (kind), and a prime-count table covering [0, pix_n]

@@ -178,15 +178,14 @@ i128 run_sigma(i128 x, int64_t y, int64_t x_star, const EngineView& v)
 const uint32_t* phi_tiny_device(const PhiTiny& t, const EngineView& v)
 {
   uint32_t* dtbl = (uint32_t*) engine_buffer(3, (size_t) t.pp * 4);
-  static const void* uploaded = nullptr;     // the device copy stays until the buffer moves
-  static uint32_t uploaded_k = 0;
-  static uint64_t uploaded_gen = 0;
-  if (dtbl != uploaded || uploaded_k != t.k || uploaded_gen != engine_generation()) {
+  // the device copy stays until the buffer moves (per work set)
+  PhiUpload& up = engine_phi_upload();
+  if (dtbl != up.buf || up.k != t.k || up.gen != engine_generation()) {
     PIX_HIP(hipMemcpyAsync(dtbl, t.tbl.data(), (size_t) t.pp * 4, hipMemcpyHostToDevice, v.stream));
     PIX_HIP(hipStreamSynchronize(v.stream));
-    uploaded = dtbl;
-    uploaded_k = t.k;
-    uploaded_gen = engine_generation();
+    up.buf = dtbl;
+    up.k = t.k;
+    up.gen = engine_generation();
   }
   return dtbl;
 }
@@ -637,9 +636,7 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
 {
   const size_t n = idx.size();
   if (n == 0) return;
-  static hipEvent_t ev[6] = {};
-  if (!ev[0])
-    for (auto& e : ev) PIX_HIP(hipEventCreate(&e));
+  hipEvent_t* ev = engine_term_events();
   unsigned long long* acc = (unsigned long long*) engine_buffer(0, n * kTermWords * 8);
   uint64_t* dn = (uint64_t*) engine_buffer(1, n * 8);
   uint64_t* dpts = (uint64_t*) engine_buffer(2, n * 8);
@@ -682,6 +679,9 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
   for (size_t j = 0; j < n; j++) {
     launch_ac(L[j], v, dd, acc + kTermWords * j + 10);
     dd += L[j].bytes();
+    // a pass point after each query's A/C launch: other work queued now
+    // runs after it
+    if (j + 1 < n) engine_pass_point();
   }
   PIX_HIP(hipEventRecord(ev[4], v.stream));
 
@@ -793,6 +793,53 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st, ui
 i128 pi_gourdon(i128 x, PixStats* st)
 {
   return pi_gourdon_batch({x}, st)[0];
+}
+
+void pi_validate(i128 x)
+{
+  if (x < 0) throw pcg_error("pi(x): x must be >= 0");
+  if (x <= (i128) 100000000) return;
+  const GourdonParams gp = gourdon_params(x);
+  check_params(x, gp.y, gp.z, gp.k);
+  (void) table_need(x, gp.y, gp.x_star);
+}
+
+/// Upper estimate of pi(n): n / (ln n - 1.1) for n >= 100 (Rosser-Schoenfeld
+/// type bound; pi(n) < 1.26 n / ln n), small n counted exactly enough.
+static uint64_t pi_upper(uint64_t n)
+{
+  if (n < 100) return 26;
+  const double l = std::log((double) n);
+  return (uint64_t) (1.26 * (double) n / l) + 1;
+}
+
+uint64_t table_bytes_D(int64_t y, int64_t z)
+{
+  const uint64_t pmax = (uint64_t) std::max<int64_t>(y, (int64_t) isqrt((u128) z)) + 1;
+  const uint64_t ft = (uint64_t) ft_to_index((uint64_t) z) + 1;
+  return pi_upper(pmax) * 4 + (pmax / 240 + 1) * 12 + ft * (z > kFactorTableD16Max ? 4 : 2);
+}
+
+uint64_t pi_footprint(i128 x)
+{
+  if (x <= (i128) 100000000) return (uint64_t) 100000000 / 240 * sizeof(PixWord) + (64u << 20);
+  const GourdonParams gp = gourdon_params(x);
+  const uint64_t need = table_need(x, gp.y, gp.x_star);
+  // prime-count table (one 16-byte word per 240 numbers), the D tables, and
+  // the per-query scratch: A/C descriptors and term sums (under 64 MiB up to
+  // x = 1e19, about x^(1/3) growth beyond)
+  const double scale = std::max(1.0, std::cbrt((double) x / 1e19));
+  return (need / 240 + 1) * sizeof(PixWord) + table_bytes_D(gp.y, gp.z) +
+         (uint64_t) (scale * (64u << 20));
+}
+
+uint64_t nth_prime_footprint(uint64_t n)
+{
+  if (n <= 5761455) return pi_footprint(100000000);
+  // pi(x0) at x0 ~ n ln n plus the interval sieve (bitmap and counts)
+  const double l = std::log((double) n);
+  const double x0 = (double) n * (l + std::log(l));
+  return pi_footprint((i128) x0) + (256u << 20);
 }
 
 // ---------------------------------------------------------------------------

@@ -50,6 +50,17 @@ uint64_t nth_prime(uint64_t n, PixStats* st);
 std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,
                                       uint64_t max_slots = 0);
 
+/// Throws pcg_error (kErrInvalid) unless pi(x) is inside the device path's
+/// domain (x >= 0; the parameter checks and the prime-count table limit).
+void pi_validate(i128 x);
+
+/// Estimated device bytes of the tables and per-query scratch that pi(x),
+/// nth_prime(n) and a D query with tables (y, z) need: the sizes a serving
+/// context admits work by.
+uint64_t pi_footprint(i128 x);
+uint64_t nth_prime_footprint(uint64_t n);
+uint64_t table_bytes_D(int64_t y, int64_t z);
+
 /// Li(x) = li(x) - li(2), the offset logarithmic integral (long double).
 long double offset_log_integral(long double x);
 

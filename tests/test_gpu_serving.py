@@ -134,17 +134,76 @@ def test_serving_pi_and_nth_prime(pc, oracle):
             assert got[i]["status"] == 0 and got[i]["value"] == w, q
 
 
-def test_serving_small_hbm_budget(pc):
-    """A small slot budget (hbm_budget) cuts the D launches of pi(x) and D
-    queries into many passes by segment range; the answers do not change."""
+def test_serving_small_pass_slots(pc):
+    """Small passes (pcg_set_pass_slots) cut the D launches of pi(x) and D
+    queries into many passes by segment range, with pass points between them;
+    the answers do not change."""
     known = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
              if int(r["x"]) <= 10 ** 15]
     drow = fixture_rows(10 ** 15)[:8]
-    with pc.Server(hbm_budget=1 << 20) as sv:
-        log("serving with a 1 MiB slot budget: %d pi, %d D" % (len(known), len(drow)))
-        qs = [(pc.KIND_PI, x) for x, _ in known] + [(pc.KIND_D, x) for x, _ in drow]
-        want = [v for _, v in known] + [v for _, v in drow]
-        ids = sv.submit(qs)
-        got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=180000)}
-        for i, q, w in zip(ids, qs, want):
-            assert got[i]["status"] == 0 and got[i]["value"] == w, q
+    try:
+        pc.set_pass_slots(1 << 16)
+        with pc.Server() as sv:
+            log("serving with 2^16-slot passes: %d pi, %d D" % (len(known), len(drow)))
+            qs = [(pc.KIND_PI, x) for x, _ in known] + [(pc.KIND_D, x) for x, _ in drow]
+            want = [v for _, v in known] + [v for _, v in drow]
+            ids = sv.submit(qs)
+            got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=180000)}
+            for i, q, w in zip(ids, qs, want):
+                assert got[i]["status"] == 0 and got[i]["value"] == w, q
+    finally:
+        pc.set_pass_slots(0)
+
+
+def test_serving_memory_admission(pc):
+    """A context with a 2 GiB device-memory budget answers the queries whose
+    tables and scratch fit it and refuses pi(1e19) (7.5 GB of prime-count
+    table alone) at once with PCG_E_NOMEM; an out-of-domain pi query is
+    refused with PCG_E_INVALID (pi(1e30): x / y beyond the 2^49 limit of the
+    prime-count table) and the others still share launches."""
+    small = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
+             if int(r["x"]) <= 10 ** 14]
+    with pc.Server(hbm_budget=2 << 30) as sv:
+        log("serving with a 2 GiB budget")
+        big = sv.submit([(pc.KIND_PI, 10 ** 19)])[0]
+        bad = sv.submit([(pc.KIND_PI, 10 ** 30)])[0]       # x / y beyond 2^49
+        ids = sv.submit([(pc.KIND_PI, x) for x, _ in small])
+        got = {r["id"]: r for r in sv.poll_all(len(ids) + 2, timeout_ms=120000)}
+        assert got[big]["status"] == pc.E_NOMEM
+        assert got[bad]["status"] == pc.E_INVALID
+        for i, (x, v) in zip(ids, small):
+            assert got[i]["status"] == 0 and got[i]["value"] == v, x
+        st = sv.stats()
+        assert st["rejected_nomem"] == 1
+        assert st["batches"] < len(ids)
+
+
+def test_serving_light_queries_interleave(pc):
+    """Light queries submitted while a heavy batch runs are answered at its
+    pass points: before the heavy queries finish, with the same values."""
+    heavy = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
+             if 10 ** 18 <= int(r["x"]) <= 10 ** 19]
+    light = [(int(r["x"]), int(r["pi"])) for r in golden("pi_known.json")["rows"]
+             if int(r["x"]) <= 10 ** 13]
+    with pc.Server() as sv:
+        # tables for the largest query first
+        w = sv.submit([(pc.KIND_PI, heavy[-1][0])])
+        sv.poll_all(len(w), timeout_ms=120000)
+        log("serving: %d heavy pi, then %d light pi during them" % (len(heavy), len(light)))
+        hid = sv.submit([(pc.KIND_PI, x) for x, _ in heavy] * 3)
+        time.sleep(0.03)
+        lid = sv.submit([(pc.KIND_PI, x) for x, _ in light])
+        order = []
+        got = {}
+        while len(got) < len(hid) + len(lid):
+            for r in sv.poll(64, 120000):
+                got[r["id"]] = r
+                order.append(r["id"])
+        for i, (x, v) in zip(hid, heavy * 3):
+            assert got[i]["status"] == 0 and got[i]["value"] == v, x
+        for i, (x, v) in zip(lid, light):
+            assert got[i]["status"] == 0 and got[i]["value"] == v, x
+        last_light = max(order.index(i) for i in lid)
+        last_heavy = max(order.index(i) for i in hid)
+        assert last_light < last_heavy
+        assert sv.stats()["interleaved"] >= 1
