@@ -276,8 +276,8 @@ int pcg_last_pi_stats(pcg_pi_stats* out);
  * n <= 2e14, D / S2_hard with x <= 1e16: about 10 ms or less) run as soon as
  * the device reaches a point where other work may start: at once when it is
  * idle, otherwise at the next pass point of the running heavy batch (between
- * its sieve passes of at most 2^28 slots, about 20 ms, and between the term
- * launches of its queries), with work buffers of their own.  Heavy queries
+ * the term launches of its queries and between its sieve passes of at most
+ * 2^31 slots), with work buffers of their own.  Heavy queries
  * run in batches that share one launch sequence per kind -- D / S2_hard in
  * one persistent launch per table set, as pcg_D_batch; pi(x) and nth_prime
  * with the D parts of all of them in one launch -- taken in arrival order up

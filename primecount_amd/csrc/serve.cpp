@@ -65,10 +65,13 @@ uint64_t footprint(const ServeQuery& q)
   return table_bytes_D(q.q.y, z) + kQueryScratch;
 }
 
-/// Slots of one sieve pass of a heavy batch: 2^28 (1 GiB of slot words,
-/// about 20 ms of the D(1e19) sieve), so that light queries wait at most
-/// about that long for a pass point.
-constexpr uint64_t kPassSlots = 1ull << 28;
+/// Slots of one sieve pass of a heavy batch: 2^31 (8 GiB of slot words), the
+/// engine's default.  Smaller passes would give light queries more pass
+/// points but cost the heavy work dearly: each pass starts on its heaviest
+/// (lowest) segments and drains the persistent grid unevenly (D(1e19) on the
+/// MI355X: 120.6 ms in one pass, 134.1 / 153.4 / 172.7 ms in passes of
+/// 2^30 / 2^29 / 2^28 slots).
+constexpr uint64_t kPassSlots = 1ull << 31;
 
 /// Heavy work taken per batch: queries in arrival order until their
 /// estimates pass this (at least one).  With more than kBacklog heavy queries
