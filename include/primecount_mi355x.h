@@ -184,6 +184,9 @@ typedef struct pcg_stats {
   uint64_t device_allocations; /* device and pinned host allocations the library
                                 has made in this process so far (at the end of
                                 the call)                                     */
+  uint64_t escaped_slots;    /* slots whose M did not fit the 32-bit slot word
+                                and went to the escape table                  */
+  uint64_t escape_bound;     /* the host's upper bound of those slots         */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

@@ -74,6 +74,8 @@ class Stats(ctypes.Structure):
         ("prefix_builds", ctypes.c_uint64),
         ("queue_steps", ctypes.c_uint64),
         ("device_allocations", ctypes.c_uint64),
+        ("escaped_slots", ctypes.c_uint64),
+        ("escape_bound", ctypes.c_uint64),
     ]
 
     def as_dict(self):

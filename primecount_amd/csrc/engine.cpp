@@ -773,10 +773,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   // escape tables: per pass at least twice the bound of its escaping slots
   std::vector<uint32_t> esc_cap(npass);
   uint32_t max_cap = 0;
+  uint64_t esc_bound_total = 0;
   for (size_t pi = 0; pi < npass; pi++) {
     uint64_t bound = 0;
     for (const PassSpan& sp : passes[pi].spans)
       bound += escape_bound(plans[slot_query[sp.query]], sp.s0, sp.s1);
+    esc_bound_total += bound;
     uint64_t cap = 1024;
     while (cap < 2 * bound + 2) cap *= 2;
     if (cap > (1ull << 31)) throw pcg_error("internal: escape table too large", kErrInternal);
@@ -936,11 +938,14 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   std::vector<uint32_t> esc_ctl(2 * npass);
   PCG_HIP(hipMemcpyAsync(esc_ctl.data(), Wk.esc_ctl.p, npass * 8, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipStreamSynchronize(s));
-  for (size_t pi = 0; pi < npass; pi++)
+  uint64_t n_escaped = 0;
+  for (size_t pi = 0; pi < npass; pi++) {
     if (esc_ctl[2 * pi + 1] != 0)
       throw pcg_error("internal: escape table overflow (" + std::to_string(esc_ctl[2 * pi]) +
                           " escaping slots, capacity " + std::to_string(esc_cap[pi]) + ")",
                       kErrInternal);
+    n_escaped += esc_ctl[2 * pi];
+  }
   for (size_t pi = 0; pi < npass; pi++) {
     float mk = 0, mc = 0;
     PCG_HIP(hipEventElapsedTime(&mk, Wk.ev[3 * pi], Wk.ev[3 * pi + 1]));
@@ -984,6 +989,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->prefix_builds = counters[10];
     st->queue_steps = counters[11];
     st->device_allocations = g_allocations.load();
+    st->escaped_slots = n_escaped;
+    st->escape_bound = esc_bound_total;
   }
   return result;
 }
@@ -1212,6 +1219,8 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
     stats->factor_entries = counters[2];
     stats->pairs = counters[3];
     stats->device_allocations = g_allocations.load();
+    stats->escaped_slots = ectl[0];
+    stats->escape_bound = bound;
   }
   return r;
 }
@@ -1261,6 +1270,8 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.group_steps += st.group_steps;
     acc.prefix_builds += st.prefix_builds;
     acc.queue_steps += st.queue_steps;
+    acc.escaped_slots += st.escaped_slots;
+    acc.escape_bound += st.escape_bound;
   }
   acc.device_allocations = g_allocations.load();
   acc.seconds_total = now() - t0;

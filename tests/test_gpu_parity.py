@@ -246,6 +246,11 @@ def test_slot_escape_invariance(pc, bits):
         log("slot bits %d: D default 1e13, 1e15, 1e17, S2_hard 1e15" % bits)
         for e in (13, 15, 17):
             assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+            st = pc.last_stats()
+            # the host's bound holds, and with 28 bits for C slots do escape
+            assert st["escaped_slots"] <= st["escape_bound"]
+            if bits == 28:
+                assert st["escaped_slots"] > 0
         assert pc.S2_hard_default(10 ** 15) == int(s2[10 ** 15]["value"])
         for x, y, z, k, want in kat:
             assert pc.D(x, y, z, k) == want, (x, y, z, k)
