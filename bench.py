@@ -178,20 +178,33 @@ def measure_batch(pc):
                                      "p99_ms": round(lat[int(0.99 * len(lat))] * 1000, 3)}}
 
 
-def measure_int128(pc):
+def _sched(st):
+    return {"segment_words": st["segment_words"], "kernel_waves": st["kernel_waves"],
+            "grid_waves": st["grid_waves"], "group_size": st["group_size"]}
+
+
+def measure_int128(pc, tune=True):
     """Config 4: D(x) and S2_hard(x) at x = 1e21 (x > 2^64, the 128-bit path)
     with the reference's default parameters, against the values the
     reference build printed: the first call (which builds the tables) and
-    the second (tables resident)."""
+    the second (tables resident) on the plain schedule (the segment-size
+    rule), then the autotuner on a fresh cache file: the tuning call (the
+    rule's schedule and the candidates around it, each checked to give the
+    same value) and a call on the tuned schedule."""
     path = os.path.join(REPO, "tests", "golden", "configs_big.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         big = json.load(f)
     out = {}
+    import tempfile
+    fd, cache = tempfile.mkstemp(prefix="pcg_schedules_", suffix=".txt")
+    os.close(fd)
+    os.unlink(cache)                 # a fresh cache: every class is tuned here
     for name, fn in (("D", pc.D_default), ("S2_hard", pc.S2_hard_default)):
         r = big[name][0]
         x, want = int(r["x"]), int(r["value"])
+        pc.set_autotune(pc.TUNE_OFF)
         t = time.perf_counter()
         v1 = fn(x)
         first = time.perf_counter() - t
@@ -203,6 +216,31 @@ def measure_int128(pc):
                      "seconds_first_call": round(first, 3), "seconds": round(second, 3),
                      "kernel_s": round(st["seconds_kernel"], 3),
                      "segment_words": st["segment_words"]}
+        if not tune:
+            continue
+        try:
+            pc.set_schedule_cache(cache)
+            pc.set_autotune(pc.TUNE_ON)
+            t = time.perf_counter()
+            v3 = fn(x)
+            tune_s = time.perf_counter() - t
+            assert pc.last_stats()["schedule_source"] == 2
+            t = time.perf_counter()
+            v4 = fn(x)
+            tuned_s = time.perf_counter() - t
+            st4 = pc.last_stats()
+            out[name]["plain_schedule"] = _sched(st)
+            out[name]["tuned"] = {"seconds": round(tuned_s, 3),
+                                  "kernel_s": round(st4["seconds_kernel"], 3),
+                                  "bit_exact": v3 == want and v4 == want,
+                                  "tuning_call_s": round(tune_s, 3),
+                                  "schedule": _sched(st4),
+                                  "vs_plain": round(second / tuned_s, 3)}
+        finally:
+            pc.set_autotune(pc.TUNE_CACHED)
+            pc.set_schedule_cache(None)
+    if os.path.exists(cache):
+        os.unlink(cache)
     return out
 
 

@@ -121,9 +121,47 @@ int pcg_D_batch(const pcg_i128* xs, size_t n, pcg_i128* out);
 /* ---- engine configuration and statistics ----------------------------- */
 
 /* Segment size in 64-bit sieve words (240 numbers each); a multiple of 128
- * in [128, 4096].  0 selects the autotuned default.  Segment size never changes
- * results (replaces the segment sizing of src/LoadBalancerS2.cpp:65-110). */
+ * in [128, 4096].  0 selects the schedule (the autotuner's cached winner or
+ * the segment-size rule).  Segment size never changes results (replaces the
+ * segment sizing of src/LoadBalancerS2.cpp:65-110). */
 int pcg_set_segment_words(uint32_t words);
+
+/* Schedule autotuner of the sieve launches (segment size, queries per shared
+ * segment, persistent grid; none of them changes results).  Launches are
+ * classed by device, kind (D / S2_hard), factor-table width, the sum of
+ * their sieve limits (half-octaves) and their number of queries (octaves).
+ *   PCG_TUNE_OFF    always the segment-size rule;
+ *   PCG_TUNE_CACHED (default) a cached winner of the class when there is
+ *                   one, else the rule;
+ *   PCG_TUNE_ON     as CACHED, and a class not yet cached is tuned on the
+ *                   launch itself: the rule's schedule first (its sums are
+ *                   the answer), then a few candidates around it, every one
+ *                   checked to give the same sums; the fastest is cached.
+ * Launches whose sieve limits sum to less than 2^30 use the rule.  The
+ * initial mode can also be set with the environment variable PCG_AUTOTUNE
+ * (0, 1, 2).  No counterpart in the reference (src/LoadBalancerS2.cpp sizes
+ * segments by rules). */
+/* Pins the schedule of the following launches: segment words (as
+ * pcg_set_segment_words), queries per shared segment (2 or 4), the persistent
+ * grid as a percentage (1..100) of the waves occupancy allows, and the
+ * single-query kernel build (0 unconstrained, 6 built for 6 waves per SIMD).
+ * 0 (waves: -1) leaves a knob to the schedule; any knob set turns the
+ * autotuner off.  Never changes results. */
+int pcg_set_schedule(uint32_t words, uint32_t group, uint32_t grid_pct, int32_t waves);
+
+#define PCG_TUNE_OFF 0
+#define PCG_TUNE_CACHED 1
+#define PCG_TUNE_ON 2
+int pcg_set_autotune(int mode);
+
+/* The autotuner's cache file: path, or NULL / "" for the default
+ * ($PCG_TUNE_CACHE, else $XDG_CACHE_HOME/primecount_mi355x/schedules.txt,
+ * else $HOME/.cache/primecount_mi355x/schedules.txt).  Winners are appended
+ * as text lines keyed by device and class; a file that cannot be written
+ * keeps them in memory only.  Forgets the winners in memory; the file is
+ * read on next use.  *entries (if not NULL) receives the number of winners
+ * the file holds for any device. */
+int pcg_set_schedule_cache(const char* path, uint64_t* entries);
 
 /* Whether the queries of one launch (pcg_D_batch, serving, pi(x) batches)
  * share the sieve of the segments they have in common: each segment is
@@ -187,6 +225,12 @@ typedef struct pcg_stats {
   uint64_t escaped_slots;    /* slots whose M did not fit the 32-bit slot word
                                 and went to the escape table                  */
   uint64_t escape_bound;     /* the host's upper bound of those slots         */
+  uint64_t schedule_source;  /* 0 the segment-size rule (or pcg_set_segment_words),
+                                1 a cached autotuner winner, 2 tuned by this call */
+  uint64_t group_size;       /* queries sieved together per shared segment     */
+  uint64_t grid_waves;       /* waves of the persistent sieve grid (at most)   */
+  uint64_t kernel_waves;     /* 6: the sieve kernel built for 6 waves per SIMD,
+                                0: the unconstrained build                    */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

@@ -76,6 +76,10 @@ class Stats(ctypes.Structure):
         ("device_allocations", ctypes.c_uint64),
         ("escaped_slots", ctypes.c_uint64),
         ("escape_bound", ctypes.c_uint64),
+        ("schedule_source", ctypes.c_uint64),
+        ("group_size", ctypes.c_uint64),
+        ("grid_waves", ctypes.c_uint64),
+        ("kernel_waves", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -151,6 +155,10 @@ def load():
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_slot_bits": ([ctypes.c_uint32], ctypes.c_int),
+        "pcg_set_autotune": ([ctypes.c_int], ctypes.c_int),
+        "pcg_set_schedule": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32],
+                             ctypes.c_int),
+        "pcg_set_schedule_cache": ([ctypes.c_char_p, ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
         "pcg_factor_table_check": ([ctypes.c_int, I64, I64, ctypes.POINTER(ctypes.c_uint64)],
                                    ctypes.c_int),
         "pcg_last_stats": ([ctypes.POINTER(Stats)], ctypes.c_int),
@@ -404,6 +412,31 @@ def set_shared_segments(on):
 def set_slot_bits(bits):
     """Bits of C in a slot word (0 = automatic); more bits force escapes."""
     _check(load().pcg_set_slot_bits(int(bits)))
+
+
+TUNE_OFF, TUNE_CACHED, TUNE_ON = 0, 1, 2
+
+
+def set_schedule(words=0, group=0, grid_pct=0, waves=-1):
+    """Pins the sieve launch schedule (0 / -1: left to the schedule); any
+    knob set turns the autotuner off.  set_schedule() restores the default."""
+    _check(load().pcg_set_schedule(int(words), int(group), int(grid_pct), int(waves)))
+
+
+def set_autotune(mode):
+    """Schedule autotuner: TUNE_OFF (the segment-size rule), TUNE_CACHED
+    (default: a cached winner of the launch class, else the rule) or TUNE_ON
+    (also tune classes not cached yet, on the launch itself)."""
+    _check(load().pcg_set_autotune(int(mode)))
+
+
+def set_schedule_cache(path=None):
+    """Cache file of the autotuner (None: the default location); returns the
+    number of winners it holds."""
+    n = ctypes.c_uint64(0)
+    _check(load().pcg_set_schedule_cache(None if path is None else os.fsencode(path),
+                                         ctypes.byref(n)))
+    return n.value
 
 
 def set_pass_slots(n):

@@ -37,6 +37,7 @@ SOURCES = [
     ("capi.cpp", [], False),
     ("serve.cpp", [], False),
     ("verify.cpp", [], False),
+    ("tuner.cpp", [], False),
     ("lds_probe.hip", [], True),
 ]
 

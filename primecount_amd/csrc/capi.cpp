@@ -10,6 +10,7 @@
 #include "params.hpp"
 #include "pix.hpp"
 #include "serve.hpp"
+#include "tuner.hpp"
 
 #include <exception>
 #include <new>
@@ -99,6 +100,8 @@ extern "C" {
 static_assert(kErrInvalid == PCG_E_INVALID && kErrDevice == PCG_E_DEVICE &&
                   kErrNoMem == PCG_E_NOMEM && kErrInternal == PCG_E_INTERNAL,
               "pcg_error status codes are the C ABI's");
+static_assert(kTuneOff == PCG_TUNE_OFF && kTuneCached == PCG_TUNE_CACHED && kTuneOn == PCG_TUNE_ON,
+              "autotune modes are the C ABI's");
 
 const char* pcg_last_error(void) { return t_error.c_str(); }
 
@@ -456,6 +459,41 @@ int pcg_set_segment_words(uint32_t words)
     if (words != 0 && (words < 128 || words > 4096 || (words % 128) != 0))
       throw pcg_error("segment words must be 0 or a multiple of 128 in [128, 4096]");
     engine_config().segment_words = words;
+  });
+}
+
+int pcg_set_schedule(uint32_t words, uint32_t group, uint32_t grid_pct, int32_t waves)
+{
+  return guarded([&] {
+    if (words != 0 && (words < 128 || words > 4096 || (words % 128) != 0))
+      throw pcg_error("segment words must be 0 or a multiple of 128 in [128, 4096]");
+    if (group != 0 && group != 2 && group != 4) throw pcg_error("group must be 0, 2 or 4");
+    if (grid_pct > 100) throw pcg_error("grid percentage must be in [0, 100]");
+    if (waves != -1 && waves != 0 && waves != 6) throw pcg_error("waves must be -1, 0 or 6");
+    auto lock = engine_lock();
+    EngineConfig& c = engine_config();
+    c.segment_words = words;
+    c.group_size = group;
+    c.grid_pct = grid_pct;
+    c.kernel_waves = waves;
+  });
+}
+
+int pcg_set_autotune(int mode)
+{
+  return guarded([&] {
+    if (mode < PCG_TUNE_OFF || mode > PCG_TUNE_ON) throw pcg_error("autotune mode must be 0, 1 or 2");
+    auto lock = engine_lock();
+    engine_config().tune_mode = mode;
+  });
+}
+
+int pcg_set_schedule_cache(const char* path, uint64_t* entries)
+{
+  return guarded([&] {
+    auto lock = engine_lock();
+    tune_set_path(path);
+    if (entries) *entries = tune_entries();
   });
 }
 

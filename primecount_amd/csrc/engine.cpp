@@ -14,12 +14,15 @@
 #include "pix_kernels.hpp"
 #include "params.hpp"
 #include "tables.hpp"
+#include "tuner.hpp"
 
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <functional>
@@ -38,7 +41,13 @@ namespace pcg {
 
 EngineConfig& engine_config()
 {
-  static EngineConfig cfg;
+  static EngineConfig cfg = [] {
+    EngineConfig c;
+    // PCG_AUTOTUNE=0|1|2 sets the initial schedule mode (pcg_set_autotune)
+    if (const char* e = std::getenv("PCG_AUTOTUNE"); e && e[0] >= '0' && e[0] <= '2' && !e[1])
+      c.tune_mode = e[0] - '0';
+    return c;
+  }();
   return cfg;
 }
 
@@ -176,6 +185,7 @@ struct Device {
   bool ready = false;
   int dev = 0;
   int cus = 0;
+  std::string ident;                  // architecture, CU count, board name (schedule cache key)
   hipStream_t stream = nullptr;
   DevBuf primes, factor, pi_bits, pi_cnt;
   DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
@@ -197,6 +207,9 @@ struct Device {
       throw pcg_error(std::string("unsupported device ") + prop.gcnArchName + " (need gfx950)",
                       kErrDevice);
     cus = prop.multiProcessorCount;
+    ident = std::string(prop.gcnArchName) + ":" + std::to_string(cus) + "cu:" + prop.name;
+    for (char& c : ident)
+      if (c == ' ' || c == '\t' || c == '\n') c = '_';
     PCG_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     PCG_HIP(hsl_upload_constants());
     void* sp = nullptr;
@@ -439,42 +452,42 @@ uint64_t escape_bound(const QueryPlan& qp, uint32_t s0, uint32_t s1)
   return n;
 }
 
-/// Segment size.  One wave owns a segment; smaller segments mean more waves
-/// in flight and a finer-grained work queue, larger ones fewer (segment, b)
-/// slots per number.  Measured on the MI355X for D with the default
-/// parameters (one call, ms; the sieve limit is x / z; 10 W + 1,008 bytes of
-/// LDS per wave: 28 waves per CU up to W = 384, 26 at 512, 21 at 640):
+/// Segment size (the schedule's default; the autotuner, tuner.hpp, times
+/// candidates around it).  One wave owns a segment; smaller segments mean
+/// more waves in flight and a finer-grained work queue, larger ones fewer
+/// (segment, b) slots per number.  Measured on the MI355X for D with the
+/// default parameters (one call, kernel + combine ms; the sieve limit is
+/// x / z; 10 W + 1,008 bytes of LDS per wave: 28 waves per CU up to W = 384,
+/// 26 at 512, 21 at 640, and at most 20 for the 81 VGPRs of the kernel):
 ///
 ///   x      limit    W=128   256    384    512    640    768
-///   1e17   8.4e9    16.3   16.2   21.9   27.5   34.3   39.7
-///   1e18   3.1e10   63.6   45.6   44.9   54.4   66.2   74.0
-///   1e19   1.1e11  266.9  185.8  156.2  139.4  137.6  155.0
-///   3e19   2.1e11                        274.3  268.9  278.1
-///   1e20   4.8e11                        572.0  563.0  579.1
+///   1e17   8.4e9    14.5   13.5   18.5   .      .      .
+///   1e18   3.1e10   .      41.7   37.6   47.7   .      .
+///   1e19   1.1e11   .      .     145.0  129.5  119.2  136.3
+///   1e21   1.5e12   .      .      .     2275   2067   2205
 ///
-/// S2_hard (sieve limit z) turns to 640 later:
+/// (an older build of the kernel: 1e20, limit 4.8e11: 572 / 563 / 579 ms at
+/// W = 512 / 640 / 768).  S2_hard (sieve limit z) turns to 640 later:
 ///
-///   x      limit    W=256   384    512    640
-///   1e18   3.4e10   57.2   63.9   76.2   87.7
-///   1e19   1.3e11  241.2  196.5  178.6  205.8
-///   1e20   4.7e11 1037.9  838.5  731.8  711.3
-///   1e21   1.8e12                3212   3097
+///   x      limit    W=128   256    384    512    640    768
+///   1e17   9.4e9    18.2   21.1   27.3   .      .      .
+///   1e19   1.3e11   .      .     184.3  162.6  180.1  210.1
+///   1e21   1.8e12   .      .      .     3082   2766   2865
 ///
 /// Small segments win while a few heavy first segments would otherwise
 /// finish last; in a batch the heavy segments of many queries overlap and
 /// larger segments win, at the same total: the 256 reference x values
-/// (1e12 .. 1e18) in one launch took 1.14, 0.94, 0.83, 0.76, 0.76, 0.80 s at
-/// W = 256, 384, 512, 640, 768, 896.  So the rule takes the sum of the sieve
-/// limits of the launch.
+/// (1e12 .. 1e18) in one launch took 773, 711, 794 ms at W = 512, 640, 768.
+/// So the rule takes the sum of the sieve limits of the launch.
 uint32_t auto_segment_words(uint64_t limit, LeafSum kind)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
-  if (limit < 15000000000ull) return 128;
-  if (limit < 35000000000ull) return 256;
+  const bool d = kind == LeafSum::D;
+  if (limit < (d ? 5000000000ull : 15000000000ull)) return 128;
+  if (limit < (d ? 15000000000ull : 35000000000ull)) return 256;
   if (limit < 80000000000ull) return 384;
-  const uint64_t to640 = kind == LeafSum::D ? 100000000000ull : 300000000000ull;
-  return limit < to640 ? 512 : 640;
+  return limit < (d ? 100000000000ull : 300000000000ull) ? 512 : 640;
 }
 
 /// Table requirements of a group of queries.
@@ -579,7 +592,15 @@ const TableSet& resident_tables(const std::vector<HslQuery>& qs, uint64_t pmin =
 /// written by the sieve kernel and folded by the combine kernels.  Items are
 /// processed in passes of at most kMaxSlotsPerPass slots so that the slot
 /// arrays stay bounded; the per-column carry survives between passes.
-std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint64_t slots_override = 0)
+std::vector<i128> tune_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint64_t slots_override,
+                             const TuneKey& key, const Schedule& def, bool shareable);
+
+/// Sieve-limit sums below this are not tuned: their launches take well under
+/// a millisecond and the rule's schedule is as good as any.
+constexpr uint64_t kTuneMinLimit = 1ull << 30;
+
+std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint64_t slots_override = 0,
+                            const Schedule* forced = nullptr)
 {
   Device& D = g_dev;
   D.init();
@@ -597,19 +618,55 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
 
   std::vector<QueryPlan> plans(qs.size());
   uint64_t sum_limit = 1;
+  uint32_t n_work = 0;
   for (size_t i = 0; i < qs.size(); i++) {
     plans[i] = plan_query(qs[i], T.P);
-    if (!plans[i].empty) sum_limit += plans[i].qd.limit;
+    if (!plans[i].empty) {
+      sum_limit += plans[i].qd.limit;
+      n_work++;
+    }
+  }
+  // The schedule: forced (a tuning trial), set by the caller (segment words),
+  // a cached winner of this launch class, or the rule; with tuning on, a
+  // class not yet cached is tuned on this launch.
+  const EngineConfig& cfg = engine_config();
+  const bool overridden =
+      cfg.segment_words || cfg.group_size || cfg.grid_pct || cfg.kernel_waves >= 0;
+  Schedule sch;
+  sch.W = auto_segment_words(sum_limit, T.kind);
+  int sched_source = 0;
+  if (forced) {
+    sch = *forced;
+  } else if (!overridden && cfg.tune_mode != kTuneOff && sum_limit >= kTuneMinLimit) {
+    TuneKey key;
+    key.device = D.ident;
+    key.kind = T.kind == LeafSum::D ? 0 : 1;
+    key.ft32 = T.factor32 ? 1 : 0;
+    key.lclass = (int) std::floor(2.0 * std::log2((double) sum_limit));
+    key.qclass = std::min(8, 31 - __builtin_clz(std::max<uint32_t>(n_work, 1)));
+    TuneEntry e;
+    if (tune_lookup(key, &e)) {
+      sch = e.s;
+      sched_source = 1;
+    } else if (engine_config().tune_mode == kTuneOn) {
+      return tune_group(qs, st, slots_override, key, sch,
+                        n_work > 1 && engine_config().share_segments);
+    }
+  } else if (overridden) {
+    if (cfg.group_size) sch.gmax = cfg.group_size;
+    if (cfg.grid_pct) sch.grid_pct = cfg.grid_pct;
+    if (cfg.kernel_waves >= 0) sch.waves = (uint32_t) cfg.kernel_waves;
   }
   int bpc = 0;
-  uint32_t W = auto_segment_words(sum_limit, T.kind);
+  uint32_t W = sch.W;
+  const uint32_t waves = sch.waves == 6 ? 6u : 0u;
   if (W < 128 || W > 4096 || (W % 128) != 0)
     throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
   const uint32_t cbits = std::max(slot_format(W).cbits, engine_config().slot_cbits);
   if (cbits > 28) throw pcg_error("slot bits must be at most 28");
-  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
+  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, waves, &bpc));
   bpc = std::max(bpc, 1);
-  const uint64_t slots_wg = (uint64_t) D.cus * bpc;
+  const uint64_t slots_wg = std::max<uint64_t>(1, (uint64_t) D.cus * bpc * sch.grid_pct / 100);
 
   // Queries with work, largest first: items are taken from the work queue in
   // (query, segment) order, so the heaviest items (the first segments of the
@@ -800,7 +857,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   // (64 values: 294 ms against 253 ms).
   std::vector<std::vector<SegGroup>> pass_groups(npass);
   size_t max_groups = 0;
-  const uint32_t gmax = nq >= 128 ? 4u : 2u;
+  const uint32_t gmax = sch.gmax ? sch.gmax : (nq >= 128 ? 4u : 2u);
   if (share) {
     std::vector<uint32_t> order(nq);
     for (uint32_t i = 0; i < nq; i++) order[i] = i;
@@ -914,6 +971,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
       a.group_max = gmax;
     }
     a.W = W;
+    a.waves = share ? 0u : waves;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);
 
     PCG_HIP(hipEventRecord(Wk.event(3 * pi), s));
@@ -991,8 +1049,95 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->device_allocations = g_allocations.load();
     st->escaped_slots = n_escaped;
     st->escape_bound = esc_bound_total;
+    st->schedule_source = (uint64_t) sched_source;
+    st->group_size = share ? gmax : 1;
+    st->grid_waves = slots_wg;
+    st->kernel_waves = share ? 0 : waves;
   }
   return result;
+}
+
+/// Tunes one launch class on the launch itself: the default schedule first
+/// (its sums are the answer), then segment sizes around the rule's, then the
+/// group size of shared segments and a smaller grid, each change kept only
+/// if it is faster.  Every trial must give the same sums as the first (a
+/// schedule never changes results; a difference is an engine fault).  Short
+/// launches are timed three times and take the fastest.  The winner is
+/// cached for the device and class; `st` receives the winner's run.
+std::vector<i128> tune_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint64_t slots_override,
+                             const TuneKey& key, const Schedule& def, bool shareable)
+{
+  std::vector<i128> ref;
+  Schedule best = def;
+  double best_ms = 0;
+  bool have_best = false;
+  pcg_stats best_st{};
+  auto trial = [&](const Schedule& s) {
+    double ms = 1e300;
+    pcg_stats ls{};
+    for (int rep = 0; rep < 3; rep++) {
+      pcg_stats rs{};
+      std::vector<i128> r = run_group(qs, &rs, slots_override, &s);
+      if (ref.empty()) ref = r;
+      else if (r != ref)
+        throw pcg_error("internal: schedule W=" + std::to_string(s.W) + " g=" + std::to_string(s.gmax) +
+                            " grid=" + std::to_string(s.grid_pct) + "% changed a result",
+                        kErrInternal);
+      const double t = (rs.seconds_kernel + rs.seconds_combine) * 1e3;
+      if (t < ms) {
+        ms = t;
+        ls = rs;
+      }
+      if (t >= 20.0) break;                 // long launches: one run is enough
+    }
+    if (!have_best || ms < best_ms) {
+      best = s;
+      best_ms = ms;
+      best_st = ls;
+      have_best = true;
+    }
+    return ms;
+  };
+  const double def_ms = trial(def);
+  static const uint32_t kWords[] = {128, 256, 384, 512, 640, 768, 1024};
+  const int nw = (int) (sizeof kWords / sizeof kWords[0]);
+  int i0 = 0;
+  while (i0 + 1 < nw && kWords[i0] < def.W) i0++;
+  for (int i = std::max(0, i0 - 2); i <= std::min(nw - 1, i0 + 2); i++) {
+    if (kWords[i] == def.W) continue;
+    Schedule s = def;
+    s.W = kWords[i];
+    trial(s);
+  }
+  if (shareable) {
+    const uint32_t g0 = best_st.group_size;
+    for (uint32_t g : {2u, 4u}) {
+      if (g == g0) continue;
+      Schedule s = best;
+      s.gmax = g;
+      trial(s);
+    }
+  }
+  if (!shareable) {
+    Schedule s = best;
+    s.waves = best.waves == 6 ? 0u : 6u;
+    trial(s);
+  }
+  {
+    Schedule s = best;
+    s.grid_pct = 75;
+    trial(s);
+  }
+  TuneEntry e;
+  e.s = best;
+  e.ms_default = def_ms;
+  e.ms_best = best_ms;
+  tune_store(key, e);
+  if (st) {
+    *st = best_st;
+    st->schedule_source = 2;
+  }
+  return ref;
 }
 
 /// Queries can share a D factor table when y >= z (then no entry <= z
@@ -1164,7 +1309,7 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
                                  (const uint64_t*) (base + o_sfx), (SegItem*) (base + o_items),
                                  nitems, s));
   int bpc = 0;
-  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, &bpc));
+  PCG_HIP(hsl_sieve_occupancy(T.factor32, W, 0, &bpc));
   bpc = std::max(bpc, 1);
   HslArgs a{};
   a.queries = (const QueryDesc*) (base + o_qd);
@@ -1272,6 +1417,10 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.queue_steps += st.queue_steps;
     acc.escaped_slots += st.escaped_slots;
     acc.escape_bound += st.escape_bound;
+    acc.schedule_source = std::max(acc.schedule_source, st.schedule_source);
+    acc.group_size = std::max(acc.group_size, st.group_size);
+    acc.grid_waves = std::max(acc.grid_waves, st.grid_waves);
+    acc.kernel_waves = std::max(acc.kernel_waves, st.kernel_waves);
   }
   acc.device_allocations = g_allocations.load();
   acc.seconds_total = now() - t0;

@@ -30,6 +30,11 @@ struct EngineConfig {
   bool share_segments = true;   // queries of a launch share the sieve of a segment
   uint32_t slot_cbits = 0;      // 0 = automatic; else bits of C in a slot word (at least
                                 // what the segment size needs): fewer bits for M, more escapes
+  int tune_mode = 1;            // TuneMode (tuner.hpp): 0 rule, 1 cached winners, 2 tune new classes
+  // schedule overrides (pcg_set_schedule); any of them set turns tuning off
+  uint32_t group_size = 0;      // 0 = schedule's; 2 or 4 queries per shared segment
+  uint32_t grid_pct = 0;        // 0 = schedule's; else percent of the occupancy grid
+  int kernel_waves = -1;        // -1 = schedule's; 0 or 6 (single-query kernel build)
 };
 
 EngineConfig& engine_config();

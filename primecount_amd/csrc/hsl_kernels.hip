@@ -397,7 +397,7 @@ __device__ __forceinline__ uint32_t pack_slot(uint32_t C, int32_t M, uint32_t cb
 /// pointers so the compiler knows the tables are never written by the kernel
 /// and can read uniform entries (BInfo, work items) through the scalar cache.
 template <typename FT, int NQ>
-__global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
+__device__ __forceinline__ void sieve_segments(
     const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
     const SegGroup* __restrict__ groups, const BInfo* __restrict__ binfo_all,
     const uint32_t* __restrict__ primes, const FT* __restrict__ factor,
@@ -941,13 +941,52 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(
   flush_counters();
 }
 
+#define PCG_SIEVE_PARAMS                                                                       \
+    const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,                \
+    const SegGroup* __restrict__ groups, const BInfo* __restrict__ binfo_all,                \
+    const uint32_t* __restrict__ primes, const FT* __restrict__ factor,                      \
+    const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,               \
+    uint32_t* __restrict__ T_S, EscTable esc, unsigned long long* __restrict__ S_acc,        \
+    unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,             \
+    uint32_t nitems, uint32_t W, uint32_t cbits
+#define PCG_SIEVE_ARGS                                                                         \
+    queries, items, groups, binfo_all, primes, factor, pi_bits, pi_cnt, T_S, esc, S_acc,     \
+    counters, queue, nitems, W, cbits
+
+template <typename FT, int NQ>
+__global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(PCG_SIEVE_PARAMS)
+{
+  sieve_segments<FT, NQ>(PCG_SIEVE_ARGS);
+}
+
+/// The single-query kernel compiled for at least 6 waves per SIMD: the
+/// compiler keeps it within 80 VGPRs (the unconstrained build takes 81, which
+/// allows 5), so segments small enough for LDS to allow more than 20 waves
+/// per CU get them.  A schedule choice (HslArgs::waves); results are the
+/// same.
+template <typename FT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6)))
+void hsl_sieve_kernel_w6(PCG_SIEVE_PARAMS)
+{
+  sieve_segments<FT, 1>(PCG_SIEVE_ARGS);
+}
+
+#undef PCG_SIEVE_PARAMS
+#undef PCG_SIEVE_ARGS
+
 template <typename FT, int NQ>
 static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st)
 {
-  hipLaunchKernelGGL((hsl_sieve_kernel<FT, NQ>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
-                     a.queries, a.items, a.groups, a.binfo, a.primes,
-                     reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
-                     a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
+  if (NQ == 1 && a.waves == 6)
+    hipLaunchKernelGGL((hsl_sieve_kernel_w6<FT>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
+                       a.queries, a.items, a.groups, a.binfo, a.primes,
+                       reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
+                       a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
+  else
+    hipLaunchKernelGGL((hsl_sieve_kernel<FT, NQ>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
+                       a.queries, a.items, a.groups, a.binfo, a.primes,
+                       reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
+                       a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
   return hipGetLastError();
 }
 
@@ -1254,9 +1293,14 @@ hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipS
   return factor32 ? launch_sieve_t<uint32_t, 1>(a, grid, st) : launch_sieve_t<uint16_t, 1>(a, grid, st);
 }
 
-hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu)
+hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, uint32_t waves, int* blocks_per_cu)
 {
   size_t lds = hsl_lds_bytes(W);
+  if (waves == 6)
+    return factor32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          blocks_per_cu, hsl_sieve_kernel_w6<uint32_t>, kBlock, lds)
+                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                          blocks_per_cu, hsl_sieve_kernel_w6<uint16_t>, kBlock, lds);
   if (factor32)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint32_t, 1>, kBlock, lds);
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint16_t, 1>, kBlock, lds);

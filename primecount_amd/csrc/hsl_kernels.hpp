@@ -139,6 +139,7 @@ struct HslArgs {
   uint32_t nitems;              // work items: SegItems, or SegGroups when groups is set
   uint32_t group_max;           // 2 or 4: the largest group (when groups is set)
   uint32_t W;                   // words per segment (multiple of 128, 128..4096)
+  uint32_t waves;               // 6: the single-query kernel built for 6 waves per SIMD, else 0
 };
 
 constexpr int kBlock = 64;     // one wave per segment
@@ -174,7 +175,10 @@ hipError_t hsl_upload_constants();
 /// looked up with hipGetSymbolAddress on the first call for that device.
 hipError_t hsl_scratch_address(int device, void** ptr);
 hipError_t hsl_launch_sieve(const HslArgs& a, bool factor32, uint32_t grid, hipStream_t st);
-hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, int* blocks_per_cu);
+/// Sieve workgroups (one wave each) a CU holds at once for W words per
+/// segment, for the single-query kernel (waves = 6: its 6-waves-per-SIMD
+/// build).
+hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, uint32_t waves, int* blocks_per_cu);
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st);
 
 /// Factor table entries for the indices 0 .. ft_to_index(limit) (numbers

@@ -178,3 +178,33 @@ def test_status_codes_are_typed(pc):
         with pytest.raises(pc.PrimecountError) as e:
             pc.D(10 ** 12, 76500, 76500, 8)
         assert e.value.status == pc.E_DEVICE
+
+
+def test_schedule_cache_file(pc, tmp_path):
+    """The autotuner's cache file: one winner per (device, class) line, a
+    later line replacing an earlier one of the same class, lines with a
+    schedule the engine would reject or another format skipped."""
+    dev = "gfx950:256cu:test_board"
+    p = tmp_path / "schedules.txt"
+    p.write_text("\n".join([
+        "pcg-schedule-v1 %s 0 0 73 0 512 0 100 0 120.5 118.0" % dev,
+        "pcg-schedule-v1 %s 0 0 73 0 640 0 100 6 120.5 117.0" % dev,   # same class
+        "pcg-schedule-v1 %s 1 0 73 0 768 2 75 0 1.0 0.9" % dev,
+        "pcg-schedule-v1 other:8cu:x 1 0 73 0 768 2 75 0 1.0 0.9",      # another device
+        "pcg-schedule-v1 %s 1 0 74 0 1000 2 75 0 1.0 0.9" % dev,       # W not 128 k
+        "pcg-schedule-v1 %s 1 0 75 0 512 2 0 0 1.0 0.9" % dev,         # no grid
+        "pcg-schedule-v1 %s 1 0 76 0 512 9 100 0 1.0 0.9" % dev,       # group of 9
+        "pcg-schedule-v1 %s 1 0 78 0 512 2 100 7 1.0 0.9" % dev,       # 7 waves: no build
+        "pcg-schedule-v1 %s 1 0 77 0" % dev,                           # cut short
+        "garbage",
+        "pcg-schedule-v0 %s 0 0 1 0 512 0 100 0 1 1" % dev,
+    ]) + "\n")
+    try:
+        assert pc.set_schedule_cache(str(p)) == 3
+        assert pc.set_schedule_cache(str(tmp_path / "absent.txt")) == 0
+        with pytest.raises(pc.PrimecountError) as e:
+            pc.set_autotune(3)
+        assert e.value.status == pc.E_INVALID
+    finally:
+        pc.set_schedule_cache(None)
+        pc.set_autotune(pc.TUNE_CACHED)
