@@ -32,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "hsl_kernels.hpp"
 #include "quotient_fixup.hpp"
@@ -41,6 +42,10 @@ namespace pcg {
 
 __constant__ int16_t c_cop_index[2310];   // largest index of a coprime <= r
 __constant__ uint16_t c_cop[480];         // coprimes to 2310 in [0, 2310)
+// c_cop continued for 64 entries past one period: entry i is the i-th number
+// coprime to 2310 (i < 544), so a wave's 64 consecutive factor-table indices
+// starting at residue r < 480 of a period read entries r .. r + 63
+__constant__ uint16_t c_cop_ext[544];
 
 /// Device memory of the no-allocation entry point (pcg_D_i64_noalloc): its
 /// tables and work buffers, kHslScratchBytes per device for the life of the
@@ -470,6 +475,7 @@ __device__ __forceinline__ void sieve_segments(
     uint32_t b_stop_m[NQ], stop_b[NQ];
     bool live[NQ];
     long long S[NQ];
+
     uint32_t b_stop = 0;
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
@@ -616,6 +622,8 @@ __device__ __forceinline__ void sieve_segments(
       const unsigned long long in_win = nwin >= 64 ? ~0ull : ((1ull << nwin) - 1);
       // par (monotone in p) holds for the primes [seq_end, nwin) of the window
       const unsigned long long pbal = __ballot(par) & in_win;
+      // primes with cofactors below p (q_lo < p)
+      const unsigned long long zmask = __ballot(prm.z != 0) & in_win;
       const uint32_t seq_end = pbal ? (uint32_t) __ffsll((long long) pbal) - 1 : nwin;
       // primes with factor entries of any member; the cross-offs between two
       // of them are applied together just before the count of the second
@@ -673,7 +681,6 @@ __device__ __forceinline__ void sieve_segments(
             const uint32_t cls = lane & 7;
             const uint32_t r = wheel_residue(cls);
             const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;   // < 30
-            const uint32_t n = delta < nq ? (nq - 1 - delta) / 30 + 1 : 0u;   // class count
             // byte and bit of the class's first multiple p (q_lo + delta):
             // t = cc0 + rp delta < 871, so t / 30 = (t 2185) >> 16
             const uint32_t t = ((py >> 16) & 31u) + __umul24(rp, delta);
@@ -684,19 +691,23 @@ __device__ __forceinline__ void sieve_segments(
             // bit within the word does not change
             const uint32_t B0 = (py & 0xFFFFu) + __umul24(a, delta) + tq + j0 * p;
             const uint32_t mask = 1u << (((B0 & 3) << 3) | wheel_bit(t - 30 * tq));
-            const uint32_t nmine = n > j0 ? (n - j0 + 7) >> 3 : 0u;
             const uint32_t nmax = (nq + 29) / 30;       // >= every class count
             const uint32_t nsteps = (nmax + 7) / 8;
             u_dsteps += nsteps;
             const uint32_t wstep = 2 * p;
             uint32_t wi = B0 >> 2;
-            // two multiples per lane per step: both atomics are in flight
-            // together (one prime's removals may be counted in any order)
+            // Two multiples per lane per step: both atomics are in flight
+            // together (one prime's removals may be counted in any order).
+            // A lane stops at the end of the bitmap instead of at its class
+            // count: a multiple p q with q > q_hi lies at or beyond `high`,
+            // so it is outside the bitmap or on a bit that is already clear
+            // (the numbers of a short last segment from seg_len on).
+            const uint32_t nw = 2 * W;
             for (uint32_t u = 0; u < nsteps; u += 2) {
               uint32_t o0 = 0, o1 = 0;
-              if (u < nmine)
+              if (wi < nw)
                 o0 = atomicAnd(&bm[wi], ~mask);
-              if (u + 1 < nmine)
+              if (wi + wstep < nw)
                 o1 = atomicAnd(&bm[wi + wstep], ~mask);
               removed += (uint32_t) __popcll(__ballot((o0 & mask) != 0)) +
                          (uint32_t) __popcll(__ballot((o1 & mask) != 0));
@@ -710,51 +721,71 @@ __device__ __forceinline__ void sieve_segments(
         if (d_end >= r1)
           return;
         const bool ordered = d_end < seq_end;
-        for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
-          u_gsteps++;
-          const uint32_t jo = jb + grp;
-          // the cross-off data of prime jo, read from lane jo (all lanes active)
-          const int src = (int) ((jo < r1 ? jo : r0) << 2);
-          uint4 pp;
-          pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
-          pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
-          pp.z = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z);
-          pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
-          const uint32_t kk = lane & (G - 1);
-          const bool has = jo < r1 && kk < pp.w;
-          // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
-          // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
-          // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
-          const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
-          const uint32_t idx = (pp.y >> 26) + kk;
-          const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
-          const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
-          const uint32_t tq = (t * 2185u) >> 16;
-          const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
-          const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(t - 30 * tq));
-          uint32_t old = 0;
-          if (!ordered) {
-            if (has)
-              old = atomicAnd(&bm[B >> 2], ~mask);
-          } else {
-            for (uint32_t u = 0; u < ngrp; u++) {
-              if (has && grp == u)
+        // a removal charged to a cofactor q < p needs a prime with q_lo < p
+        const uint32_t np_part = r1 - d_end;
+        const unsigned long long part =
+            (np_part >= 64 ? ~0ull : ((1ull << np_part) - 1)) << d_end;
+        const bool attr_any = !ordered && (zmask & part) != 0;
+        unsigned long long fresh_any = 0;
+        const uint32_t kk = lane & (G - 1);
+        // one loop per kind of part, so that the per-step work has no
+        // uniform branches on the kind
+        auto steps = [&](auto ord_c, auto attr_c) {
+          constexpr bool ORD = decltype(ord_c)::value, ATTR = decltype(attr_c)::value;
+          for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
+            u_gsteps++;
+            const uint32_t jo = jb + grp;
+            // the cross-off data of prime jo, read from lane jo (all lanes active)
+            const int src = (int) ((jo < r1 ? jo : r0) << 2);
+            uint4 pp;
+            pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
+            pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
+            pp.z = ATTR ? (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z) : 0u;
+            pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
+            const bool has = jo < r1 && kk < pp.w;
+            // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
+            // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
+            // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
+            const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
+            const uint32_t idx = (pp.y >> 26) + kk;
+            const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
+            const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
+            const uint32_t tq = (t * 2185u) >> 16;
+            const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
+            const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(t - 30 * tq));
+            uint32_t old = 0;
+            if constexpr (ORD) {
+              // the groups of this step one after the other
+              const uint32_t ng = min(ngrp, r1 - jb);
+              for (uint32_t u = 0; u < ng; u++) {
+                if (has && grp == u)
+                  old = atomicAnd(&bm[B >> 2], ~mask);
+                wave_lds_sync();
+              }
+            } else {
+              if (has)
                 old = atomicAnd(&bm[B >> 2], ~mask);
-              wave_lds_sync();
+            }
+            const bool fresh = has && (old & mask);
+            fresh_any |= __ballot(fresh);
+            if (fresh) {
+              uint32_t attr = jo;
+              if constexpr (ATTR) {
+                if (delta < pp.z) {                     // q = q_lo + delta < p
+                  const uint32_t q = 30 * a + rp - pp.z + delta;
+                  if (q > 1)
+                    attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
+                }
+              }
+              atomicAdd(&w_rem[attr], 1u);
             }
           }
-          const bool fresh = has && (old & mask);
-          dirty |= __ballot(fresh) != 0;
-          if (fresh) {
-            uint32_t attr = jo;
-            if (!ordered && delta < pp.z) {             // q = q_lo + delta < p
-              const uint32_t q = 30 * a + rp - pp.z + delta;
-              if (q > 1)
-                attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
-            }
-            atomicAdd(&w_rem[attr], 1u);
-          }
-        }
+        };
+        if (ordered)
+          steps(std::true_type{}, std::false_type{});
+        else
+          steps(std::false_type{}, std::true_type{});
+        dirty |= fresh_any != 0;
       };
 
       auto next_leaf = [&](uint32_t c) -> uint32_t {
@@ -830,44 +861,57 @@ __device__ __forceinline__ void sieve_segments(
               if (lane < n) {
                 const uint32_t e = lq[(qt + lane) & 127];
                 const uint32_t m = (e << 1) | 1u;
-                const uint32_t cnt = seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
-                S[k] += (e >> 31) ? (long long) cnt : -(long long) cnt;
+                const int cnt = (int) seg_count(bm, pre2, lut, leaf_stop(xlo, low, m));
+                S[k] += (long long) ((e >> 31) ? cnt : -cnt);
               }
               wave_lds_sync();
               qt += n;
               u_qsteps++;
             };
-            // entries and coprime residues of the next step are loaded
-            // before this step is filtered, queued and counted
+            // The factor-table index of the wave's step, idx = base + lane,
+            // is walked as a scalar period start mb = 2310 (base / 480) and
+            // residue rb = base % 480, so m = mb + c_cop_ext[rb + lane].
+            // Entries and coprimes of the next step are loaded before this
+            // step is filtered, queued and counted.
             uint32_t f = first[k];
-            uint32_t cop = c_cop[(lo + 1 + lane) % 480];
+            uint32_t np_tot = 0;                          // wave-uniform
+            uint32_t rb = (lo + 1) % 480, mb = 2310 * ((lo + 1) / 480);
+            uint32_t cop = c_cop_ext[rb + lane];
 #pragma nounroll
             for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
-              const uint32_t idx = base + lane;
-              const uint32_t nidx = idx + kBlock;
+              const uint32_t nidx = base + lane + kBlock;
+              // read in bounds (index clamped to hi), then zeroed beyond hi:
+              // no branch around the load
               const uint32_t f_next = nidx <= hi ? (uint32_t) factor[nidx] : 0u;
-              const uint32_t cop_next = c_cop[nidx % 480];
+              uint32_t rbn = rb + kBlock, mbn = mb;
+              if (rbn >= 480) {
+                rbn -= 480;
+                mbn += 2310;
+              }
+              const uint32_t cop_next = c_cop_ext[rbn + lane];
               const bool leaf = f > T;
-              const bool pos = leaf && (f & 1);
               const unsigned long long lb = __ballot(leaf);
               const uint32_t nl = (uint32_t) __popcll(lb);
-              const uint32_t np = (uint32_t) __popcll(__ballot(pos));
-              u_leaves += nl;
-              M += (int) (2 * np) - (int) nl;
+              np_tot += (uint32_t) __popcll(__ballot(leaf && (f & 1)));
               if (leaf) {
                 const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
                     (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
-                const uint32_t m = 2310 * (idx / 480) + cop;
-                lq[(qh + rank) & 127] = (m >> 1) | (pos ? 0x80000000u : 0u);
+                const uint32_t m = mb + cop;
+                lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
               }
               qh += nl;
               if (qh - qt >= 64)
                 drain(64);
               f = f_next;
               cop = cop_next;
+              rb = rbn;
+              mb = mbn;
             }
             if (qh != qt)
               drain(qh - qt);
+            // qh counts the prime's leaves, np_tot the positive ones
+            u_leaves += qh;
+            M = (int) (2 * np_tot) - (int) qh;
           } else {
             // primes q: leaves +phi(xp / q, b - 1) (src/gourdon/D.cpp:279-302)
             u_leaves += hi - lo;
@@ -1277,8 +1321,11 @@ hipError_t hsl_upload_constants()
     }
     ci[n] = (int16_t) idx;
   }
+  uint16_t cx[544];
+  for (int i = 0; i < 544; i++) cx[i] = (uint16_t) (cp[i % 480] + (i >= 480 ? 2310 : 0));
   hipError_t e;
   if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cop_index), ci, sizeof(ci))) != hipSuccess) return e;
+  if ((e = hipMemcpyToSymbol(HIP_SYMBOL(c_cop_ext), cx, sizeof(cx))) != hipSuccess) return e;
   return hipMemcpyToSymbol(HIP_SYMBOL(c_cop), cp, sizeof(cp));
 }
 
