@@ -190,6 +190,15 @@ int pcg_set_pass_slots(uint64_t slots);
  * in the reference. */
 int pcg_set_slot_bits(uint32_t cbits);
 
+/* Batched leaf counting (default on): in the part of a window of 64 sieving
+ * primes where cross-off order does not matter, the leaves of several primes
+ * are counted against one word-pair prefix build, each corrected by the
+ * recorded first-time removals of the primes between it and the build.
+ * Never changes results; it exists to compare the two paths.  No
+ * counterpart in the reference (its leaves are counted one prime at a time,
+ * src/gourdon/D.cpp:122-273). */
+int pcg_set_leaf_batching(int on);
+
 /* Host threads used by the table builders (0 = all hardware threads). */
 int pcg_set_host_threads(int threads);
 
@@ -231,6 +240,10 @@ typedef struct pcg_stats {
   uint64_t grid_waves;       /* waves of the persistent sieve grid (at most)   */
   uint64_t kernel_waves;     /* 6: the sieve kernel built for 6 waves per SIMD,
                                 0: the unconstrained build                    */
+  uint64_t leaf_batches;     /* batches of primes whose leaves were counted on
+                                one prefix build (pcg_set_leaf_batching)      */
+  uint64_t batch_records;    /* first-time removals recorded for the batches'
+                                corrections                                   */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

@@ -80,6 +80,8 @@ class Stats(ctypes.Structure):
         ("group_size", ctypes.c_uint64),
         ("grid_waves", ctypes.c_uint64),
         ("kernel_waves", ctypes.c_uint64),
+        ("leaf_batches", ctypes.c_uint64),
+        ("batch_records", ctypes.c_uint64),
     ]
 
     def as_dict(self):
@@ -154,6 +156,7 @@ def load():
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
+        "pcg_set_leaf_batching": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_slot_bits": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_set_autotune": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_schedule": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32],
@@ -407,6 +410,12 @@ def factor_table_check(kind, y, limit):
 
 def set_shared_segments(on):
     _check(load().pcg_set_shared_segments(1 if on else 0))
+
+
+def set_leaf_batching(on):
+    """Count the leaves of several primes on one prefix build (default on);
+    results are the same either way."""
+    _check(load().pcg_set_leaf_batching(1 if on else 0))
 
 
 def set_slot_bits(bits):

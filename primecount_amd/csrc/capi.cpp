@@ -512,6 +512,11 @@ int pcg_set_shared_segments(int on)
   return guarded([&] { engine_config().share_segments = on != 0; });
 }
 
+int pcg_set_leaf_batching(int on)
+{
+  return guarded([&] { engine_config().leaf_batching = on != 0; });
+}
+
 int pcg_set_pass_slots(uint64_t slots)
 {
   return guarded([&] {

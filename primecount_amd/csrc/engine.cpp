@@ -972,6 +972,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     }
     a.W = W;
     a.waves = share ? 0u : waves;
+    a.bcap = engine_config().leaf_batching ? kBatchRecords : 0u;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);
 
     PCG_HIP(hipEventRecord(Wk.event(3 * pi), s));
@@ -989,7 +990,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
 
   std::vector<i128> col_out(ncols_total);
   std::vector<uint64_t> sacc(2 * nq);
-  uint64_t counters[12] = {};
+  uint64_t counters[14] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), Wk.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(sacc.data(), Wk.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, Wk.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
@@ -1046,6 +1047,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->group_steps = counters[9];
     st->prefix_builds = counters[10];
     st->queue_steps = counters[11];
+    st->leaf_batches = counters[12];
+    st->batch_records = counters[13];
     st->device_allocations = g_allocations.load();
     st->escaped_slots = n_escaped;
     st->escape_bound = esc_bound_total;
@@ -1331,6 +1334,7 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
   a.queue = (unsigned int*) (base + o_queue);
   a.nitems = nitems;
   a.W = W;
+  a.bcap = engine_config().leaf_batching ? kBatchRecords : 0u;
   const uint32_t grid = (uint32_t) std::min<uint64_t>((uint64_t) D.cus * bpc, nitems);
   PCG_HIP(hsl_launch_sieve(a, T.factor32, grid, s));
   PCG_HIP(hsl_launch_combine((const CombineCol*) (base + o_cols), ncols, a.items, a.T_S, a.esc,
@@ -1338,7 +1342,7 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
                              (long long*) (base + o_sm), base + o_smp,
                              (long long*) (base + o_carry), base + o_out, s));
   std::vector<i128> col_out(ncols);
-  uint64_t sacc[2] = {}, counters[12] = {};
+  uint64_t sacc[2] = {}, counters[14] = {};
   uint32_t ectl[2] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), base + o_out, (size_t) ncols * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(sacc, base + o_sacc, 16, hipMemcpyDeviceToHost, s));
@@ -1415,6 +1419,8 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.group_steps += st.group_steps;
     acc.prefix_builds += st.prefix_builds;
     acc.queue_steps += st.queue_steps;
+    acc.leaf_batches += st.leaf_batches;
+    acc.batch_records += st.batch_records;
     acc.escaped_slots += st.escaped_slots;
     acc.escape_bound += st.escape_bound;
     acc.schedule_source = std::max(acc.schedule_source, st.schedule_source);

@@ -176,9 +176,8 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v)
 
 /// build_pair_prefix for PER = W / 128 pairs per lane known at compile time:
 /// the lane's PER 128-bit loads are issued together, then counted.
-template <int PER>
-__device__ __forceinline__ void build_pair_prefix_n(const uint32_t* bm, uint32_t* pre2,
-                                                    uint32_t lane)
+template <int PER, typename PT>
+__device__ __forceinline__ void build_pair_prefix_n(const uint32_t* bm, PT* pre2, uint32_t lane)
 {
   const uint4* w4 = reinterpret_cast<const uint4*>(bm) + lane * PER;
   uint4 q[PER];
@@ -193,10 +192,10 @@ __device__ __forceinline__ void build_pair_prefix_n(const uint32_t* bm, uint32_t
     tot += c[i];
   }
   uint32_t run = wave_incl_scan(tot) - tot;
-  uint32_t* out = pre2 + lane * PER;
+  PT* out = pre2 + lane * PER;
 #pragma unroll
   for (int i = 0; i < PER; i++) {
-    out[i] = run;
+    out[i] = (PT) run;
     run += c[i];
   }
 }
@@ -206,7 +205,8 @@ __device__ __forceinline__ void build_pair_prefix_n(const uint32_t* bm, uint32_t
 /// 128-bit LDS load each), then a wave scan gives every lane its base.  The
 /// segment sizes up to 640 words take a copy specialised for their pair
 /// count (wave-uniform switch).
-__device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* pre2, uint32_t W,
+template <typename PT>
+__device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, PT* pre2, uint32_t W,
                                                   uint32_t lane)
 {
   switch (W >> 7) {
@@ -219,7 +219,7 @@ __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* 
   }
   const uint32_t per = W >> 7;
   const uint4* w4 = reinterpret_cast<const uint4*>(bm) + lane * per;
-  uint32_t* out = pre2 + lane * per;
+  PT* out = pre2 + lane * per;
   if (per <= 8) {
     uint32_t c[8];
     uint32_t tot = 0;
@@ -237,7 +237,7 @@ __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* 
 #pragma unroll
     for (uint32_t i = 0; i < 8; i++) {
       if (i < per)
-        out[i] = run;
+        out[i] = (PT) run;
       run += c[i];
     }
   } else {
@@ -249,7 +249,7 @@ __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* 
     uint32_t run = wave_incl_scan(tot) - tot;
     for (uint32_t i = 0; i < per; i++) {
       const uint4 q = w4[i];
-      out[i] = run;
+      out[i] = (PT) run;
       run += __popc(q.x) + __popc(q.y) + __popc(q.z) + __popc(q.w);
     }
   }
@@ -260,7 +260,8 @@ __device__ __forceinline__ void build_pair_prefix(const uint32_t* bm, uint32_t* 
 /// the whole first word of the pair when stop lies in its second word, and
 /// the low lut[stop % 240] bits of the word of stop (lut[r] = sieve bits of
 /// the offsets <= r in a word).
-__device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t* pre2,
+template <typename PT>
+__device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const PT* pre2,
                                               const uint8_t* lut, uint32_t stop)
 {
   const uint32_t w = stop / 240;
@@ -269,8 +270,27 @@ __device__ __forceinline__ uint32_t seg_count(const uint32_t* bm, const uint32_t
   const uint4 q = reinterpret_cast<const uint4*>(bm)[g];
   const uint32_t n = lut[r];
   const bool odd = (w & 1) != 0;
-  const uint32_t c = pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
+  const uint32_t c = (uint32_t) pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
   const uint64_t word = odd ? (((uint64_t) q.w << 32) | q.z) : (((uint64_t) q.y << 32) | q.x);
+  return c + (n ? (uint32_t) __popcll(word << (64 - n)) : 0u);
+}
+
+/// seg_count that also gives the bit-index bound of the count: the counted
+/// numbers are the unsieved ones among the bits 64 w + i, i < lut[r], of the
+/// segment, so a bit index below `bound` lies at or below stop.
+template <typename PT>
+__device__ __forceinline__ uint32_t seg_count_b(const uint32_t* bm, const PT* pre2,
+                                                const uint8_t* lut, uint32_t stop, uint32_t& bound)
+{
+  const uint32_t w = stop / 240;
+  const uint32_t r = stop - w * 240;
+  const uint32_t g = w >> 1;
+  const uint4 q = reinterpret_cast<const uint4*>(bm)[g];
+  const uint32_t n = lut[r];
+  const bool odd = (w & 1) != 0;
+  const uint32_t c = (uint32_t) pre2[g] + (odd ? __popc(q.x) + __popc(q.y) : 0u);
+  const uint64_t word = odd ? (((uint64_t) q.w << 32) | q.z) : (((uint64_t) q.y << 32) | q.x);
+  bound = (w << 6) + n;
   return c + (n ? (uint32_t) __popcll(word << (64 - n)) : 0u);
 }
 
@@ -409,9 +429,10 @@ __device__ __forceinline__ void sieve_segments(
     const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,
     uint32_t* __restrict__ T_S, EscTable esc, unsigned long long* __restrict__ S_acc,
     unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,
-    uint32_t nitems, uint32_t W, uint32_t cbits)
+    uint32_t nitems, uint32_t W, uint32_t cbits, uint32_t bcap)
 {
   extern __shared__ __align__(16) uint32_t lds[];
+  constexpr bool BATCH = NQ == 1;                             // batched leaf counting
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
   uint8_t* lut = reinterpret_cast<uint8_t*>(pre2 + W / 2);    // 240 u8
@@ -422,8 +443,17 @@ __device__ __forceinline__ void sieve_segments(
   // grouped cross-offs fetch their prime's data from its lane (ds_bpermute).
   __shared__ uint32_t w_rem[64];                              // per window prime: removals
   __shared__ uint32_t lq[128];                                // leaf queue: factor index | sign << 31
+  // batched leaf counting (single-query kernel): the window lane (prime) of
+  // each queued leaf, and the fresh removals recorded for the corrections
+  __shared__ uint8_t lqb[BATCH ? 128 : 1];
+  __shared__ uint16_t blist[BATCH ? kBatchRecords : 1];
 
   const uint32_t lane = threadIdx.x;
+  // batched leaf counting (16-bit bit indices, W <= 1024) records at most
+  // kBatchRecords removals
+  if (!BATCH || W > 1024)
+    bcap = 0;
+  bcap = min(bcap, kBatchRecords);
   for (uint32_t r = lane; r < 240; r += kBlock)
     lut[r] = (uint8_t) (8 * (r / 30) + le_count(r % 30));
 
@@ -437,17 +467,21 @@ __device__ __forceinline__ void sieve_segments(
   // pair-prefix builds
   uint32_t u_win = 0, u_lprimes = 0, u_lsteps = 0, u_dprimes = 0, u_dsteps = 0, u_gsteps = 0,
            u_pbuilds = 0, u_qsteps = 0;
+  // leaf batches and the first-time removals recorded for their corrections
+  uint32_t u_batch = 0, u_brec = 0;
   auto flush_counters = [&]() {
     if (lane == 0) {
-      const uint32_t v[12] = {u_leaves, u_hits, u_entries, n_slots, u_win, u_lprimes,
-                              u_lsteps, u_dprimes, u_dsteps, u_gsteps, u_pbuilds, u_qsteps};
+      const uint32_t v[14] = {u_leaves, u_hits, u_entries, n_slots, u_win, u_lprimes,
+                              u_lsteps, u_dprimes, u_dsteps, u_gsteps, u_pbuilds, u_qsteps,
+                              u_batch, u_brec};
 #pragma unroll
-      for (int i = 0; i < 12; i++)
+      for (int i = 0; i < 14; i++)
         if (v[i])
           atomicAdd(&counters[i], (unsigned long long) v[i]);
     }
     n_slots = u_hits = u_leaves = u_entries = 0;
     u_win = u_lprimes = u_lsteps = u_dprimes = u_dsteps = u_gsteps = u_pbuilds = u_qsteps = 0;
+    u_batch = u_brec = 0;
   };
 
   for (;;) {
@@ -538,6 +572,8 @@ __device__ __forceinline__ void sieve_segments(
       uint64_t r_xlo[NQ];
       bool brk[NQ];
       uint32_t r_nq = 0, r_h = 0, r_p = 0;
+      // cofactor range [q_lo, q_hi] clipped to 32 bits (empty: lo > hi)
+      uint32_t cq_lo = 0xFFFFFFFFu, cq_hi = 0;
       bool par = false;
       uint4 prm = make_uint4(0, 0, 0, 0);
 #pragma unroll
@@ -584,6 +620,10 @@ __device__ __forceinline__ void sieve_segments(
         const uint64_t q_lo = div_exact_r(low_d, low, p, inv_p) + 1;
         const uint64_t q_hi = div_exact_r(last_d, high - 1, p, inv_p);
         r_nq = q_hi >= q_lo ? (uint32_t) (q_hi - q_lo + 1) : 0u;
+        if (r_nq > 0) {
+          cq_lo = q_lo < 0xFFFFFFFFull ? (uint32_t) q_lo : 0xFFFFFFFFu;
+          cq_hi = q_hi < 0xFFFFFFFFull ? (uint32_t) q_hi : 0xFFFFFFFFu;
+        }
         const uint32_t qm = (uint32_t) (q_lo % 30);
         const uint32_t off0 = (uint32_t) (p * q_lo - low);   // < 2^20 when nq > 0
         // Every cofactor is below p^2: a number of the segment that is still
@@ -634,7 +674,8 @@ __device__ __forceinline__ void sieve_segments(
       const uint32_t h_win = lane < nwin ? r_h : 0u;
       // every prime of the window is crossed off by the window's end: its
       // hits are its cofactors coprime to 30
-      u_hits += readlane_u32(wave_incl_scan(h_win), 63);
+      const uint32_t h_scan = wave_incl_scan(h_win);
+      u_hits += readlane_u32(h_scan, 63);
       // primes [0, d_win) include every one with more than 32 hits; the
       // others take 2^lg lanes each in a grouped cross-off, 2^lg >= hits
       const unsigned long long big = __ballot(h_win > 32);
@@ -664,6 +705,83 @@ __device__ __forceinline__ void sieve_segments(
       //    and a number removed for the first time is charged to its least
       //    prime factor: p, or the cofactor q when 1 < q < p (q is then a
       //    prime of this part).
+      const uint32_t kk = lane & (G - 1);
+      uint32_t rec_n = 0;              // fresh removals recorded in blist (wave-uniform)
+      // Grouped cross-off steps of the primes [s0, r1) (r0 <= s0: a lane
+      // past r1 reads lane r0's data and does nothing).  ORD: the groups of a
+      // step issue their atomics one after the other; ATTR: a removal whose
+      // cofactor q is a prime below p is charged to q; REC: the fresh
+      // removals of the primes from rec_from on are appended to blist.
+      auto steps = [&](auto ord_c, auto attr_c, auto rec_c, uint32_t s0, uint32_t r0, uint32_t r1,
+                     uint32_t rec_from) -> unsigned long long {
+        constexpr bool ORD = decltype(ord_c)::value, ATTR = decltype(attr_c)::value;
+        constexpr bool REC = decltype(rec_c)::value;
+        unsigned long long fresh_any = 0;
+        for (uint32_t jb = s0; jb < r1; jb += ngrp) {
+          u_gsteps++;
+          const uint32_t jo = jb + grp;
+          // the cross-off data of prime jo, read from lane jo (all lanes active)
+          const int src = (int) ((jo < r1 ? jo : r0) << 2);
+          uint4 pp;
+          pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
+          pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
+          pp.z = ATTR ? (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z) : 0u;
+          pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
+          const bool has = jo < r1 && kk < pp.w;
+          // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
+          // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
+          // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
+          const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
+          const uint32_t idx = (pp.y >> 26) + kk;
+          const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
+          const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
+          const uint32_t tq = (t * 2185u) >> 16;
+          const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
+          const uint32_t bit = ((B & 3) << 3) | wheel_bit(t - 30 * tq);
+          const uint32_t mask = 1u << bit;
+          uint32_t old = 0;
+          if constexpr (ORD) {
+            // the groups of this step one after the other
+            const uint32_t ng = min(ngrp, r1 - jb);
+            for (uint32_t u = 0; u < ng; u++) {
+              if (has && grp == u)
+                old = atomicAnd(&bm[B >> 2], ~mask);
+              wave_lds_sync();
+            }
+          } else {
+            if (has)
+              old = atomicAnd(&bm[B >> 2], ~mask);
+          }
+          const bool fresh = has && (old & mask);
+          fresh_any |= __ballot(fresh);
+          if constexpr (REC) {
+            // bit index 8 B + wheel bit (below 2^16 for W <= 1024); the
+            // lanes of a step hold its primes in order, so the records come
+            // out sorted by prime
+            const bool rec = fresh && jo >= rec_from;
+            const unsigned long long rb = __ballot(rec);
+            if (rec) {
+              const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                  (uint32_t) (rb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) rb, 0u));
+              blist[rec_n + rank] = (uint16_t) (((B & ~3u) << 3) | bit);
+            }
+            rec_n += (uint32_t) __popcll(rb);
+          }
+          if (fresh) {
+            uint32_t attr = jo;
+            if constexpr (ATTR) {
+              if (delta < pp.z) {                     // q = q_lo + delta < p
+                const uint32_t q = 30 * a + rp - pp.z + delta;
+                if (q > 1)
+                  attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
+              }
+            }
+            atomicAdd(&w_rem[attr], 1u);
+          }
+        }
+        return fresh_any;
+      };
+
       auto flush = [&](uint32_t r0, uint32_t r1) {
         if (r0 >= r1)
           return;
@@ -721,70 +839,11 @@ __device__ __forceinline__ void sieve_segments(
         if (d_end >= r1)
           return;
         const bool ordered = d_end < seq_end;
-        // a removal charged to a cofactor q < p needs a prime with q_lo < p
-        const uint32_t np_part = r1 - d_end;
-        const unsigned long long part =
-            (np_part >= 64 ? ~0ull : ((1ull << np_part) - 1)) << d_end;
-        const bool attr_any = !ordered && (zmask & part) != 0;
-        unsigned long long fresh_any = 0;
-        const uint32_t kk = lane & (G - 1);
-        // one loop per kind of part, so that the per-step work has no
-        // uniform branches on the kind
-        auto steps = [&](auto ord_c, auto attr_c) {
-          constexpr bool ORD = decltype(ord_c)::value, ATTR = decltype(attr_c)::value;
-          for (uint32_t jb = d_end; jb < r1; jb += ngrp) {
-            u_gsteps++;
-            const uint32_t jo = jb + grp;
-            // the cross-off data of prime jo, read from lane jo (all lanes active)
-            const int src = (int) ((jo < r1 ? jo : r0) << 2);
-            uint4 pp;
-            pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
-            pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
-            pp.z = ATTR ? (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z) : 0u;
-            pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
-            const bool has = jo < r1 && kk < pp.w;
-            // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
-            // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
-            // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
-            const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
-            const uint32_t idx = (pp.y >> 26) + kk;
-            const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
-            const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
-            const uint32_t tq = (t * 2185u) >> 16;
-            const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
-            const uint32_t mask = 1u << (((B & 3) << 3) | wheel_bit(t - 30 * tq));
-            uint32_t old = 0;
-            if constexpr (ORD) {
-              // the groups of this step one after the other
-              const uint32_t ng = min(ngrp, r1 - jb);
-              for (uint32_t u = 0; u < ng; u++) {
-                if (has && grp == u)
-                  old = atomicAnd(&bm[B >> 2], ~mask);
-                wave_lds_sync();
-              }
-            } else {
-              if (has)
-                old = atomicAnd(&bm[B >> 2], ~mask);
-            }
-            const bool fresh = has && (old & mask);
-            fresh_any |= __ballot(fresh);
-            if (fresh) {
-              uint32_t attr = jo;
-              if constexpr (ATTR) {
-                if (delta < pp.z) {                     // q = q_lo + delta < p
-                  const uint32_t q = 30 * a + rp - pp.z + delta;
-                  if (q > 1)
-                    attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
-                }
-              }
-              atomicAdd(&w_rem[attr], 1u);
-            }
-          }
-        };
+        unsigned long long fresh_any;
         if (ordered)
-          steps(std::true_type{}, std::false_type{});
+          fresh_any = steps(std::true_type{}, std::false_type{}, std::false_type{}, d_end, r0, r1, 64u);
         else
-          steps(std::false_type{}, std::true_type{});
+          fresh_any = steps(std::false_type{}, std::true_type{}, std::false_type{}, d_end, r0, r1, 64u);
         dirty |= fresh_any != 0;
       };
 
@@ -793,8 +852,196 @@ __device__ __forceinline__ void sieve_segments(
         return m ? (uint32_t) __ffsll((long long) m) - 1 : 64u;
       };
 
+      // Batched leaf counting (single-query kernel, W <= 1024).  From
+      // bat_from = max(seq_end, d_win) on, the window's primes are crossed
+      // off in any order; if in addition no such prime's cofactor range
+      // [q_lo, q_hi] meets the window's primes, no number of the segment is a
+      // multiple of two of them that is still unsieved, so each first-time
+      // removal belongs to the one prime that hits it whatever the order.
+      // Then the leaves of consecutive primes j .. jb with leaves are counted
+      // against one prefix build, made after crossing off every prime before
+      // jb: a leaf of prime jj < jb adds the first-time removals of the
+      // primes jj .. jb - 1 that lie at or below its stop (they are unsieved
+      // in its own state, sieved in the prefix state), which the cross-off
+      // steps record in blist, sorted by prime.  A batch holds at most
+      // kBatchEntries factor entries (leaves) and its recorded primes at most
+      // bcap hits (records), which bounds blist and the correction work.
+      bool bat = false;
+      uint32_t bat_from = 64;
+      uint32_t s_ent = 0;                 // per lane: factor entries of the leaf primes <= it
+      if constexpr (BATCH) {
+        bat_from = max(seq_end, d_win);
+        const unsigned long long cand = bat_from < 64 ? lmask & (~0ull << bat_from) : 0ull;
+        if (bcap != 0 && __popcll(cand) >= 2) {
+          // no prime from bat_from on has a window prime among its cofactors
+          const uint32_t pf = readlane_u32(r_p, 0), pl = readlane_u32(r_p, nwin - 1);
+          bat = __ballot(lane >= bat_from && lane < nwin && cq_lo <= pl && cq_hi >= pf) == 0;
+          if (bat)
+            s_ent = wave_incl_scan(((lmask >> lane) & 1) ? r_hi[0] - r_lo[0] : 0u);
+        }
+      }
+      // the last prime with leaves that can join a batch starting at j: the
+      // batch's primes with leaves have at most kBatchEntries factor entries
+      // (or prime leaves), its recorded primes [j, jb) at most bcap hits
+      auto batch_end = [&](uint32_t j) -> uint32_t {
+        const uint32_t be = readlane_u32(s_ent, j) - (readlane_u32(r_hi[0], j) - readlane_u32(r_lo[0], j));
+        const uint32_t bh = readlane_u32(h_scan, j) - readlane_u32(h_win, j);
+        const bool ok = ((lmask >> lane) & 1) && lane >= j && s_ent - be <= kBatchEntries &&
+                        h_scan - h_win - bh <= bcap;
+        const unsigned long long bal = __ballot(ok);
+        return bal ? 63u - (uint32_t) __clzll((long long) bal) : j;
+      };
+      // Counts the leaves of the primes with leaves in [j, jb] (NQ == 1).
+      auto run_batch = [&](uint32_t j, uint32_t jb, uint32_t r0) {
+        if (r0 < bat_from) {
+          flush(r0, bat_from);
+          r0 = bat_from;
+        }
+        rec_n = 0;
+        if (r0 < jb)
+          dirty |= steps(std::false_type{}, std::false_type{}, std::true_type{}, r0, r0, jb, j) != 0;
+        wave_lds_sync();
+        // first record of each prime of [j, jb]; the records of primes >= jj
+        // are blist[foff(jj) .. R)
+        const uint32_t fr = lane >= j && lane < jb ? w_rem[lane] : 0u;
+        const uint32_t foff = wave_incl_scan(fr) - fr;
+        const uint32_t R = rec_n;
+        u_batch++;
+        u_brec += R;
+        if (dirty) {
+          build_pair_prefix(bm, pre2, W, lane);
+          wave_lds_sync();
+          dirty = false;
+          u_pbuilds++;
+        }
+        const uint32_t xlo_lo = (uint32_t) r_xlo[0], xlo_hi = (uint32_t) (r_xlo[0] >> 32);
+        uint32_t qh = 0, qt = 0;                   // queue head and tail (mod 128)
+        auto drain_b = [&](uint32_t n) {
+          wave_lds_sync();
+          uint32_t e = 0, jl = 0;
+          if (lane < n) {
+            e = lq[(qt + lane) & 127];
+            jl = lqb[(qt + lane) & 127];
+          }
+          // the leaf's prime: x / p and its first record, from its lane
+          const int src = (int) (jl << 2);
+          const uint32_t xl = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_lo);
+          const uint32_t xh = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_hi);
+          const uint32_t st = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) foff);
+          uint32_t bound = 0;
+          int cnt = 0;
+          if (lane < n) {
+            const uint32_t m = (e << 1) | 1u;
+            cnt = (int) seg_count_b(bm, pre2, lut, leaf_stop(((uint64_t) xh << 32) | xl, low, m),
+                                    bound);
+          }
+          // lane 0 holds the earliest prime of the step (the queue is in
+          // prime order), so records before its first one count for no lane
+          const uint32_t e0 = (uint32_t) __builtin_amdgcn_readfirstlane((int) st);
+          for (uint32_t cb = e0; cb < R; cb += kBlock) {
+            const uint32_t ix = cb + lane;
+            const uint32_t rv = ix < R ? (uint32_t) blist[ix] : 0u;
+            // wave-uniform trip count, kept in a scalar register
+            const uint32_t ne = (uint32_t) __builtin_amdgcn_readfirstlane(
+                (int) min((uint32_t) kBlock, R - cb));
+            for (uint32_t i = 0; i < ne; i++) {
+              const uint32_t v = readlane_u32(rv, i);
+              cnt += (cb + i >= st && v < bound) ? 1 : 0;
+            }
+          }
+          if (lane < n)
+            S[0] += (long long) ((e >> 31) ? cnt : -cnt);
+          wave_lds_sync();
+          qt += n;
+          u_qsteps++;
+        };
+        for (uint32_t jj = j; jj <= jb; jj = next_leaf(jj + 1)) {
+          u_lprimes++;
+          const uint32_t lo = readlane_u32(r_lo[0], jj);
+          const uint32_t hi = readlane_u32(r_hi[0], jj);
+          const uint32_t Tt = readlane_u32(r_T[0], jj);
+          const uint32_t T = Tt & 0x7fffffffu;
+          int M = 0;                                      // wave-uniform
+          if (Tt >> 31) {
+            u_entries += hi - lo;
+            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
+            uint32_t np_tot = 0, nl_tot = 0;
+            uint32_t rb = (lo + 1) % 480, mb = 2310 * ((lo + 1) / 480);
+            uint32_t f = lo + 1 + lane <= hi ? (uint32_t) factor[lo + 1 + lane] : 0u;
+            uint32_t cop = c_cop_ext[rb + lane];
+#pragma nounroll
+            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+              const uint32_t nidx = base + lane + kBlock;
+              const uint32_t f_next = nidx <= hi ? (uint32_t) factor[nidx] : 0u;
+              uint32_t rbn = rb + kBlock, mbn = mb;
+              if (rbn >= 480) {
+                rbn -= 480;
+                mbn += 2310;
+              }
+              const uint32_t cop_next = c_cop_ext[rbn + lane];
+              const bool leaf = f > T;
+              const unsigned long long lb = __ballot(leaf);
+              const uint32_t nl = (uint32_t) __popcll(lb);
+              np_tot += (uint32_t) __popcll(__ballot(leaf && (f & 1)));
+              if (leaf) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                    (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
+                const uint32_t m = mb + cop;
+                lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
+                lqb[(qh + rank) & 127] = (uint8_t) jj;
+              }
+              qh += nl;
+              nl_tot += nl;
+              if (qh - qt >= 64)
+                drain_b(64);
+              f = f_next;
+              cop = cop_next;
+              rb = rbn;
+              mb = mbn;
+            }
+            u_leaves += nl_tot;
+            M = (int) (2 * np_tot) - (int) nl_tot;
+          } else {
+            // primes q: leaves +phi(xp / q, b - 1), queued with a positive sign
+            u_leaves += hi - lo;
+            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
+            M = (int) (hi - lo);
+            uint32_t q = lo + 1 + lane <= hi ? primes[lo + 1 + lane] : 0u;
+#pragma nounroll
+            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
+              const uint32_t idx = base + lane;
+              const uint32_t q_next = idx + kBlock <= hi ? primes[idx + kBlock] : 0u;
+              const uint32_t nl = min((uint32_t) kBlock, hi - base + 1);
+              if (idx <= hi) {
+                lq[(qh + lane) & 127] = (q >> 1) | 0x80000000u;
+                lqb[(qh + lane) & 127] = (uint8_t) jj;
+              }
+              qh += nl;
+              if (qh - qt >= 64)
+                drain_b(64);
+              q = q_next;
+            }
+          }
+          w_m[0] = lane == jj ? M : w_m[0];
+        }
+        if (qh != qt)
+          drain_b(qh - qt);
+      };
+
+
       uint32_t run = 0;                 // first prime whose cross-off is pending
       for (uint32_t j = next_leaf(0); j < nwin; j = next_leaf(j + 1)) {
+        if constexpr (BATCH) {
+          if (bat && j >= bat_from) {
+            const uint32_t jb = batch_end(j);
+            if (jb > j) {
+              run_batch(j, jb, run);
+              run = jb;
+              j = jb;
+              continue;
+            }
+          }
+        }
         // members with leaves at j: factor entries that fit one step and
         // hold no leaf (every entry <= T, a quarter of the primes with
         // entries at 1e19) are passed over, and the cross-offs stay pending
@@ -977,7 +1224,7 @@ __device__ __forceinline__ void sieve_segments(
       }
     }
     if ((n_slots | u_hits | u_leaves | u_entries | u_win | u_lprimes | u_lsteps | u_dprimes |
-         u_dsteps | u_gsteps | u_pbuilds | u_qsteps) >= (1u << 30))
+         u_dsteps | u_gsteps | u_pbuilds | u_qsteps | u_batch | u_brec) >= (1u << 30))
       flush_counters();
   }
 
@@ -992,10 +1239,10 @@ __device__ __forceinline__ void sieve_segments(
     const uint64_t* __restrict__ pi_bits, const uint32_t* __restrict__ pi_cnt,               \
     uint32_t* __restrict__ T_S, EscTable esc, unsigned long long* __restrict__ S_acc,        \
     unsigned long long* __restrict__ counters, unsigned int* __restrict__ queue,             \
-    uint32_t nitems, uint32_t W, uint32_t cbits
+    uint32_t nitems, uint32_t W, uint32_t cbits, uint32_t bcap
 #define PCG_SIEVE_ARGS                                                                         \
     queries, items, groups, binfo_all, primes, factor, pi_bits, pi_cnt, T_S, esc, S_acc,     \
-    counters, queue, nitems, W, cbits
+    counters, queue, nitems, W, cbits, bcap
 
 template <typename FT, int NQ>
 __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(PCG_SIEVE_PARAMS)
@@ -1003,9 +1250,8 @@ __global__ __launch_bounds__(kBlock) void hsl_sieve_kernel(PCG_SIEVE_PARAMS)
   sieve_segments<FT, NQ>(PCG_SIEVE_ARGS);
 }
 
-/// The single-query kernel compiled for at least 6 waves per SIMD: the
-/// compiler keeps it within 80 VGPRs (the unconstrained build takes 81, which
-/// allows 5), so segments small enough for LDS to allow more than 20 waves
+/// The single-query kernel compiled for at least 6 waves per SIMD (at most
+/// 80 VGPRs), so segments small enough for LDS to allow more than 20 waves
 /// per CU get them.  A schedule choice (HslArgs::waves); results are the
 /// same.
 template <typename FT>
@@ -1021,16 +1267,16 @@ void hsl_sieve_kernel_w6(PCG_SIEVE_PARAMS)
 template <typename FT, int NQ>
 static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st)
 {
+#define PCG_SIEVE_LAUNCH(K)                                                                     \
+  hipLaunchKernelGGL(K, dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st, a.queries, a.items,  \
+                     a.groups, a.binfo, a.primes, reinterpret_cast<const FT*>(a.factor),        \
+                     a.pi_bits, a.pi_cnt, a.T_S, a.esc, a.S_acc, a.counters, a.queue, a.nitems, \
+                     a.W, a.cbits, a.bcap)
   if (NQ == 1 && a.waves == 6)
-    hipLaunchKernelGGL((hsl_sieve_kernel_w6<FT>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
-                       a.queries, a.items, a.groups, a.binfo, a.primes,
-                       reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
-                       a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
+    PCG_SIEVE_LAUNCH((hsl_sieve_kernel_w6<FT>));
   else
-    hipLaunchKernelGGL((hsl_sieve_kernel<FT, NQ>), dim3(grid), dim3(kBlock), hsl_lds_bytes(a.W), st,
-                       a.queries, a.items, a.groups, a.binfo, a.primes,
-                       reinterpret_cast<const FT*>(a.factor), a.pi_bits, a.pi_cnt, a.T_S, a.esc,
-                       a.S_acc, a.counters, a.queue, a.nitems, a.W, a.cbits);
+    PCG_SIEVE_LAUNCH((hsl_sieve_kernel<FT, NQ>));
+#undef PCG_SIEVE_LAUNCH
   return hipGetLastError();
 }
 
@@ -1349,8 +1595,10 @@ hipError_t hsl_sieve_occupancy(bool factor32, uint32_t W, uint32_t waves, int* b
                     : hipOccupancyMaxActiveBlocksPerMultiprocessor(
                           blocks_per_cu, hsl_sieve_kernel_w6<uint16_t>, kBlock, lds);
   if (factor32)
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint32_t, 1>, kBlock, lds);
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint16_t, 1>, kBlock, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint32_t, 1>,
+                                                        kBlock, lds);
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, hsl_sieve_kernel<uint16_t, 1>,
+                                                      kBlock, lds);
 }
 
 hipError_t hsl_launch_fill_zero(void* p, size_t bytes, hipStream_t st)

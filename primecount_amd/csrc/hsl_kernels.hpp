@@ -140,7 +140,15 @@ struct HslArgs {
   uint32_t group_max;           // 2 or 4: the largest group (when groups is set)
   uint32_t W;                   // words per segment (multiple of 128, 128..4096)
   uint32_t waves;               // 6: the single-query kernel built for 6 waves per SIMD, else 0
+  uint32_t bcap;                // batched leaf counting: most cross-off hits recorded per batch
+                                // (<= kBatchRecords), 0 = off; single-query kernel, W <= 1024
 };
+
+/// Batched leaf counting (single-query kernel): a batch holds at most
+/// kBatchEntries factor entries (or prime leaves) and records at most
+/// kBatchRecords first-time removals for the corrections.
+constexpr uint32_t kBatchEntries = 128;
+constexpr uint32_t kBatchRecords = 128;
 
 constexpr int kBlock = 64;     // one wave per segment
 

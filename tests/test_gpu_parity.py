@@ -261,3 +261,39 @@ def test_slot_escape_invariance(pc, bits):
     finally:
         pc.set_pass_slots(0)
         pc.set_slot_bits(0)
+
+
+@pytest.mark.parametrize("words", [256, 640, 1024, 2048])
+def test_leaf_batching_invariance(pc, oracle, words):
+    # The leaves of several primes counted on one prefix build (with the
+    # recorded first-time removals as corrections) give the same sums as the
+    # one-prime-at-a-time path; segments above 1024 words never batch.
+    rows = {int(v["x"]): v for v in golden("configs.json")["D"]}
+    s2 = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}
+    kat = [[int(v) for v in r] for r in golden("kat_D.json")["rows"]][::4]
+    fx = [r for r in golden("batch_D.json")["rows"] if int(r["x"]) <= 10 ** 15][:16]
+    rng = random.Random(4242 + words)
+    rnd = []
+    for _ in range(4):
+        x = rng.randrange(10 ** 10, 10 ** 11)
+        x13 = round(x ** (1 / 3))
+        y = rng.randint(x13 + 1, 3 * x13)
+        rnd.append((x, y, rng.randint(y, 2 * y), rng.choice([6, 7, 8])))
+    try:
+        pc.set_segment_words(words)
+        for on in (True, False):
+            pc.set_leaf_batching(on)
+            log("leaf batching %s, %d-word segments: D 1e15, 1e17, S2_hard 1e15" % (on, words))
+            for e in (15, 17):
+                assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+                st = pc.last_stats()
+                assert (st["leaf_batches"] > 0) == (on and words <= 1024)
+            assert pc.S2_hard_default(10 ** 15) == int(s2[10 ** 15]["value"])
+            for x, y, z, k, want in kat:
+                assert pc.D(x, y, z, k) == want, (x, y, z, k)
+            for x, y, z, k in rnd:
+                assert pc.D(x, y, z, k) == oracle.D(x, y, z, k), (x, y, z, k)
+            assert pc.D_batch([int(r["x"]) for r in fx]) == [int(r["D"]) for r in fx]
+    finally:
+        pc.set_leaf_batching(True)
+        pc.set_segment_words(0)
