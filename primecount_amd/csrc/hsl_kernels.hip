@@ -707,12 +707,11 @@ __device__ __forceinline__ void sieve_segments(
       //    prime of this part).
       const uint32_t kk = lane & (G - 1);
       uint32_t rec_n = 0;              // fresh removals recorded in blist (wave-uniform)
-      // Grouped cross-off steps of the primes [s0, r1) (r0 <= s0: a lane
-      // past r1 reads lane r0's data and does nothing).  ORD: the groups of a
+      // Grouped cross-off steps of the primes [s0, r1).  ORD: the groups of a
       // step issue their atomics one after the other; ATTR: a removal whose
       // cofactor q is a prime below p is charged to q; REC: the fresh
       // removals of the primes from rec_from on are appended to blist.
-      auto steps = [&](auto ord_c, auto attr_c, auto rec_c, uint32_t s0, uint32_t r0, uint32_t r1,
+      auto steps = [&](auto ord_c, auto attr_c, auto rec_c, uint32_t s0, uint32_t r1,
                      uint32_t rec_from) -> unsigned long long {
         constexpr bool ORD = decltype(ord_c)::value, ATTR = decltype(attr_c)::value;
         constexpr bool REC = decltype(rec_c)::value;
@@ -720,8 +719,10 @@ __device__ __forceinline__ void sieve_segments(
         for (uint32_t jb = s0; jb < r1; jb += ngrp) {
           u_gsteps++;
           const uint32_t jo = jb + grp;
-          // the cross-off data of prime jo, read from lane jo (all lanes active)
-          const int src = (int) ((jo < r1 ? jo : r0) << 2);
+          // the cross-off data of prime jo, read from lane jo (all lanes
+          // active; ds_bpermute takes the lane modulo 64, and a lane past r1
+          // has no hit, so what it reads does not matter)
+          const int src = (int) (jo << 2);
           uint4 pp;
           pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
           pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
@@ -735,7 +736,7 @@ __device__ __forceinline__ void sieve_segments(
           const uint32_t idx = (pp.y >> 26) + kk;
           const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
           const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
-          const uint32_t tq = (t * 2185u) >> 16;
+          const uint32_t tq = __umul24(t, 2185u) >> 16;
           const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
           const uint32_t bit = ((B & 3) << 3) | wheel_bit(t - 30 * tq);
           const uint32_t mask = 1u << bit;
@@ -752,7 +753,8 @@ __device__ __forceinline__ void sieve_segments(
             if (has)
               old = atomicAnd(&bm[B >> 2], ~mask);
           }
-          const bool fresh = has && (old & mask);
+          // old is 0 on a lane without a hit
+          const bool fresh = (old & mask) != 0;
           fresh_any |= __ballot(fresh);
           if constexpr (REC) {
             // bit index 8 B + wheel bit (below 2^16 for W <= 1024); the
@@ -841,9 +843,9 @@ __device__ __forceinline__ void sieve_segments(
         const bool ordered = d_end < seq_end;
         unsigned long long fresh_any;
         if (ordered)
-          fresh_any = steps(std::true_type{}, std::false_type{}, std::false_type{}, d_end, r0, r1, 64u);
+          fresh_any = steps(std::true_type{}, std::false_type{}, std::false_type{}, d_end, r1, 64u);
         else
-          fresh_any = steps(std::false_type{}, std::true_type{}, std::false_type{}, d_end, r0, r1, 64u);
+          fresh_any = steps(std::false_type{}, std::true_type{}, std::false_type{}, d_end, r1, 64u);
         dirty |= fresh_any != 0;
       };
 
@@ -899,7 +901,7 @@ __device__ __forceinline__ void sieve_segments(
         }
         rec_n = 0;
         if (r0 < jb)
-          dirty |= steps(std::false_type{}, std::false_type{}, std::true_type{}, r0, r0, jb, j) != 0;
+          dirty |= steps(std::false_type{}, std::false_type{}, std::true_type{}, r0, jb, j) != 0;
         wave_lds_sync();
         // first record of each prime of [j, jb]; the records of primes >= jj
         // are blist[foff(jj) .. R)
