@@ -272,6 +272,32 @@ def valu_from_profile(x):
     return None
 
 
+def pi_small(pc, n):
+    """Wall time of pcg_pi for small x against the device time of its terms
+    (stream events), n calls per size after one warm-up call.  The gap is
+    host work plus launch overhead; a kernel trace of the same run counts the
+    dispatches per call."""
+    out = {}
+    for e in (9, 10, 11, 12, 13):
+        x = 10 ** e
+        pc.pi(x)
+        time.sleep(0.05)
+        walls, dev = [], []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            pc.pi(x)
+            walls.append(time.perf_counter() - t0)
+            st = pc.last_pi_stats()
+            dev.append(st["seconds_sigma"] + st["seconds_phi0"] + st["seconds_ac"] +
+                       st["seconds_b"] + st["seconds_d"])
+        walls.sort()
+        dev.sort()
+        out["1e%d" % e] = {"calls": n, "wall_ms_median": round(1e3 * walls[n // 2], 4),
+                           "terms_device_ms_median": round(1e3 * dev[n // 2], 4)}
+        time.sleep(0.05)
+    return out
+
+
 def device_copy(nbytes, reps=2):
     """A plain device-to-device copy of `nbytes` (HIP's own copy kernel), used
     to calibrate the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -342,6 +368,10 @@ def main():
                     help="use the nominal LDS peak instead of running the LDS probe kernel")
     ap.add_argument("--no-configs", action="store_true",
                     help="skip the 256-query batch and the 1e21 (128-bit) measurements")
+    ap.add_argument("--pi-small", type=int, default=0,
+                    help="only time N calls each of pi(10^9) .. pi(10^13) (launch-overhead "
+                         "measurement; 50 ms pauses separate the sizes in a kernel trace) and "
+                         "print their JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -353,6 +383,9 @@ def main():
         sys.exit(2)
 
     import primecount_amd as pc
+    if a.pi_small:
+        print(json.dumps({"pi_small": pi_small(pc, a.pi_small)}), flush=True)
+        return
     if a.segment_words:
         pc.set_segment_words(a.segment_words)
     x = int(round(a.x))
