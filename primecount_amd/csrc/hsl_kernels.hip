@@ -637,12 +637,11 @@ __device__ __forceinline__ void sieve_segments(
           const uint32_t v = qm + r_nq - 1;
           const uint32_t vb = v / 30;
           r_h = 8 * vb + le_count(v - 30 * vb) - c0;
-          // off0 = 30 b0 + cc0: the multiple p (q_lo + delta) lies in byte
-          // b0 + a delta + (cc0 + rp delta) / 30 of the segment (p = 30 a + rp)
-          const uint32_t b0 = off0 / 30;
-          prm.y = b0 | ((off0 - 30 * b0) << 16) | (qm << 21) | (c0 << 26);
+          // the multiple p (q_lo + delta) lies at offset off0 + p delta of
+          // the segment: byte (off0 + p delta) / 30, its wheel bit from the
+          // remainder (off0 < 2^20, c0 < 8, qm < 30)
+          prm.y = off0 | (c0 << 20) | (qm << 23);
         }
-        prm.x = (p / 30) | ((p % 30) << 21);            // p < 2^25: a < 2^21
         prm.z = q_lo < p ? (uint32_t) (p - q_lo) : 0u;
       }
       // per member: its window end and its primes with factor entries
@@ -724,21 +723,21 @@ __device__ __forceinline__ void sieve_segments(
           // has no hit, so what it reads does not matter)
           const int src = (int) (jo << 2);
           uint4 pp;
-          pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.x);
+          pp.x = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_p);
           pp.y = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.y);
           pp.z = ATTR ? (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.z) : 0u;
           pp.w = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) prm.w);
           const bool has = jo < r1 && kk < pp.w;
-          // cofactor q_lo + delta: the k-th coprime to 30 from q_lo; for a
-          // lane with a hit (kk < hits <= 32) delta < 150, so t < 4400 and
-          // t / 30 = (t 2185) >> 16 (other lanes do not touch the segment)
-          const uint32_t a = pp.x & 0x1FFFFFu, rp = pp.x >> 21;
-          const uint32_t idx = (pp.y >> 26) + kk;
-          const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - ((pp.y >> 21) & 31u);
-          const uint32_t t = ((pp.y >> 16) & 31u) + __umul24(rp, delta);
-          const uint32_t tq = __umul24(t, 2185u) >> 16;
-          const uint32_t B = (pp.y & 0xFFFFu) + __umul24(a, delta) + tq;
-          const uint32_t bit = ((B & 3) << 3) | wheel_bit(t - 30 * tq);
+          // cofactor q_lo + delta: the k-th coprime to 30 from q_lo (delta <
+          // 180 for a lane with a hit, kk < hits <= 32); its multiple lies at
+          // offset off < 2^20 of the segment, byte off / 30 = mulhi(off,
+          // ceil(2^32 / 30)) (exact below 2^28); other lanes do not touch the
+          // segment
+          const uint32_t idx = ((pp.y >> 20) & 7u) + kk;
+          const uint32_t delta = 30 * (idx >> 3) + wheel_residue(idx & 7) - (pp.y >> 23);
+          const uint32_t off = (pp.y & 0xFFFFFu) + __umul24(pp.x, delta);
+          const uint32_t B = __umulhi(off, 143165577u);
+          const uint32_t bit = ((B & 3) << 3) | wheel_bit(off - 30 * B);
           const uint32_t mask = 1u << bit;
           uint32_t old = 0;
           if constexpr (ORD) {
@@ -773,7 +772,7 @@ __device__ __forceinline__ void sieve_segments(
             uint32_t attr = jo;
             if constexpr (ATTR) {
               if (delta < pp.z) {                     // q = q_lo + delta < p
-                const uint32_t q = 30 * a + rp - pp.z + delta;
+                const uint32_t q = pp.x - pp.z + delta;
                 if (q > 1)
                   attr = (uint32_t) pi_lookup32(pi_bits, pi_cnt, q) - b;
               }
@@ -794,23 +793,22 @@ __device__ __forceinline__ void sieve_segments(
           u_dprimes++;
           uint32_t removed = 0;                         // wave-uniform
           if (nq > 0 && p >= 7) {
-            const uint32_t px = readlane_u32(prm.x, j);
             const uint32_t py = readlane_u32(prm.y, j);
-            const uint32_t a = px & 0x1FFFFFu, rp = px >> 21;
-            const uint32_t qm = (py >> 21) & 31u;
+            const uint32_t qm = py >> 23;
             const uint32_t cls = lane & 7;
             const uint32_t r = wheel_residue(cls);
             const uint32_t delta = r >= qm ? r - qm : r + 30 - qm;   // < 30
-            // byte and bit of the class's first multiple p (q_lo + delta):
-            // t = cc0 + rp delta < 871, so t / 30 = (t 2185) >> 16
-            const uint32_t t = ((py >> 16) & 31u) + __umul24(rp, delta);
-            const uint32_t tq = (t * 2185u) >> 16;
+            // byte and bit of the class's first multiple p (q_lo + delta), at
+            // offset off < 2^20 + 29 p < 2^28 (these primes have more than 32
+            // hits in the segment, so p < 2^16)
+            const uint32_t off = (py & 0xFFFFFu) + __umul24(p, delta);
+            const uint32_t B = __umulhi(off, 143165577u);
             const uint32_t j0 = lane >> 3;
             // lane j0 takes multiples j0, j0 + 8, ... of its class: p bytes
             // apart per multiple, so 8 p bytes (2 p words) per step, and the
             // bit within the word does not change
-            const uint32_t B0 = (py & 0xFFFFu) + __umul24(a, delta) + tq + j0 * p;
-            const uint32_t mask = 1u << (((B0 & 3) << 3) | wheel_bit(t - 30 * tq));
+            const uint32_t B0 = B + j0 * p;
+            const uint32_t mask = 1u << (((B0 & 3) << 3) | wheel_bit(off - 30 * B));
             const uint32_t nmax = (nq + 29) / 30;       // >= every class count
             const uint32_t nsteps = (nmax + 7) / 8;
             u_dsteps += nsteps;
