@@ -298,6 +298,29 @@ def pi_small(pc, n):
     return out
 
 
+def phase_split(pc, x):
+    """Share of the sieve kernel's wave time per phase at D(x) (default
+    parameters), from the phase-timer build (clock read at every phase
+    change, cycles summed over the waves), beside the kernel time of the
+    plain build."""
+    pc.D_default(x)
+    pc.D_default(x)
+    plain = pc.last_stats()
+    pc.set_phase_timers(True)
+    try:
+        pc.D_default(x)
+        pc.D_default(x)
+        st = pc.last_stats()
+    finally:
+        pc.set_phase_timers(False)
+    cyc = st["phase_cycles"]
+    tot = sum(cyc.values()) or 1
+    return {"x": str(x), "kernel_ms_plain": round(1e3 * plain["seconds_kernel"], 3),
+            "kernel_ms_timed": round(1e3 * st["seconds_kernel"], 3),
+            "share": {k: round(v / tot, 4) for k, v in cyc.items()},
+            "wave_cycles": tot}
+
+
 def device_copy(nbytes, reps=2):
     """A plain device-to-device copy of `nbytes` (HIP's own copy kernel), used
     to calibrate the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -372,6 +395,9 @@ def main():
                     help="only time N calls each of pi(10^9) .. pi(10^13) (launch-overhead "
                          "measurement; 50 ms pauses separate the sizes in a kernel trace) and "
                          "print their JSON line")
+    ap.add_argument("--phase-split", action="store_true",
+                    help="only measure the sieve kernel's time per phase at D(x) with the "
+                         "phase-timer build and print its JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -385,6 +411,9 @@ def main():
     import primecount_amd as pc
     if a.pi_small:
         print(json.dumps({"pi_small": pi_small(pc, a.pi_small)}), flush=True)
+        return
+    if a.phase_split:
+        print(json.dumps({"phase_split": phase_split(pc, int(round(a.x)))}), flush=True)
         return
     if a.segment_words:
         pc.set_segment_words(a.segment_words)

@@ -199,6 +199,12 @@ int pcg_set_slot_bits(uint32_t cbits);
  * src/gourdon/D.cpp:122-273). */
 int pcg_set_leaf_batching(int on);
 
+/* Phase timers (default off): single-query launches run a build of the sieve
+ * kernel that reads the clock at every phase change and reports the cycles
+ * per phase in pcg_stats.phase_cycles.  Same results; the timers slow the
+ * kernel down.  A profiling aid with no counterpart in the reference. */
+int pcg_set_phase_timers(int on);
+
 /* Host threads used by the table builders (0 = all hardware threads). */
 int pcg_set_host_threads(int threads);
 
@@ -244,6 +250,14 @@ typedef struct pcg_stats {
                                 one prefix build (pcg_set_leaf_batching)      */
   uint64_t batch_records;    /* first-time removals recorded for the batches'
                                 corrections                                   */
+  uint64_t phase_cycles[11]; /* with pcg_set_phase_timers(1): the sieve kernel's
+                                clock cycles per phase, summed over its waves:
+                                segment set-up and end, window set-up and end,
+                                one-at-a-time cross-offs, grouped cross-offs,
+                                batches' recording cross-offs, pair-prefix
+                                builds, pre-scan of primes with factor entries,
+                                factor-entry steps, counting steps, batches'
+                                entry steps, batches' counting steps (else 0) */
 } pcg_stats;
 
 /* Statistics of the last D/S2_hard call on this thread. */

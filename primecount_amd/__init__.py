@@ -82,10 +82,17 @@ class Stats(ctypes.Structure):
         ("kernel_waves", ctypes.c_uint64),
         ("leaf_batches", ctypes.c_uint64),
         ("batch_records", ctypes.c_uint64),
+        ("phase_cycles", ctypes.c_uint64 * 11),
     ]
 
+    PHASES = ("segment", "window", "dense_crossoff", "group_crossoff", "batch_crossoff",
+              "prefix_build", "prescan", "entry_steps", "counting_steps", "batch_entry_steps",
+              "batch_counting_steps")
+
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_}
+        d["phase_cycles"] = dict(zip(self.PHASES, list(self.phase_cycles)))
+        return d
 
 
 class _Query(ctypes.Structure):
@@ -157,6 +164,7 @@ def load():
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_leaf_batching": ([ctypes.c_int], ctypes.c_int),
+        "pcg_set_phase_timers": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_slot_bits": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_set_autotune": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_schedule": ([ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int32],
@@ -416,6 +424,12 @@ def set_leaf_batching(on):
     """Count the leaves of several primes on one prefix build (default on);
     results are the same either way."""
     _check(load().pcg_set_leaf_batching(1 if on else 0))
+
+
+def set_phase_timers(on):
+    """Single-query launches report the sieve kernel's cycles per phase in
+    last_stats()["phase_cycles"] (a slower kernel build; same results)."""
+    _check(load().pcg_set_phase_timers(1 if on else 0))
 
 
 def set_slot_bits(bits):

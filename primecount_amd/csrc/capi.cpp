@@ -517,6 +517,11 @@ int pcg_set_leaf_batching(int on)
   return guarded([&] { engine_config().leaf_batching = on != 0; });
 }
 
+int pcg_set_phase_timers(int on)
+{
+  return guarded([&] { engine_config().phase_timers = on != 0; });
+}
+
 int pcg_set_pass_slots(uint64_t slots)
 {
   return guarded([&] {

@@ -715,12 +715,12 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   Wk.carry.ensure((size_t) ncols_total * 8);
   Wk.out.ensure((size_t) ncols_total * 16);
   Wk.sacc.ensure(nq * 16);
-  Wk.counters.ensure(128);
+  Wk.counters.ensure(kHslCounters * 8);
   hipStream_t s = D.stream;
   PCG_HIP(hsl_launch_fill_zero(Wk.carry.p, (size_t) ncols_total * 8, s));
   PCG_HIP(hsl_launch_fill_zero(Wk.out.p, (size_t) ncols_total * 16, s));
   PCG_HIP(hsl_launch_fill_zero(Wk.sacc.p, nq * 16, s));
-  PCG_HIP(hsl_launch_fill_zero(Wk.counters.p, 128, s));
+  PCG_HIP(hsl_launch_fill_zero(Wk.counters.p, kHslCounters * 8, s));
 
   uint64_t max_slots = slots_override ? slots_override : engine_config().max_slots_per_pass;
   if (max_slots == 0) max_slots = kMaxSlotsPerPass;
@@ -973,6 +973,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     a.W = W;
     a.waves = share ? 0u : waves;
     a.bcap = engine_config().leaf_batching ? kBatchRecords : 0u;
+    a.timers = engine_config().phase_timers ? 1u : 0u;
     uint32_t grid = (uint32_t) std::min<uint64_t>(slots_wg, a.nitems);
 
     PCG_HIP(hipEventRecord(Wk.event(3 * pi), s));
@@ -990,7 +991,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
 
   std::vector<i128> col_out(ncols_total);
   std::vector<uint64_t> sacc(2 * nq);
-  uint64_t counters[14] = {};
+  uint64_t counters[kHslCounters] = {};
   PCG_HIP(hipMemcpyAsync(col_out.data(), Wk.out.p, (size_t) ncols_total * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(sacc.data(), Wk.sacc.p, nq * 16, hipMemcpyDeviceToHost, s));
   PCG_HIP(hipMemcpyAsync(counters, Wk.counters.p, sizeof(counters), hipMemcpyDeviceToHost, s));
@@ -1049,6 +1050,8 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
     st->queue_steps = counters[11];
     st->leaf_batches = counters[12];
     st->batch_records = counters[13];
+    for (uint32_t i = 0; i < kHslPhases; i++)
+      st->phase_cycles[i] = counters[kHslPhaseCounter0 + i];
     st->device_allocations = g_allocations.load();
     st->escaped_slots = n_escaped;
     st->escape_bound = esc_bound_total;
@@ -1421,6 +1424,8 @@ std::vector<i128> hsl_compute_batch_locked(const std::vector<HslQuery>& qs, pcg_
     acc.queue_steps += st.queue_steps;
     acc.leaf_batches += st.leaf_batches;
     acc.batch_records += st.batch_records;
+    for (uint32_t i = 0; i < kHslPhases; i++)
+      acc.phase_cycles[i] += st.phase_cycles[i];
     acc.escaped_slots += st.escaped_slots;
     acc.escape_bound += st.escape_bound;
     acc.schedule_source = std::max(acc.schedule_source, st.schedule_source);

@@ -29,6 +29,7 @@ struct EngineConfig {
   uint64_t max_slots_per_pass = 0;   // 0 = default (2^31 slots, 16 GiB)
   bool share_segments = true;   // queries of a launch share the sieve of a segment
   bool leaf_batching = true;    // count the leaves of several primes on one prefix build
+  bool phase_timers = false;    // single-query launches run the kernel build with phase timers
   uint32_t slot_cbits = 0;      // 0 = automatic; else bits of C in a slot word (at least
                                 // what the segment size needs): fewer bits for M, more escapes
   int tune_mode = 1;            // TuneMode (tuner.hpp): 0 rule, 1 cached winners, 2 tune new classes

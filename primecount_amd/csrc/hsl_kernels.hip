@@ -421,7 +421,7 @@ __device__ __forceinline__ uint32_t pack_slot(uint32_t C, int32_t M, uint32_t cb
 /// Read-only tables and written arrays come in as separate __restrict__
 /// pointers so the compiler knows the tables are never written by the kernel
 /// and can read uniform entries (BInfo, work items) through the scalar cache.
-template <typename FT, int NQ>
+template <typename FT, int NQ, bool TIMERS = false>
 __device__ __forceinline__ void sieve_segments(
     const QueryDesc* __restrict__ queries, const SegItem* __restrict__ items,
     const SegGroup* __restrict__ groups, const BInfo* __restrict__ binfo_all,
@@ -484,7 +484,34 @@ __device__ __forceinline__ void sieve_segments(
     u_batch = u_brec = 0;
   };
 
+  // Phase timers (the TIMERS build only): the wave's clock cycles per phase
+  // (kHslPhase*), summed over the waves into counters[16 + phase].  A phase
+  // switch reads s_memtime and adds the cycles since the last switch to the
+  // phase that ends.
+  uint32_t ph_acc[kHslPhases];
+  uint64_t ph_t = 0;
+  uint32_t ph_cur = kHslPhaseSegment;
+  if constexpr (TIMERS) {
+    for (uint32_t i = 0; i < kHslPhases; i++)
+      ph_acc[i] = 0;
+    ph_t = __builtin_amdgcn_s_memtime();
+  }
+  auto phase = [&](uint32_t next) {
+    if constexpr (TIMERS) {
+      const uint64_t t = __builtin_amdgcn_s_memtime();
+      const uint32_t d = (uint32_t) (t - ph_t);
+      // a switch on a uniform value keeps ph_acc in scalar registers
+#pragma unroll
+      for (uint32_t i = 0; i < kHslPhases; i++)
+        if (ph_cur == i)
+          ph_acc[i] += d;
+      ph_t = t;
+      ph_cur = next;
+    }
+  };
+
   for (;;) {
+    phase(kHslPhaseSegment);
     uint32_t qi = 0;
     if (lane == 0)
       qi = atomicAdd(queue, 1u);
@@ -561,6 +588,7 @@ __device__ __forceinline__ void sieve_segments(
     // first-time removal happened since they were built (wave-uniform)
     bool dirty = true;
     while (!done) {
+      phase(kHslPhaseWindow);
       // Window of up to 64 primes b .. b + 63, lane l <-> b + l, kept in
       // registers; an epoch reads its prime's values with v_readlane.  A
       // member's window ends before its first b whose leaf range is provably
@@ -786,6 +814,8 @@ __device__ __forceinline__ void sieve_segments(
       auto flush = [&](uint32_t r0, uint32_t r1) {
         if (r0 >= r1)
           return;
+        const uint32_t ph_back = ph_cur;
+        phase(kHslPhaseDense);
         const uint32_t d_end = min(max(d_win, r0), r1);
         for (uint32_t j = r0; j < d_end; j++) {
           const uint32_t p = readlane_u32(r_p, j);
@@ -836,8 +866,11 @@ __device__ __forceinline__ void sieve_segments(
           if (lane == 0)
             w_rem[j] = removed;
         }
-        if (d_end >= r1)
+        if (d_end >= r1) {
+          phase(ph_back);
           return;
+        }
+        phase(kHslPhaseGroup);
         const bool ordered = d_end < seq_end;
         unsigned long long fresh_any;
         if (ordered)
@@ -845,6 +878,7 @@ __device__ __forceinline__ void sieve_segments(
         else
           fresh_any = steps(std::false_type{}, std::true_type{}, std::false_type{}, d_end, r1, 64u);
         dirty |= fresh_any != 0;
+        phase(ph_back);
       };
 
       auto next_leaf = [&](uint32_t c) -> uint32_t {
@@ -898,6 +932,7 @@ __device__ __forceinline__ void sieve_segments(
           r0 = bat_from;
         }
         rec_n = 0;
+        phase(kHslPhaseBatchFlush);
         if (r0 < jb)
           dirty |= steps(std::false_type{}, std::false_type{}, std::true_type{}, r0, jb, j) != 0;
         wave_lds_sync();
@@ -909,14 +944,17 @@ __device__ __forceinline__ void sieve_segments(
         u_batch++;
         u_brec += R;
         if (dirty) {
+          phase(kHslPhasePrefix);
           build_pair_prefix(bm, pre2, W, lane);
           wave_lds_sync();
           dirty = false;
           u_pbuilds++;
         }
+        phase(kHslPhaseBatchEntry);
         const uint32_t xlo_lo = (uint32_t) r_xlo[0], xlo_hi = (uint32_t) (r_xlo[0] >> 32);
         uint32_t qh = 0, qt = 0;                   // queue head and tail (mod 128)
         auto drain_b = [&](uint32_t n) {
+          phase(kHslPhaseBatchDrain);
           wave_lds_sync();
           uint32_t e = 0, jl = 0;
           if (lane < n) {
@@ -954,6 +992,7 @@ __device__ __forceinline__ void sieve_segments(
           wave_lds_sync();
           qt += n;
           u_qsteps++;
+          phase(kHslPhaseBatchEntry);
         };
         for (uint32_t jj = j; jj <= jb; jj = next_leaf(jj + 1)) {
           u_lprimes++;
@@ -1045,6 +1084,7 @@ __device__ __forceinline__ void sieve_segments(
         // members with leaves at j: factor entries that fit one step and
         // hold no leaf (every entry <= T, a quarter of the primes with
         // entries at 1e19) are passed over, and the cross-offs stay pending
+        phase(kHslPhaseScan);
         uint32_t active = 0;
         // the first step of each member's leaf range: its factor entries (or
         // primes), loaded here so that the loads are in flight during the
@@ -1076,11 +1116,13 @@ __device__ __forceinline__ void sieve_segments(
         run = j;
         // word-pair prefixes of the current sieve state for the count queries
         if (dirty) {
+          phase(kHslPhasePrefix);
           build_pair_prefix(bm, pre2, W, lane);
           wave_lds_sync();
           dirty = false;
           u_pbuilds++;
         }
+        phase(kHslPhaseEntry);
 #pragma unroll
         for (int k = 0; k < NQ; k++) {
           if (!((active >> k) & 1))
@@ -1104,6 +1146,7 @@ __device__ __forceinline__ void sieve_segments(
             // head and tail of the queue (mod 128), wave-uniform.
             uint32_t qh = 0, qt = 0;
             auto drain = [&](uint32_t n) {
+              phase(kHslPhaseDrain);
               wave_lds_sync();
               if (lane < n) {
                 const uint32_t e = lq[(qt + lane) & 127];
@@ -1114,6 +1157,7 @@ __device__ __forceinline__ void sieve_segments(
               wave_lds_sync();
               qt += n;
               u_qsteps++;
+              phase(kHslPhaseEntry);
             };
             // The factor-table index of the wave's step, idx = base + lane,
             // is walked as a scalar period start mb = 2310 (base / 480) and
@@ -1178,6 +1222,7 @@ __device__ __forceinline__ void sieve_segments(
         }
       }
       flush(run, nwin);
+      phase(kHslPhaseWindow);
       wave_lds_sync();
       {
         // C(s, b + l) = count at the window start minus the removals of the
@@ -1204,6 +1249,7 @@ __device__ __forceinline__ void sieve_segments(
       u_win++;
       b += nwin;
     }
+    phase(kHslPhaseSegment);
 #pragma unroll
     for (int k = 0; k < NQ; k++) {
       if (!live[k])
@@ -1230,6 +1276,12 @@ __device__ __forceinline__ void sieve_segments(
 
   // the u_* counters are updated in wave-uniform control flow only
   flush_counters();
+  if constexpr (TIMERS) {
+    phase(kHslPhaseSegment);
+    if (lane == 0)
+      for (uint32_t i = 0; i < kHslPhases; i++)
+        atomicAdd(&counters[kHslPhaseCounter0 + i], (unsigned long long) ph_acc[i]);
+  }
 }
 
 #define PCG_SIEVE_PARAMS                                                                       \
@@ -1261,6 +1313,14 @@ void hsl_sieve_kernel_w6(PCG_SIEVE_PARAMS)
   sieve_segments<FT, 1>(PCG_SIEVE_ARGS);
 }
 
+/// The single-query kernel with phase timers (HslArgs::timers): the same
+/// sums, plus each wave's clock cycles per phase in counters[16 ..].
+template <typename FT>
+__global__ __launch_bounds__(kBlock) void hsl_sieve_kernel_timed(PCG_SIEVE_PARAMS)
+{
+  sieve_segments<FT, 1, true>(PCG_SIEVE_ARGS);
+}
+
 #undef PCG_SIEVE_PARAMS
 #undef PCG_SIEVE_ARGS
 
@@ -1272,7 +1332,9 @@ static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st
                      a.groups, a.binfo, a.primes, reinterpret_cast<const FT*>(a.factor),        \
                      a.pi_bits, a.pi_cnt, a.T_S, a.esc, a.S_acc, a.counters, a.queue, a.nitems, \
                      a.W, a.cbits, a.bcap)
-  if (NQ == 1 && a.waves == 6)
+  if (NQ == 1 && a.timers)
+    PCG_SIEVE_LAUNCH((hsl_sieve_kernel_timed<FT>));
+  else if (NQ == 1 && a.waves == 6)
     PCG_SIEVE_LAUNCH((hsl_sieve_kernel_w6<FT>));
   else
     PCG_SIEVE_LAUNCH((hsl_sieve_kernel<FT, NQ>));

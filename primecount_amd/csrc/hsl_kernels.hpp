@@ -142,7 +142,23 @@ struct HslArgs {
   uint32_t waves;               // 6: the single-query kernel built for 6 waves per SIMD, else 0
   uint32_t bcap;                // batched leaf counting: most cross-off hits recorded per batch
                                 // (<= kBatchRecords), 0 = off; single-query kernel, W <= 1024
+  uint32_t timers;              // 1: the single-query kernel with phase timers
 };
+
+/// Phases of the sieve kernel's phase timers (the timed build): segment set-up
+/// and end, window set-up and end, one-at-a-time cross-offs, grouped
+/// cross-offs, a batch's recording cross-offs, pair-prefix builds, the
+/// pre-scan of primes with factor entries, factor-entry (or prime) steps,
+/// counting steps, and the entry and counting steps of leaf batches.
+enum : uint32_t {
+  kHslPhaseSegment, kHslPhaseWindow, kHslPhaseDense, kHslPhaseGroup, kHslPhaseBatchFlush,
+  kHslPhasePrefix, kHslPhaseScan, kHslPhaseEntry, kHslPhaseDrain, kHslPhaseBatchEntry,
+  kHslPhaseBatchDrain, kHslPhases
+};
+/// counters[kHslPhaseCounter0 + phase]: the phase's cycles summed over waves
+constexpr uint32_t kHslPhaseCounter0 = 16;
+/// 64-bit kernel counters: 14 work counts, 2 spare, then the phase cycles
+constexpr uint32_t kHslCounters = kHslPhaseCounter0 + kHslPhases;
 
 /// Batched leaf counting (single-query kernel): a batch holds at most
 /// kBatchEntries factor entries (or prime leaves) and records at most

@@ -297,3 +297,24 @@ def test_leaf_batching_invariance(pc, oracle, words):
     finally:
         pc.set_leaf_batching(True)
         pc.set_segment_words(0)
+
+
+def test_phase_timer_build(pc):
+    # the kernel build with phase timers gives the same sums and reports the
+    # cycles of every phase that ran
+    rows = {int(v["x"]): v for v in golden("configs.json")["D"]}
+    try:
+        pc.set_phase_timers(True)
+        for e in (15, 17):
+            log("phase timers: D default 1e%d" % e)
+            assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+            cyc = pc.last_stats()["phase_cycles"]
+            assert sum(cyc.values()) > 0
+            for k in ("segment", "window", "dense_crossoff", "group_crossoff", "prefix_build",
+                      "entry_steps", "counting_steps"):
+                assert cyc[k] > 0, k
+        pc.set_phase_timers(False)
+        assert pc.D_default(10 ** 15) == int(rows[10 ** 15]["value"])
+        assert sum(pc.last_stats()["phase_cycles"].values()) == 0
+    finally:
+        pc.set_phase_timers(False)
