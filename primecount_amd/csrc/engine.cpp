@@ -479,11 +479,20 @@ uint64_t escape_bound(const QueryPlan& qp, uint32_t s0, uint32_t s1)
 /// larger segments win, at the same total: the 256 reference x values
 /// (1e12 .. 1e18) in one launch took 773, 711, 794 ms at W = 512, 640, 768.
 /// So the rule takes the sum of the sieve limits of the launch.
-uint32_t auto_segment_words(uint64_t limit, LeafSum kind)
+uint32_t auto_segment_words(uint64_t limit, LeafSum kind, bool single)
 {
   uint32_t cfg = engine_config().segment_words;
   if (cfg) return cfg;
   const bool d = kind == LeafSum::D;
+  if (single && engine_config().leaf_batching) {
+    // one query per launch, batched leaf counting (measured on the MI355X at
+    // 1e15 .. 1e21: D 128 words up to 1e17, 256 at 1e18, 512 at 1e19, 640
+    // from 1e20; S2_hard 128 up to 1e17, 256, 384, 512, 640 at 1e18 .. 1e21)
+    if (limit < (d ? 16000000000ull : 18000000000ull)) return 128;
+    if (limit < (d ? 45000000000ull : 66000000000ull)) return 256;
+    if (limit < (d ? 60000000000ull : 250000000000ull)) return 384;
+    return limit < (d ? 220000000000ull : 900000000000ull) ? 512 : 640;
+  }
   if (limit < (d ? 5000000000ull : 15000000000ull)) return 128;
   if (limit < (d ? 15000000000ull : 35000000000ull)) return 256;
   if (limit < 80000000000ull) return 384;
@@ -633,7 +642,7 @@ std::vector<i128> run_group(const std::vector<HslQuery>& qs, pcg_stats* st, uint
   const bool overridden =
       cfg.segment_words || cfg.group_size || cfg.grid_pct || cfg.kernel_waves >= 0;
   Schedule sch;
-  sch.W = auto_segment_words(sum_limit, T.kind);
+  sch.W = auto_segment_words(sum_limit, T.kind, n_work == 1);
   int sched_source = 0;
   if (forced) {
     sch = *forced;
@@ -1248,7 +1257,7 @@ i128 hsl_compute_noalloc(const HslQuery& q, pcg_stats* stats)
 
   QueryPlan qp = plan_query(q, T.P);
   if (qp.empty) return 0;
-  const uint32_t W = auto_segment_words(qp.qd.limit + 1, T.kind);
+  const uint32_t W = auto_segment_words(qp.qd.limit + 1, T.kind, true);
   if (W < 128 || W > 4096 || (W % 128) != 0)
     throw pcg_error("segment words must be a multiple of 128 in [128, 4096]");
   const uint32_t cbits = std::max(slot_format(W).cbits, engine_config().slot_cbits);

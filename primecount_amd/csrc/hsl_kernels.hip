@@ -994,74 +994,76 @@ __device__ __forceinline__ void sieve_segments(
           u_qsteps++;
           phase(kHslPhaseBatchEntry);
         };
-        for (uint32_t jj = j; jj <= jb; jj = next_leaf(jj + 1)) {
-          u_lprimes++;
-          const uint32_t lo = readlane_u32(r_lo[0], jj);
-          const uint32_t hi = readlane_u32(r_hi[0], jj);
-          const uint32_t Tt = readlane_u32(r_T[0], jj);
-          const uint32_t T = Tt & 0x7fffffffu;
-          int M = 0;                                      // wave-uniform
-          if (Tt >> 31) {
-            u_entries += hi - lo;
-            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
-            uint32_t np_tot = 0, nl_tot = 0;
-            uint32_t rb = (lo + 1) % 480, mb = 2310 * ((lo + 1) / 480);
-            uint32_t f = lo + 1 + lane <= hi ? (uint32_t) factor[lo + 1 + lane] : 0u;
-            uint32_t cop = c_cop_ext[rb + lane];
-#pragma nounroll
-            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
-              const uint32_t nidx = base + lane + kBlock;
-              const uint32_t f_next = nidx <= hi ? (uint32_t) factor[nidx] : 0u;
-              uint32_t rbn = rb + kBlock, mbn = mb;
-              if (rbn >= 480) {
-                rbn -= 480;
-                mbn += 2310;
-              }
-              const uint32_t cop_next = c_cop_ext[rbn + lane];
-              const bool leaf = f > T;
-              const unsigned long long lb = __ballot(leaf);
-              const uint32_t nl = (uint32_t) __popcll(lb);
-              np_tot += (uint32_t) __popcll(__ballot(leaf && (f & 1)));
-              if (leaf) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                    (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
-                const uint32_t m = mb + cop;
-                lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
-                lqb[(qh + rank) & 127] = (uint8_t) jj;
-              }
-              qh += nl;
-              nl_tot += nl;
-              if (qh - qt >= 64)
-                drain_b(64);
-              f = f_next;
-              cop = cop_next;
-              rb = rbn;
-              mb = mbn;
+        // The leaves of the batch's primes with entries in [j, jb]: their
+        // factor entries (or primes) are gathered 64 at a time across the
+        // primes.  The ranges follow each other in prime order (s_ent is the
+        // inclusive scan of the window's entry counts), so entry e of the
+        // batch belongs to the last prime pe whose first entry pst is <= e,
+        // at factor-table (or prime) index lo(pe) + 1 + e - pst.  One load
+        // per lane and step replaces a step per prime; the leaves go to the
+        // queue with their prime's lane.
+        const uint32_t E1 = readlane_u32(s_ent, jb);
+        uint32_t jc = j;                                  // first prime of the step
+        uint32_t jc_st = readlane_u32(s_ent, j) - (readlane_u32(r_hi[0], j) - readlane_u32(r_lo[0], j));
+        {
+          // M of a prime whose leaves are primes: all of them, positive
+          const bool in_b = lane >= j && lane <= jb && ((lmask >> lane) & 1) && !(r_T[0] >> 31);
+          w_m[0] = in_b ? (int) (r_hi[0] - r_lo[0]) : w_m[0];
+          u_lprimes += (uint32_t) __popcll(lmask & (~0ull << j) & (jb >= 63 ? ~0ull : ((2ull << jb) - 1)));
+        }
+        for (uint32_t e0 = jc_st; e0 < E1; e0 += kBlock) {
+          const uint32_t e = e0 + lane;
+          uint32_t pe = jc, pst = jc_st;
+          uint32_t nst = readlane_u32(s_ent, jc);        // first entry of the next prime
+          for (uint32_t jj = next_leaf(jc + 1); jj <= jb && nst < e0 + kBlock;
+               jj = next_leaf(jj + 1)) {
+            if (e >= nst) {
+              pe = jj;
+              pst = nst;
             }
-            u_leaves += nl_tot;
-            M = (int) (2 * np_tot) - (int) nl_tot;
-          } else {
-            // primes q: leaves +phi(xp / q, b - 1), queued with a positive sign
-            u_leaves += hi - lo;
-            u_lsteps += (hi - lo + kBlock - 1) / kBlock;
-            M = (int) (hi - lo);
-            uint32_t q = lo + 1 + lane <= hi ? primes[lo + 1 + lane] : 0u;
-#pragma nounroll
-            for (uint32_t base = lo + 1; base <= hi; base += kBlock) {
-              const uint32_t idx = base + lane;
-              const uint32_t q_next = idx + kBlock <= hi ? primes[idx + kBlock] : 0u;
-              const uint32_t nl = min((uint32_t) kBlock, hi - base + 1);
-              if (idx <= hi) {
-                lq[(qh + lane) & 127] = (q >> 1) | 0x80000000u;
-                lqb[(qh + lane) & 127] = (uint8_t) jj;
-              }
-              qh += nl;
-              if (qh - qt >= 64)
-                drain_b(64);
-              q = q_next;
+            nst = readlane_u32(s_ent, jj);
+          }
+          const bool valid = e < E1;
+          const int src = (int) (pe << 2);
+          const uint32_t lo = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_lo[0]);
+          const uint32_t Tt = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_T[0]);
+          const uint32_t idx = lo + 1 + (e - pst);
+          const bool t1 = (Tt >> 31) != 0;
+          uint32_t f = 1, m = 0;                         // a prime leaf: positive
+          if (valid) {
+            if (t1) {
+              f = (uint32_t) factor[idx];
+              const uint32_t qd = idx / 480;
+              m = 2310 * qd + c_cop[idx - 480 * qd];     // to_number(idx)
+            } else {
+              m = primes[idx];
             }
           }
-          w_m[0] = lane == jj ? M : w_m[0];
+          const bool leaf = valid && (!t1 || f > (Tt & 0x7fffffffu));
+          const unsigned long long lb = __ballot(leaf);
+          if (leaf) {
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
+            lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
+            lqb[(qh + rank) & 127] = (uint8_t) pe;
+          }
+          const uint32_t nl = (uint32_t) __popcll(lb);
+          u_entries += (uint32_t) __popcll(__ballot(valid && t1));
+          u_leaves += nl;
+          u_lsteps++;
+          // M of the step's primes with factor entries: 2 (positive leaves) - leaves
+          const uint32_t pe_last = readlane_u32(pe, 63);
+          for (uint32_t jj = jc; jj <= pe_last; jj = next_leaf(jj + 1)) {
+            const bool mine = leaf && t1 && pe == jj;
+            const int a = (int) __popcll(__ballot(mine));
+            const int pp = (int) __popcll(__ballot(mine && (f & 1)));
+            w_m[0] += lane == jj ? 2 * pp - a : 0;
+          }
+          qh += nl;
+          if (qh - qt >= 64)
+            drain_b(64);
+          jc = pe_last;
+          jc_st = readlane_u32(pst, 63);
         }
         if (qh != qt)
           drain_b(qh - qt);
