@@ -25,6 +25,7 @@ import math
 import os
 import re
 import subprocess
+import statistics
 import sys
 import time
 
@@ -321,6 +322,34 @@ def phase_split(pc, x):
             "wave_cycles": tot}
 
 
+SCHEDULES = ((512, 6), (512, 0), (640, 0))
+
+
+def schedules(pc, x, calls):
+    """Kernel time of D(x) (default parameters) under fixed schedules: segment
+    words W and the kernel build (6 = the 6-waves build, 0 = the plain one),
+    median of `calls` calls each after one call under the default schedule
+    (which builds the tables), every result checked against that one."""
+    want = pc.D_default(x)
+    out = []
+    try:
+        for w, waves in SCHEDULES:
+            pc.set_schedule(words=w, waves=waves)
+            ts = []
+            for _ in range(calls):
+                if pc.D_default(x) != want:
+                    raise SystemExit("D(%d) differs under W=%d waves=%d" % (x, w, waves))
+                ts.append(1e3 * pc.last_stats()["seconds_kernel"])
+            st = pc.last_stats()
+            out.append({"segment_words": w, "kernel_build": "6 waves" if waves == 6 else "plain",
+                        "kernel_ms": round(statistics.median(ts), 3), "calls": calls,
+                        "windows": st["windows"], "slots_visited": st["pairs"],
+                        "bit_exact": True})
+    finally:
+        pc.set_schedule()
+    return {"x": str(x), "schedules": out}
+
+
 def device_copy(nbytes, reps=2):
     """A plain device-to-device copy of `nbytes` (HIP's own copy kernel), used
     to calibrate the rocprofv3 FETCH_SIZE / WRITE_SIZE counters."""
@@ -398,6 +427,9 @@ def main():
     ap.add_argument("--phase-split", action="store_true",
                     help="only measure the sieve kernel's time per phase at D(x) with the "
                          "phase-timer build and print its JSON line")
+    ap.add_argument("--schedules", action="store_true",
+                    help="only time D(x) under fixed schedules (segment words, kernel build), "
+                         "--steps calls each, and print their JSON line")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device-to-device copy (for rocprofv3 --pmc "
                          "FETCH_SIZE/WRITE_SIZE calibration)")
@@ -409,14 +441,17 @@ def main():
         sys.exit(2)
 
     import primecount_amd as pc
+    if a.segment_words:
+        pc.set_segment_words(a.segment_words)
     if a.pi_small:
         print(json.dumps({"pi_small": pi_small(pc, a.pi_small)}), flush=True)
         return
     if a.phase_split:
         print(json.dumps({"phase_split": phase_split(pc, int(round(a.x)))}), flush=True)
         return
-    if a.segment_words:
-        pc.set_segment_words(a.segment_words)
+    if a.schedules:
+        print(json.dumps({"schedule_table": schedules(pc, int(round(a.x)), a.steps)}), flush=True)
+        return
     x = int(round(a.x))
     p = pc.gourdon_params(x)
     want, _ = expected_D(x)
