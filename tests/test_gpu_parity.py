@@ -318,3 +318,36 @@ def test_phase_timer_build(pc):
         assert sum(pc.last_stats()["phase_cycles"].values()) == 0
     finally:
         pc.set_phase_timers(False)
+
+
+@pytest.mark.parametrize("words", [128, 256, 512, 640])
+def test_kernel_builds_agree(pc, oracle, words):
+    # the plain sieve kernel and the build for 6 waves per SIMD (fewer
+    # registers, a few spills) give the reference's sums at each segment
+    # size, single queries and a batch
+    rows = {int(v["x"]): v for v in golden("configs.json")["D"]}
+    s2 = {int(v["x"]): v for v in golden("configs.json")["S2_hard"]}
+    kat = [[int(v) for v in r] for r in golden("kat_D.json")["rows"]][1::5]
+    rng = random.Random(777 + words)
+    rnd = []
+    for _ in range(3):
+        x = rng.randrange(10 ** 10, 10 ** 11)
+        x13 = round(x ** (1 / 3))
+        y = rng.randint(x13 + 1, 3 * x13)
+        rnd.append((x, y, rng.randint(y, 2 * y), rng.choice([6, 7, 8])))
+    try:
+        for waves in (0, 6):
+            pc.set_schedule(words=words, waves=waves)
+            log("kernel build %s, %d-word segments: D 1e17, 1e18, S2_hard 1e17"
+                % ("6 waves" if waves else "plain", words))
+            for e in (17, 18):
+                assert pc.D_default(10 ** e) == int(rows[10 ** e]["value"])
+                st = pc.last_stats()
+                assert st["segment_words"] == words and st["kernel_waves"] == waves
+            assert pc.S2_hard_default(10 ** 17) == int(s2[10 ** 17]["value"])
+            for x, y, z, k, want in kat:
+                assert pc.D(x, y, z, k) == want, (x, y, z, k)
+            for x, y, z, k in rnd:
+                assert pc.D(x, y, z, k) == oracle.D(x, y, z, k), (x, y, z, k)
+    finally:
+        pc.set_schedule()
