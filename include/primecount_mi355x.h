@@ -183,6 +183,16 @@ int pcg_factor_table_check(int kind, int64_t y, int64_t limit, uint64_t* mismatc
  * the reference (its threads carry phi through their own segments). */
 int pcg_set_pass_slots(uint64_t slots);
 
+/* Largest prime-count table (bytes; 16 bytes per 240 numbers) that pi(x)
+ * keeps whole in device memory (0 = default, 65% of the device memory).  A
+ * pi(x) or Deleglise-Rivat pi(x) whose table would be larger builds it in
+ * windows of this size, one after the other, and sums the leaves of A, C, B
+ * (and S2_easy, P2) window by window: device memory stays bounded, the
+ * results do not change.  Replaces the role of the reference's
+ * SegmentedPiTable (src/gourdon/SegmentedPiTable.cpp), whose x^(1/4)-sized
+ * segments bound its memory the same way. */
+int pcg_set_pi_table_bytes(uint64_t bytes);
+
 /* Bits of C in the 32-bit slot word (C | M << bits).  0 (default) takes the
  * fewest the segment size allows (C <= 64 words); more bits leave fewer for
  * M, so that more slots escape to the per-pass escape table.  14..28.  Never
@@ -338,6 +348,7 @@ typedef struct pcg_pi_stats {
                                 descriptor upload, D its host planning)       */
   uint64_t ac_leaves;        /* A and C leaves summed                         */
   uint64_t table_limit;      /* the prime-count table covers [0, table_limit] */
+  uint64_t table_windows;    /* windows the table was built in (1 = whole)    */
 } pcg_pi_stats;
 
 /* Statistics of the last pcg_pi call on this thread. */

@@ -118,7 +118,7 @@ class PiStats(ctypes.Structure):
                 ("seconds_sigma", ctypes.c_double), ("seconds_phi0", ctypes.c_double),
                 ("seconds_ac", ctypes.c_double), ("seconds_b", ctypes.c_double),
                 ("seconds_d", ctypes.c_double), ("ac_leaves", ctypes.c_uint64),
-                ("table_limit", ctypes.c_uint64)]
+                ("table_limit", ctypes.c_uint64), ("table_windows", ctypes.c_uint64)]
 
 
 KIND_D = 0
@@ -162,6 +162,7 @@ def load():
         "pcg_set_segment_words": ([ctypes.c_uint32], ctypes.c_int),
         "pcg_set_host_threads": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_pass_slots": ([ctypes.c_uint64], ctypes.c_int),
+        "pcg_set_pi_table_bytes": ([ctypes.c_uint64], ctypes.c_int),
         "pcg_set_shared_segments": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_leaf_batching": ([ctypes.c_int], ctypes.c_int),
         "pcg_set_phase_timers": ([ctypes.c_int], ctypes.c_int),
@@ -460,6 +461,13 @@ def set_schedule_cache(path=None):
     _check(load().pcg_set_schedule_cache(None if path is None else os.fsencode(path),
                                          ctypes.byref(n)))
     return n.value
+
+
+def set_pi_table_bytes(n):
+    """Largest prime-count table pi(x) keeps whole (bytes, 0 = 65% of the
+    device memory); a larger one is built and read in windows of this size
+    (same results, bounded memory)."""
+    _check(load().pcg_set_pi_table_bytes(ctypes.c_uint64(int(n))))
 
 
 def set_pass_slots(n):

@@ -227,6 +227,7 @@ int pcg_pi(pcg_i128 x, pcg_i128* out)
     t_pi_stats.seconds_d = st.seconds_d;
     t_pi_stats.ac_leaves = st.ac_leaves;
     t_pi_stats.table_limit = st.table_limit;
+    t_pi_stats.table_windows = st.table_windows;
   });
 }
 
@@ -246,6 +247,7 @@ int pcg_nth_prime(int64_t n, int64_t* out)
     t_pi_stats.seconds_d = st.seconds_d;
     t_pi_stats.ac_leaves = st.ac_leaves;
     t_pi_stats.table_limit = st.table_limit;
+    t_pi_stats.table_windows = st.table_windows;
   });
 }
 
@@ -528,6 +530,15 @@ int pcg_set_pass_slots(uint64_t slots)
     if (slots != 0 && slots < 64)
       throw pcg_error("pass slots must be 0 or >= 64");
     engine_config().max_slots_per_pass = slots;
+  });
+}
+
+int pcg_set_pi_table_bytes(uint64_t bytes)
+{
+  return guarded([&] {
+    if (bytes != 0 && bytes < (1u << 20))
+      throw pcg_error("prime-count table bytes must be 0 or >= 2^20");
+    engine_config().pix_table_bytes = bytes;
   });
 }
 

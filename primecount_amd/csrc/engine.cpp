@@ -189,6 +189,7 @@ struct Device {
   hipStream_t stream = nullptr;
   DevBuf primes, factor, pi_bits, pi_cnt;
   DevBuf pix_tab, pix_seg;            // prime-count table over [0, pix_n]
+  DevBuf pix_win;                     // one window of a prime-count table too large to keep whole
   char* scratch = nullptr;            // the device's scratch array (no-allocation entry point)
   uint64_t pix_n = 0;
 
@@ -220,7 +221,7 @@ struct Device {
 
   void release()
   {
-    for (DevBuf* b : {&primes, &factor, &pi_bits, &pi_cnt, &pix_tab, &pix_seg})
+    for (DevBuf* b : {&primes, &factor, &pi_bits, &pi_cnt, &pix_tab, &pix_seg, &pix_win})
       b->release();
     pix_n = 0;
   }
@@ -1623,11 +1624,53 @@ static void ensure_pix_table(uint64_t n)
   hipStream_t s = D.stream;
   PCG_HIP(pix_launch_sieve((PixWord*) D.pix_tab.p, (uint64_t*) D.pix_seg.p,
                            (const uint32_t*) D.primes.p, std::max<uint32_t>(j_small, 4),
-                           std::max<uint32_t>(j_end, 4), nwords, n, s));
+                           std::max<uint32_t>(j_end, 4), nwords, n, 0, s));
   PCG_HIP(pix_launch_scan((uint64_t*) D.pix_seg.p, (uint32_t) nseg, s));
-  PCG_HIP(pix_launch_counts((PixWord*) D.pix_tab.p, (const uint64_t*) D.pix_seg.p, nwords, s));
+  PCG_HIP(pix_launch_counts((PixWord*) D.pix_tab.p, (const uint64_t*) D.pix_seg.p, nwords, 0, s));
   PCG_HIP(hipStreamSynchronize(s));
   D.pix_n = nwords * 240 - 1 < n ? nwords * 240 - 1 : n;
+}
+
+uint64_t engine_pix_budget()
+{
+  const EngineConfig& cfg = engine_config();
+  if (cfg.pix_table_bytes) return cfg.pix_table_bytes;
+  g_dev.init();
+  size_t free_b = 0, total_b = 0;
+  PCG_HIP(hipMemGetInfo(&free_b, &total_b));
+  return (uint64_t) (0.65 * (double) total_b);
+}
+
+const PixWord* engine_pix_window(uint64_t w_lo, uint64_t nwords, uint64_t n_max, uint64_t base,
+                                 uint64_t* next_base)
+{
+  Device& D = g_dev;
+  throw_if_frozen();
+  const TableSet& T = g_tables;
+  if (nwords == 0) throw pcg_error("internal: empty prime-count window", kErrInternal);
+  // numbers [240 w_lo, min(240 (w_lo + nwords), n_max + 1)): sieving primes
+  // up to the square root of the last one
+  const uint64_t last = std::min<uint64_t>(240 * (w_lo + nwords) - 1, n_max);
+  if (last < 240 * w_lo) throw pcg_error("internal: prime-count window past its limit", kErrInternal);
+  const uint64_t r = isqrt(last);
+  if (r > T.P.max) throw pcg_error("internal: prime table too small for the prime-count window", kErrInternal);
+  const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
+  D.pix_win.ensure(nwords * sizeof(PixWord));
+  D.pix_seg.ensure(nseg * 8);
+  const uint32_t j_end = (uint32_t) T.P.pi(r) + 1;
+  const uint32_t j_small = (uint32_t) std::min<uint64_t>(T.P.pi(std::min<uint64_t>(r, 240ull * kPixSegWords / 64)) + 1, j_end);
+  hipStream_t s = D.stream;
+  PixWord* tab = (PixWord*) D.pix_win.p;
+  PCG_HIP(pix_launch_sieve(tab, (uint64_t*) D.pix_seg.p, (const uint32_t*) D.primes.p,
+                           std::max<uint32_t>(j_small, 4), std::max<uint32_t>(j_end, 4), nwords,
+                           n_max, w_lo, s));
+  PCG_HIP(pix_launch_scan((uint64_t*) D.pix_seg.p, (uint32_t) nseg, s));
+  PCG_HIP(pix_launch_counts(tab, (const uint64_t*) D.pix_seg.p, nwords, base, s));
+  PixWord lastw;
+  PCG_HIP(hipMemcpyAsync(&lastw, tab + nwords - 1, sizeof(PixWord), hipMemcpyDeviceToHost, s));
+  PCG_HIP(hipStreamSynchronize(s));
+  *next_base = lastw.cnt + (uint64_t) __builtin_popcountll(lastw.bits);
+  return tab;
 }
 
 EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n, LeafSum kind, uint64_t pmin)

@@ -37,6 +37,8 @@ struct EngineConfig {
   uint32_t group_size = 0;      // 0 = schedule's; 2 or 4 queries per shared segment
   uint32_t grid_pct = 0;        // 0 = schedule's; else percent of the occupancy grid
   int kernel_waves = -1;        // -1 = schedule's; 0 or 6 (single-query kernel build)
+  uint64_t pix_table_bytes = 0; // largest prime-count table pi(x) keeps whole (0 = 65% of
+                                // the device memory); above it the table is built in windows
 };
 
 EngineConfig& engine_config();
@@ -160,6 +162,18 @@ hipEvent_t* engine_term_events();
 /// caller holds the lock.
 EngineView engine_view(int64_t y, int64_t z, uint64_t pix_n, LeafSum kind = LeafSum::D,
                        uint64_t pmin = 0);
+
+/// Bytes of the largest prime-count table kept whole (pix_table_bytes, or
+/// 65% of the device memory).
+uint64_t engine_pix_budget();
+
+/// One window of the prime-count table: the words [w_lo, w_lo + nwords)
+/// (numbers up to n_max), counts absolute given `base`, the number of primes
+/// >= 7 below word w_lo; returns the window (valid until the next call) and
+/// sets *next_base to the number of primes >= 7 below its end.  Caller holds
+/// the lock; the window replaces the previous one.
+const PixWord* engine_pix_window(uint64_t w_lo, uint64_t nwords, uint64_t n_max, uint64_t base,
+                                 uint64_t* next_base);
 
 /// A device buffer of at least `bytes`, kept between calls (slot 0 .. 7);
 /// caller holds the lock.

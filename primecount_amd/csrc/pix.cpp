@@ -9,7 +9,11 @@
 // walks the same leaves segment by segment with a small pi table
 // (SegmentedPiTable, x^(1/4) memory) and clusters equal pi values; here the
 // whole table up to max(x / y, sqrt(x)) lives in HBM (7.5 GB at x = 1e19),
-// which turns every leaf into two loads.
+// which turns every leaf into two loads.  A table above the engine's budget
+// (65% of the device memory by default: 1e22 needs 354 GB) is built in
+// windows instead: the part up to sqrt(x) that Sigma and B's primes read
+// stays whole, and the leaf sums of A, C and B run once per window, each
+// leaf adding its pi value in the window that holds its quotient.
 
 #include "pix.hpp"
 
@@ -58,6 +62,52 @@ uint64_t table_need(i128 x, int64_t y, int64_t x_star)
   if (n >= ((u128) 1 << 49))
     throw pcg_error("pi(x): x / y must be < 2^49 (prime-count table limit)");
   return (uint64_t) n + 1;
+}
+
+/// Largest n that Sigma, pi(isqrt(x)) and B's primes read pi(n) or the
+/// table's bits for: the part of the table kept whole when the rest is built
+/// in windows.
+uint64_t table_low_need(i128 x, int64_t y, int64_t x_star)
+{
+  u128 n = isqrt((u128) x);
+  n = std::max<u128>(n, (u128) x / ((u128) x_star * (u128) y));
+  n = std::max<u128>(n, isqrt((u128) x / (u128) x_star));
+  n = std::max<u128>(n, (u128) y);
+  return (uint64_t) n + 1;
+}
+
+/// Table words per window when a table of `need` numbers is built in
+/// windows (0: it is kept whole): the fewest windows of at most the budget,
+/// of equal size up to a sieve segment.
+uint64_t window_words(uint64_t need)
+{
+  const uint64_t words = need / 240 + 1;
+  const uint64_t budget = std::max<uint64_t>(engine_pix_budget() / sizeof(PixWord), kPixSegWords);
+  if (words <= budget) return 0;
+  // windows of equal size, whole sieve segments
+  const uint64_t nwin = (words + budget - 1) / budget;
+  const uint64_t per = (words + nwin - 1) / nwin;
+  return (per + kPixSegWords - 1) / kPixSegWords * kPixSegWords;
+}
+
+/// Builds the prime-count table up to `need` in windows of win words, lowest
+/// first, and calls f(tab, w_lo, w_hi) for each (tab holds the words
+/// [w_lo, w_hi) with absolute counts, the primes below a window carried from
+/// the one before); adds the build time to *table_s; returns the windows.
+template <typename F>
+uint64_t for_windows(uint64_t need, uint64_t win, double* table_s, F&& f)
+{
+  const uint64_t words = need / 240 + 1;
+  uint64_t base = 0, n = 0;
+  for (uint64_t w_lo = 0; w_lo < words; w_lo += win) {
+    const uint64_t nw = std::min<uint64_t>(win, words - w_lo);
+    const double t = now_s();
+    const PixWord* tab = engine_pix_window(w_lo, nw, need, base, &base);
+    *table_s += now_s() - t;
+    f(tab, w_lo, w_lo + nw);
+    n++;
+  }
+  return n;
 }
 
 i128 acc_value(const uint64_t* a)
@@ -221,11 +271,15 @@ i128 run_phi0(i128 x, int64_t y, int64_t z, int64_t k, const EngineView& v)
   return phi_tiny_host(x, t) + acc_value(A.h);
 }
 
-/// B: the sum of pi(x / p) over y < p <= isqrt(x), into acc[0..1].
-void launch_B(i128 x, int64_t y, const EngineView& v, unsigned long long* acc)
+/// B: the sum of pi(x / p) over y < p <= isqrt(x), into acc[0..1]; the
+/// primes from the resident table, pi(x / p) from tab, which holds the
+/// table words [w_lo, w_hi) (the resident table: 0, UINT64_MAX).
+void launch_B(i128 x, int64_t y, const EngineView& v, unsigned long long* acc,
+              const PixWord* tab = nullptr, uint64_t w_lo = 0, uint64_t w_hi = ~0ull)
 {
+  if (isqrt((u128) x) > v.pix_n) throw pcg_error("internal: prime-count table too small for B", kErrInternal);
   PIX_HIP(pix_launch_B((uint64_t) x, (uint64_t) ((u128) x >> 64), (uint64_t) y, isqrt((u128) x),
-                       v.pix_tab, acc, v.stream));
+                       v.pix_tab, tab ? tab : v.pix_tab, w_lo, w_hi, acc, v.stream));
 }
 
 i128 run_B(i128 x, int64_t y, const EngineView& v)
@@ -426,7 +480,7 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P, uint64_t max_quotient)
   }
   if (top >= kPixTileMaxPrime || max_quotient >= kPixTileMaxQuotient) {
     // the tiled kernel's 32-bit residual needs primes below 2^30, its
-    // quotient guard quotients below 2^41; the flat kernel is exact below 2^49
+    // quotient guard quotients below 2^43; the flat kernel is exact below 2^49
     W.plus = std::move(L.plus);
     W.n_plus = L.n_plus;
     return W;
@@ -478,10 +532,9 @@ AcWork ac_work(LeafLists&& L, const PrimeTables& P, uint64_t max_quotient)
   return W;
 }
 
-/// Uploads the A/C work to dd (W.bytes() bytes) and launches the leaf sums:
-/// plus into acc[0..1], minus into acc[2..3].  W must stay alive until the
-/// stream has passed the copies.
-void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long long* acc)
+/// Uploads the A/C work to dd (W.bytes() bytes).  W must stay alive until
+/// the stream has passed the copies.
+void upload_ac(const AcWork& W, const EngineView& v, char* dd)
 {
   PixLeafDesc* dp = (PixLeafDesc*) dd;
   PixTileDesc* dt = (PixTileDesc*) (dp + W.plus.size() + 1);
@@ -499,13 +552,33 @@ void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long lon
   if (!W.minus.empty())
     PIX_HIP(hipMemcpyAsync(dm, W.minus.data(), W.minus.size() * sizeof(PixLeafDesc),
                            hipMemcpyHostToDevice, v.stream));
-  PIX_HIP(pix_launch_leafsum(dp, (uint32_t) W.plus.size(), W.n_plus, v.primes, v.pix_tab, acc,
-                             (uint32_t) v.cus, v.stream));
+}
+
+/// The leaf sums of the A/C work uploaded to dd: plus into acc[0..1], minus
+/// into acc[2..3], reading pi from tab, which holds the table words
+/// [w_lo, w_hi) (the resident table: nullptr); max_q bounds every quotient.
+void sum_ac(const AcWork& W, const EngineView& v, const char* dd, unsigned long long* acc,
+            uint64_t max_q, const PixWord* tab = nullptr, uint64_t w_lo = 0,
+            uint64_t w_hi = ~0ull)
+{
+  const PixLeafDesc* dp = (const PixLeafDesc*) dd;
+  const PixTileDesc* dt = (const PixTileDesc*) (dp + W.plus.size() + 1);
+  const PixTileGroup* dg = (const PixTileGroup*) (dt + W.tiles.size() + 1);
+  const PixLeafDesc* dm = (const PixLeafDesc*) (dg + W.groups.size() + 1);
+  if (!tab) tab = v.pix_tab;
+  PIX_HIP(pix_launch_leafsum(dp, (uint32_t) W.plus.size(), W.n_plus, v.primes, tab, w_lo, w_hi,
+                             acc, (uint32_t) v.cus, v.stream));
   PIX_HIP(pix_launch_leaftiles(dt, dg, (uint32_t) W.groups.size(), W.nitems, v.primes,
-                               (uint32_t) v.nprimes, v.pix_tab, v.pix_n, acc, (uint32_t) v.cus,
-                               v.stream));
-  PIX_HIP(pix_launch_leafsum(dm, (uint32_t) W.minus.size(), W.n_minus, v.primes, v.pix_tab,
+                               (uint32_t) v.nprimes, tab, w_lo, w_hi, max_q, acc,
+                               (uint32_t) v.cus, v.stream));
+  PIX_HIP(pix_launch_leafsum(dm, (uint32_t) W.minus.size(), W.n_minus, v.primes, tab, w_lo, w_hi,
                              acc + 2, (uint32_t) v.cus, v.stream));
+}
+
+void launch_ac(const AcWork& W, const EngineView& v, char* dd, unsigned long long* acc)
+{
+  upload_ac(W, v, dd);
+  sum_ac(W, v, dd, acc, v.pix_n);
 }
 
 i128 ac_value(const AcWork& W, const uint64_t* h)
@@ -628,11 +701,13 @@ void add_elapsed(hipEvent_t a, hipEvent_t b, double* sec)
 /// table set serves (view v).  Sigma, Phi0, B and the A/C leaf sums of every
 /// query are launched back to back on the engine stream (the host builds
 /// the A/C descriptor lists meanwhile), then the D leaves of all of them in
-/// one persistent launch; the term sums are read back once.  Caller holds the
-/// engine lock.
+/// one persistent launch; the term sums are read back once.  With win > 0
+/// the prime-count table up to `need` is built in windows of win words
+/// (v's resident table covers the low part only) and B and the A/C sums run
+/// once per window.  Caller holds the engine lock.
 void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector<size_t>& idx,
               const std::vector<GourdonParams>& gps, const EngineView& v,
-              std::vector<i128>& result, PixStats& s)
+              std::vector<i128>& result, PixStats& s, uint64_t need = 0, uint64_t win = 0)
 {
   const size_t n = idx.size();
   if (n == 0) return;
@@ -661,28 +736,43 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
     launch_phi0(xs[idx[j]], gp.z, gp.k, v, t, phi_tiny_device(t, v), acc + kTermWords * j + 6);
   }
   PIX_HIP(hipEventRecord(ev[2], v.stream));
-  for (size_t j = 0; j < n; j++)
-    launch_B(xs[idx[j]], gps[idx[j]].y, v, acc + kTermWords * j + 8);
+  if (!win)
+    for (size_t j = 0; j < n; j++)
+      launch_B(xs[idx[j]], gps[idx[j]].y, v, acc + kTermWords * j + 8);
   PIX_HIP(hipEventRecord(ev[3], v.stream));
 
   // A and C: descriptor lists on the host while the kernels above run
+  const uint64_t max_q = win ? need : v.pix_n;
   std::vector<AcWork> L(n);
   size_t bytes = 0;
   for (size_t j = 0; j < n; j++) {
     const GourdonParams& gp = gps[idx[j]];
-    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host,
-                   v.pix_n);
+    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host, max_q);
     bytes += L[j].bytes();
     s.ac_leaves += L[j].leaves;
   }
-  char* dd = (char*) engine_buffer(4, bytes + 64);
+  char* dd0 = (char*) engine_buffer(4, bytes + 64);
+  char* dd = dd0;
   for (size_t j = 0; j < n; j++) {
-    launch_ac(L[j], v, dd, acc + kTermWords * j + 10);
+    upload_ac(L[j], v, dd);
+    if (!win) {
+      sum_ac(L[j], v, dd, acc + kTermWords * j + 10, max_q);
+      // a pass point after each query's A/C launch: other work queued now
+      // runs after it
+      if (j + 1 < n) engine_pass_point();
+    }
     dd += L[j].bytes();
-    // a pass point after each query's A/C launch: other work queued now
-    // runs after it
-    if (j + 1 < n) engine_pass_point();
   }
+  if (win)
+    s.table_windows += for_windows(need, win, &s.seconds_table,
+                                   [&](const PixWord* tab, uint64_t w_lo, uint64_t w_hi) {
+      dd = dd0;
+      for (size_t j = 0; j < n; j++) {
+        launch_B(xs[idx[j]], gps[idx[j]].y, v, acc + kTermWords * j + 8, tab, w_lo, w_hi);
+        sum_ac(L[j], v, dd, acc + kTermWords * j + 10, max_q, tab, w_lo, w_hi);
+        dd += L[j].bytes();
+      }
+    });
   PIX_HIP(hipEventRecord(ev[4], v.stream));
 
   // D: the hard special leaves of all queries in one persistent launch
@@ -762,25 +852,36 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st, ui
   }
   if (!small.empty() || !shared.empty()) {
     auto lock = engine_lock();
-    const EngineView v = engine_view(ymax, ymax, need);
+    // a table over the budget: the low part whole, the rest in windows
+    const uint64_t win = window_words(need);
+    uint64_t low = 0;
+    if (win) {
+      for (size_t i : shared) low = std::max<uint64_t>(low, table_low_need(xs[i], gps[i].y, gps[i].x_star));
+      for (size_t i : small) low = std::max<uint64_t>(low, (uint64_t) xs[i]);
+    }
+    const EngineView v = engine_view(ymax, ymax, win ? low : need);
     s.seconds_table += now_s() - t0;
-    s.table_limit = v.pix_n;
+    s.table_limit = win ? need : v.pix_n;
+    s.table_windows = win ? 0 : 1;
     if (!small.empty()) {
       std::vector<uint64_t> ns;
       for (size_t i : small) ns.push_back((uint64_t) xs[i]);
       const std::vector<uint64_t> p = pi_points(v, ns);
       for (size_t j = 0; j < small.size(); j++) result[small[j]] = (i128) p[j];
     }
-    pi_group(max_slots, xs, shared, gps, v, result, s);
+    pi_group(max_slots, xs, shared, gps, v, result, s, need, win);
   }
   for (const auto& g : alone) {
     const GourdonParams& gp = gps[g[0]];
     auto lock = engine_lock();
     const double t = now_s();
-    const EngineView v = engine_view(gp.y, gp.z, table_need(xs[g[0]], gp.y, gp.x_star));
+    const uint64_t gneed = table_need(xs[g[0]], gp.y, gp.x_star);
+    const uint64_t win = window_words(gneed);
+    const EngineView v = engine_view(gp.y, gp.z, win ? table_low_need(xs[g[0]], gp.y, gp.x_star) : gneed);
     s.seconds_table += now_s() - t;
-    s.table_limit = std::max<uint64_t>((uint64_t) s.table_limit, (uint64_t) v.pix_n);
-    pi_group(max_slots, xs, g, gps, v, result, s);
+    s.table_limit = std::max<uint64_t>((uint64_t) s.table_limit, win ? gneed : (uint64_t) v.pix_n);
+    if (!win) s.table_windows = std::max<uint64_t>(s.table_windows, 1);
+    pi_group(max_slots, xs, g, gps, v, result, s, gneed, win);
   }
   // the reference checks every pi(x) against Schoenfeld's bound
   // (src/gourdon/pi_gourdon.cpp:89,160)
@@ -829,8 +930,11 @@ uint64_t pi_footprint(i128 x)
   // the per-query scratch: A/C descriptors and term sums (under 64 MiB up to
   // x = 1e19, about x^(1/3) growth beyond)
   const double scale = std::max(1.0, std::cbrt((double) x / 1e19));
-  return (need / 240 + 1) * sizeof(PixWord) + table_bytes_D(gp.y, gp.z) +
-         (uint64_t) (scale * (64u << 20));
+  // a table over the budget: its low part plus one window
+  const uint64_t win = window_words(need);
+  const uint64_t table = win ? (table_low_need(x, gp.y, gp.x_star) / 240 + 1 + win) * sizeof(PixWord)
+                             : (need / 240 + 1) * sizeof(PixWord);
+  return table + table_bytes_D(gp.y, gp.z) + (uint64_t) (scale * (64u << 20));
 }
 
 uint64_t nth_prime_footprint(uint64_t n)
@@ -989,6 +1093,34 @@ i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
   return ac_value(W, A.h);
 }
 
+/// S2_easy - P2 with the prime-count table up to `need` built in windows of
+/// win words (v's resident table covers [0, max(sqrt(x), y)]).
+i128 s2_easy_minus_p2_windowed(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v,
+                               uint64_t need, uint64_t win)
+{
+  const AcWork W = ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host, need);
+  Acc A;                      // S2_easy plus [0, 2), minus [2, 4), P2's sum [4, 6)
+  A.d = (unsigned long long*) engine_buffer(0, 128);
+  acc_zero(A, v.stream);
+  char* dd = (char*) engine_buffer(4, W.bytes());
+  upload_ac(W, v, dd);
+  const uint64_t sqrtx = isqrt((u128) x);
+  const bool p2 = x >= 4 && (uint64_t) y < sqrtx;
+  double table_s = 0;
+  for_windows(need, win, &table_s, [&](const PixWord* tab, uint64_t w_lo, uint64_t w_hi) {
+    sum_ac(W, v, dd, A.d, need, tab, w_lo, w_hi);
+    if (p2) launch_B(x, y, v, A.d + 4, tab, w_lo, w_hi);
+  });
+  acc_read(A, v.stream);
+  i128 r = ac_value(W, A.h);
+  if (p2) {
+    const i128 a = (i128) v.host->pi((uint64_t) y);
+    const i128 b = (i128) pi_points(v, {sqrtx})[0];
+    r -= (a - 2) * (a + 1) / 2 - (b - 2) * (b + 1) / 2 + acc_value(A.h + 4);
+  }
+  return r;
+}
+
 void check_dr(i128 x, int64_t y, int64_t z, int64_t c)
 {
   if (x < 1 || y < 1 || z < 1)
@@ -1037,13 +1169,21 @@ i128 pi_deleglise_rivat(i128 x)
   const DRParams p = deleglise_rivat_params(x);
   check_dr(x, p.y, p.z, p.c);
   auto lock = engine_lock();
-  const EngineView v = engine_view(p.y, p.z, dr_table_need(x, p.y), LeafSum::S2_hard,
-                                     (uint64_t) p.y + 1);
+  // a table over the budget: [0, max(sqrt(x), y)] whole, the rest in windows
+  const uint64_t need = dr_table_need(x, p.y);
+  const uint64_t win = window_words(need);
+  const uint64_t low = std::max<uint64_t>(isqrt((u128) x), (uint64_t) p.y) + 1;
+  const EngineView v = engine_view(p.y, p.z, win ? low : need, LeafSum::S2_hard,
+                                   (uint64_t) p.y + 1);
   if ((uint64_t) p.y > v.host->max) throw pcg_error("internal: prime table below y", kErrInternal);
   i128 r = run_s1(x, p.y, p.c, v);
   r += s2_trivial_host(x, p.y, p.z, p.c, *v.host);
-  r += run_s2_easy(x, p.y, p.z, p.c, v);
-  r -= run_p2(x, p.y, v);
+  if (win) {
+    r += s2_easy_minus_p2_windowed(x, p.y, p.z, p.c, v, need, win);
+  } else {
+    r += run_s2_easy(x, p.y, p.z, p.c, v);
+    r -= run_p2(x, p.y, v);
+  }
   r += (i128) v.host->pi((uint64_t) p.y) - 1;
   HslQuery q;
   q.kind = LeafSum::S2_hard;

@@ -15,6 +15,7 @@ struct PixStats {
          seconds_ac = 0, seconds_b = 0, seconds_d = 0;
   uint64_t ac_leaves = 0;     // A and C terms summed
   uint64_t table_limit = 0;   // prime-count table covers [0, table_limit]
+  uint64_t table_windows = 0; // windows the table was built in (1 = whole)
 };
 
 /// One term with explicit parameters (src/gourdon/AC.cpp, B.cpp, Phi0.cpp,

@@ -28,6 +28,22 @@ __device__ __forceinline__ uint64_t pix_pi(const PixWord* __restrict__ tab, uint
   return 3 + t.cnt + (uint64_t) __popcll(t.bits & le_mask(r));
 }
 
+/// pi(n) from a window of the table that holds the words [w_lo, w_hi) (its
+/// counts are absolute); 0 for an n outside the window, whose lookup
+/// another window answers.
+__device__ __forceinline__ uint64_t pix_pi_win(const PixWord* __restrict__ tab, uint64_t n,
+                                               uint64_t w_lo, uint64_t w_hi)
+{
+  const uint64_t w = n / 240;
+  if (w - w_lo >= w_hi - w_lo)
+    return 0;
+  if (n < 7)
+    return n < 2 ? 0 : (n < 3 ? 1 : (n < 5 ? 2 : 3));
+  const uint32_t r = (uint32_t) (n - w * 240);
+  const PixWord t = tab[w - w_lo];
+  return 3 + t.cnt + (uint64_t) __popcll(t.bits & le_mask(r));
+}
+
 /// Adds a signed 128-bit value to a (lo, hi) pair with two 64-bit atomics; the
 /// add whose low word wraps carries into the high word.
 __device__ __forceinline__ void atomic_add_i128(unsigned long long* acc, i128 v)
@@ -87,20 +103,22 @@ __device__ __forceinline__ uint64_t isqrt64(uint64_t n)
 /// One workgroup sieves one segment of kPixSegWords words in LDS: the numbers
 /// coprime to 30 in [low, high) (high <= N + 1), without 1, then the multiples
 /// p * q (q >= p, q coprime to 30) of the primes p_4 = 7 .. p_{j_end - 1}.
+/// The table (or window of it) starts at word wbase: its word w stands for
+/// the numbers [240 (wbase + w), 240 (wbase + w) + 240).
 /// Small primes are crossed off by all threads together (8 residue classes
 /// of q, 32 threads per class), the others by one thread each.  Writes the
 /// bits and the segment's prime count.
 __global__ __launch_bounds__(kThreads) void pix_sieve_kernel(
     PixWord* __restrict__ tab, uint64_t* __restrict__ seg_tot,
     const uint32_t* __restrict__ primes, uint32_t j_small_end, uint32_t j_end,
-    uint64_t nwords, uint64_t N)
+    uint64_t nwords, uint64_t N, uint64_t wbase)
 {
   __shared__ uint32_t bm[2 * kPixSegWords];
   __shared__ unsigned long long red[kThreads];
   const uint32_t t = threadIdx.x;
   const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
   const uint32_t nw = (uint32_t) min<uint64_t>(kPixSegWords, nwords - w0);
-  const uint64_t low = w0 * 240;
+  const uint64_t low = (wbase + w0) * 240;
   const uint64_t high = min(low + 240ull * nw, N + 1);
   for (uint32_t w = t; w < nw; w += kThreads) {
     const uint64_t base = low + 240ull * w;
@@ -197,11 +215,12 @@ __global__ __launch_bounds__(1024) void pix_scan_kernel(uint64_t* __restrict__ s
   }
 }
 
-/// tab[w].cnt = primes >= 7 below 240 w: segment base plus the prefix of the
-/// word popcounts inside the segment (16 words per thread, one block scan).
+/// tab[w].cnt = primes >= 7 below the word: base (those below the table's
+/// first word) plus the segment base plus the prefix of the word popcounts
+/// inside the segment (16 words per thread, one block scan).
 __global__ __launch_bounds__(kThreads) void pix_count_kernel(PixWord* __restrict__ tab,
                                                              const uint64_t* __restrict__ seg_base,
-                                                             uint64_t nwords)
+                                                             uint64_t nwords, uint64_t base)
 {
   __shared__ unsigned long long part[kThreads];
   const uint32_t t = threadIdx.x;
@@ -221,7 +240,7 @@ __global__ __launch_bounds__(kThreads) void pix_count_kernel(PixWord* __restrict
     part[t] += v;
     __syncthreads();
   }
-  unsigned long long run = seg_base[blockIdx.x] + part[t] - s;
+  unsigned long long run = base + seg_base[blockIdx.x] + part[t] - s;
   for (uint32_t i = 0; i < per; i++) {
     if (a + i < nw) {
       const uint64_t b = tab[w0 + a + i].bits;
@@ -259,11 +278,12 @@ __device__ __forceinline__ uint64_t div49(uint64_t a, double ad, uint32_t d, dou
 /// descriptor of its first term by binary search and then steps forward by
 /// 64 terms.  A lane works on kGroup terms at a time so that their prime
 /// loads, and then their table loads, are in flight together; the masks of
-/// the table words sit in LDS.
+/// the table words sit in LDS.  tab holds the table words [w_lo, w_hi): a
+/// term whose quotient lies outside adds nothing (another window counts it).
 __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
     const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total, uint64_t per_wave,
-    const uint32_t* __restrict__ primes, const PixWord* __restrict__ tab,
-    unsigned long long* __restrict__ acc)
+    const uint32_t* __restrict__ primes, const PixWord* __restrict__ tab, uint64_t w_lo,
+    uint64_t w_hi, unsigned long long* __restrict__ acc)
 {
   constexpr uint32_t kGroup = 8;
   __shared__ uint64_t masks[240];
@@ -316,6 +336,7 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
       uint64_t w[kGroup];
       PixWord tw[kGroup];
       uint32_t rr[kGroup];
+      bool in[kGroup];
 #pragma unroll
       for (uint32_t s = 0; s < kGroup; s++) {
         const uint32_t qs = xp[s] ? q[s] : 1u;
@@ -324,8 +345,10 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
         // w = floor(v / 240): (v + 1/2) / 240 is at least 1/480 from an
         // integer, far more than the rounding error for v < 2^49
         const double wf = __builtin_trunc(__builtin_fma(vd, 1.0 / 240.0, 0.5 / 240.0));
-        w[s] = __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
-        rr[s] = (uint32_t) v - 240u * (uint32_t) w[s];
+        const uint64_t wv = __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
+        in[s] = wv - w_lo < w_hi - w_lo;
+        w[s] = in[s] ? wv - w_lo : 0;
+        rr[s] = (uint32_t) v - 240u * (uint32_t) wv;
         xp[s] = v;
       }
 #pragma unroll
@@ -337,8 +360,9 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
       for (uint32_t s = 0; s < kGroup; s++) {
         const uint64_t v = xp[s];
         // pi(v) for v < 7 from a nibble table: 0, 0, 1, 2, 2, 3, 3
-        const uint64_t pv = v < 7 ? (uint64_t) ((0x3322100u >> (4 * (uint32_t) v)) & 15)
-                                  : 3 + tw[s].cnt + (uint64_t) __popcll(tw[s].bits & masks[rr[s]]);
+        uint64_t pv = v < 7 ? (uint64_t) ((0x3322100u >> (4 * (uint32_t) v)) & 15)
+                            : 3 + tw[s].cnt + (uint64_t) __popcll(tw[s].bits & masks[rr[s]]);
+        pv = in[s] ? pv : 0;
         part += q[s] <= q1[s] ? pv : 2 * pv;
       }
       sum += part;
@@ -354,11 +378,13 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
 /// quotients, then the eight table loads, then the sums.  Each wave takes
 /// one contiguous range of items (group, run of kPixTileChunk tiles),
 /// finding its first group by binary search.  Lanes outside a descriptor's
-/// range read word 0 with the empty mask, which adds nothing.
+/// range, or whose quotient lies outside the window [w_lo, w_hi) of table
+/// words that tab holds, read its first word and add nothing.
 __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
     const PixTileDesc* __restrict__ desc, const PixTileGroup* __restrict__ grp, uint32_t ngrp,
     uint64_t nitems, uint64_t per_wave, const uint32_t* __restrict__ primes, uint32_t nprimes,
-    const PixWord* __restrict__ tab, unsigned long long* __restrict__ acc)
+    const PixWord* __restrict__ tab, uint64_t w_lo, uint64_t w_hi,
+    unsigned long long* __restrict__ acc)
 {
   constexpr uint32_t G = kPixTileGroup;
   __shared__ uint64_t masks[240];
@@ -439,7 +465,8 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
           const double wf = __builtin_trunc(__builtin_fma(vd[k], 1.0 / 240.0, 0.5 / 240.0));
           const uint64_t wv =
               __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
-          w[k] = ok[k] ? wv : 0;
+          ok[k] = ok[k] & (wv - w_lo < w_hi - w_lo);
+          w[k] = ok[k] ? wv - w_lo : 0;
           rr[k] = ok[k] ? v_lo[k] - 240u * (uint32_t) wv : 0u;
         }
         PixWord tw[G];
@@ -450,7 +477,7 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
 #pragma unroll
         for (uint32_t k = 0; k < G; k++) {
           const uint64_t pv = tw[k].cnt + (uint64_t) __popcll(tw[k].bits & masks[rr[k]]);
-          part += pv << sh[k];
+          part += ok[k] ? pv << sh[k] : 0;
         }
         sum += part;
       }
@@ -460,10 +487,14 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
 }
 
 /// B: sum of pi(x / p) over the primes y < p <= isqrt(x), read from the
-/// table's bits (one thread per word).
+/// bits of ptab, the table from word 0 (one thread per word); pi(x / p) from
+/// tab, the window [w_lo, w_hi) of table words (quotients outside add
+/// nothing).
 __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t x_hi, uint64_t y,
                                                          uint64_t sqrtx,
+                                                         const PixWord* __restrict__ ptab,
                                                          const PixWord* __restrict__ tab,
+                                                         uint64_t win_lo, uint64_t win_hi,
                                                          unsigned long long* __restrict__ acc)
 {
   const u128 x = ((u128) x_hi << 64) | x_lo;
@@ -472,13 +503,13 @@ __global__ __launch_bounds__(kThreads) void pix_B_kernel(uint64_t x_lo, uint64_t
   u128 sum = 0;
   for (uint64_t w = w_lo + (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
        w += (uint64_t) gridDim.x * kThreads) {
-    uint64_t v = tab[w].bits;
+    uint64_t v = ptab[w].bits;
     while (v) {
       const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
       v &= v - 1;
       const uint64_t p = 240 * w + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
       if (p > y && p <= sqrtx)
-        sum += pix_pi(tab, div_exact(xd, x_lo, p));
+        sum += pix_pi_win(tab, div_exact(xd, x_lo, p), win_lo, win_hi);
     }
   }
   block_add_i128(acc, (i128) sum);
@@ -741,11 +772,11 @@ uint32_t grid_for(uint64_t work, uint32_t cap)
 
 hipError_t pix_launch_sieve(PixWord* tab, uint64_t* seg_tot, const uint32_t* primes,
                             uint32_t j_small_end, uint32_t j_end, uint64_t nwords, uint64_t N,
-                            hipStream_t st)
+                            uint64_t wbase, hipStream_t st)
 {
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
   hipLaunchKernelGGL(pix_sieve_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, tab, seg_tot,
-                     primes, j_small_end, j_end, nwords, N);
+                     primes, j_small_end, j_end, nwords, N, wbase);
   return hipGetLastError();
 }
 
@@ -756,17 +787,17 @@ hipError_t pix_launch_scan(uint64_t* seg_tot, uint32_t nseg, hipStream_t st)
 }
 
 hipError_t pix_launch_counts(PixWord* tab, const uint64_t* seg_base, uint64_t nwords,
-                             hipStream_t st)
+                             uint64_t base, hipStream_t st)
 {
   const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
   hipLaunchKernelGGL(pix_count_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, tab, seg_base,
-                     nwords);
+                     nwords, base);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
-                              const uint32_t* primes, const PixWord* tab, unsigned long long* acc,
-                              uint32_t cus, hipStream_t st)
+                              const uint32_t* primes, const PixWord* tab, uint64_t w_lo,
+                              uint64_t w_hi, unsigned long long* acc, uint32_t cus, hipStream_t st)
 {
   if (total == 0 || ndesc == 0) return hipSuccess;
   // contiguous ranges of at least 4,096 terms, up to 64 waves per CU
@@ -775,17 +806,19 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
   const uint64_t per_wave = ((total + waves - 1) / waves + 63) / 64 * 64;
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
   hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
-                     per_wave, primes, tab, acc);
+                     per_wave, primes, tab, w_lo, w_hi, acc);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp, uint32_t ngrp,
                                 uint64_t nitems, const uint32_t* primes, uint32_t nprimes,
-                                const PixWord* tab, uint64_t max_quotient,
-                                unsigned long long* acc, uint32_t cus, hipStream_t st)
+                                const PixWord* tab, uint64_t w_lo, uint64_t w_hi,
+                                uint64_t max_quotient, unsigned long long* acc, uint32_t cus,
+                                hipStream_t st)
 {
   if (nitems == 0 || ngrp == 0) return hipSuccess;
-  // the quotient guard v 2^-46 stays far below 1/2 for quotients below 2^41
+  // the quotient guard v 2^-46 stays at most 1/8 for quotients below 2^43 (at
+  // most a quarter of the lanes take the residual fix-up)
   if (max_quotient >= kPixTileMaxQuotient) return hipErrorInvalidValue;
   // contiguous item ranges, up to 64 waves per CU
   const uint64_t max_waves = (uint64_t) cus * 64;
@@ -793,17 +826,18 @@ hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp
   const uint64_t waves = (nitems + per_wave - 1) / per_wave;
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
   hipLaunchKernelGGL(pix_leaftile_kernel, dim3(grid), dim3(kThreads), 0, st, desc, grp, ngrp,
-                     nitems, per_wave, primes, nprimes, tab, acc);
+                     nitems, per_wave, primes, nprimes, tab, w_lo, w_hi, acc);
   return hipGetLastError();
 }
 
 hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
-                        const PixWord* tab, unsigned long long* acc, hipStream_t st)
+                        const PixWord* ptab, const PixWord* tab, uint64_t w_lo, uint64_t w_hi,
+                        unsigned long long* acc, hipStream_t st)
 {
   if (sqrtx <= y) return hipSuccess;
   const uint64_t words = sqrtx / 240 - (y + 1) / 240 + 1;
   hipLaunchKernelGGL(pix_B_kernel, dim3(grid_for(words, 65536)), dim3(kThreads), 0, st, x_lo, x_hi,
-                     y, sqrtx, tab, acc);
+                     y, sqrtx, ptab, tab, w_lo, w_hi, acc);
   return hipGetLastError();
 }
 

@@ -59,27 +59,36 @@ static_assert(sizeof(PixTileDesc) == 32 && sizeof(PixTileGroup) == 32, "32-byte 
 
 constexpr uint32_t kPixTileGroup = 8;
 constexpr uint32_t kPixTileMaxPrime = 1u << 30;
-constexpr uint64_t kPixTileMaxQuotient = 1ull << 41;
+constexpr uint64_t kPixTileMaxQuotient = 1ull << 43;
 constexpr uint32_t kPixTileChunk = 16;
 
 constexpr uint32_t kPixSegWords = 4096;       // words per sieve segment (983,040 numbers)
 
+/// The table words [wbase, wbase + nwords) (numbers up to N) into tab[0 ..
+/// nwords): bits and per-segment totals; then the scan and the counts, whose
+/// base is the number of primes >= 7 below word wbase.
 hipError_t pix_launch_sieve(PixWord* tab, uint64_t* seg_tot, const uint32_t* primes,
                             uint32_t j_small_end, uint32_t j_end, uint64_t nwords, uint64_t N,
-                            hipStream_t st);
+                            uint64_t wbase, hipStream_t st);
 hipError_t pix_launch_scan(uint64_t* seg_tot, uint32_t nseg, hipStream_t st);
 hipError_t pix_launch_counts(PixWord* tab, const uint64_t* seg_base, uint64_t nwords,
-                             hipStream_t st);
+                             uint64_t base, hipStream_t st);
+/// The leaf sums and B read pi from a window of the table: tab holds the
+/// words [w_lo, w_hi) with absolute counts (the whole table: 0 and
+/// UINT64_MAX); terms whose quotient lies outside add nothing.
 hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t total,
-                              const uint32_t* primes, const PixWord* tab, unsigned long long* acc,
-                              uint32_t cus, hipStream_t st);
+                              const uint32_t* primes, const PixWord* tab, uint64_t w_lo,
+                              uint64_t w_hi, unsigned long long* acc, uint32_t cus, hipStream_t st);
 /// max_quotient bounds xp / primes[i] over all terms (the table limit).
 hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp, uint32_t ngrp,
                                 uint64_t nitems, const uint32_t* primes, uint32_t nprimes,
-                                const PixWord* tab, uint64_t max_quotient,
-                                unsigned long long* acc, uint32_t cus, hipStream_t st);
+                                const PixWord* tab, uint64_t w_lo, uint64_t w_hi,
+                                uint64_t max_quotient, unsigned long long* acc, uint32_t cus,
+                                hipStream_t st);
+/// B's primes come from the bits of ptab, the table from word 0 up to sqrtx.
 hipError_t pix_launch_B(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrtx,
-                        const PixWord* tab, unsigned long long* acc, hipStream_t st);
+                        const PixWord* ptab, const PixWord* tab, uint64_t w_lo, uint64_t w_hi,
+                        unsigned long long* acc, hipStream_t st);
 hipError_t pix_launch_sigma(uint64_t x_lo, uint64_t x_hi, uint64_t y, uint64_t sqrt_xy,
                             const uint32_t* primes, uint32_t b_lo, uint32_t b_hi,
                             const PixWord* tab, unsigned long long* acc, hipStream_t st);

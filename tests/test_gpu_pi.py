@@ -140,3 +140,38 @@ def test_pi_with_alpha_z(pc):
     finally:
         pc.set_alpha_z(0)
     assert pc.pi(10 ** 13) == known[10 ** 13]
+
+
+def test_pi_table_windows(pc):
+    # a prime-count table over the budget is built in windows (each with
+    # absolute counts, the leaf sums of A, C and B once per window): pi(x) by
+    # both algorithms, nth_prime and a batch of pi queries give the README
+    # and reference values
+    known = {int(r["x"]): int(r["pi"]) for r in golden("pi_known.json")["rows"]}
+    nth = [(int(r["n"]), int(r["p"])) for r in golden("nth_prime.json")["rows"]
+           if 10 ** 10 <= int(r["n"]) <= 10 ** 13][:4]
+    try:
+        for e, budget in ((13, 1 << 20), (15, 8 << 20), (17, 128 << 20), (19, 2 << 30)):
+            x = 10 ** e
+            pc.set_pi_table_bytes(budget)
+            log("pi(1e%d) with a %d-byte prime-count table budget" % (e, budget))
+            assert pc.pi(x) == known[x]
+            st = pc.last_pi_stats()
+            assert st["table_windows"] >= 2 and st["table_limit"] > budget // 16 * 240
+            assert pc.pi_deleglise_rivat(x) == known[x]
+        pc.set_pi_table_bytes(1 << 20)
+        log("nth_prime with a 1 MiB prime-count table budget")
+        for n, p in nth:
+            assert pc.nth_prime(n) == p, n
+        log("a batch of pi queries with a 1 MiB prime-count table budget")
+        xs = [10 ** 12, 10 ** 13, 10 ** 14, 10 ** 15]
+        with pc.Server() as sv:
+            ids = sv.submit([(pc.KIND_PI, x) for x in xs])
+            got = {r["id"]: r for r in sv.poll_all(len(ids), timeout_ms=120000)}
+        for i, x in zip(ids, xs):
+            assert got[i]["status"] == 0 and got[i]["value"] == known[x], x
+        pc.set_pi_table_bytes(0)
+        assert pc.pi(10 ** 15) == known[10 ** 15]
+        assert pc.last_pi_stats()["table_windows"] == 1
+    finally:
+        pc.set_pi_table_bytes(0)
