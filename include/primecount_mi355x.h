@@ -330,6 +330,15 @@ int pcg_P2(pcg_i128 x, int64_t y, pcg_i128* out);
  * -> src/nth_prime.cpp:95-145). */
 int pcg_nth_prime(int64_t n, int64_t* out);
 
+/* The n-th prime for n beyond the 64-bit range of answers (any n >= 1; up to
+ * n = 216289611853439384 it is pcg_nth_prime): pi at x0 = R^-1(n) on the
+ * device, then a sieve of the interval between x0 and the answer anchored at
+ * a 128-bit base, with 64-bit sieving primes.  Domain: the answer's x0
+ * inside pcg_pi's domain (y < 2^32: up to about 1e23).
+ * Replaces: pc_int128_t primecount_nth_prime_128(pc_int128_t n)
+ *           include/primecount.h:99, src/nth_prime.cpp:141-175. */
+int pcg_nth_prime_128(pcg_i128 n, pcg_i128* out);
+
 /* Plausibility check of a pi(x) value: PCG_OK if |pix - Li(x)| <
  * sqrt(x) log(x) / (8 pi) (Schoenfeld's bound; always for x < 2657), else
  * PCG_E_INTERNAL with a message.  pcg_pi and pcg_pi_deleglise_rivat run it on

@@ -119,6 +119,13 @@ inline int64_t nth_prime(int64_t n)
   return r;
 }
 
+inline int128_t nth_prime(int128_t n)
+{
+  pcg_i128 r;
+  detail::check(pcg_nth_prime_128(detail::to_abi(n), &r));
+  return detail::from_abi(r);
+}
+
 /* The other Gourdon terms (include/gourdon.hpp:24-27, 34-37) */
 
 #define PRIMECOUNT_MI355X_TERM(T, NAME, PARAMS, ARGS, OUT)                         \

@@ -180,6 +180,7 @@ def load():
         "pcg_check_pi": ([_I128, _I128], ctypes.c_int),
         "pcg_lds_read_gbs": ([ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "pcg_nth_prime": ([I64, P64], ctypes.c_int),
+        "pcg_nth_prime_128": ([_I128, PI128], ctypes.c_int),
         "pcg_AC": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
         "pcg_B": ([_I128, I64, PI128], ctypes.c_int),
         "pcg_Phi0": ([_I128, I64, I64, I64, PI128], ctypes.c_int),
@@ -314,10 +315,20 @@ def check_pi(x, pix):
 
 
 def nth_prime(n):
-    """The n-th prime (n <= 216289611853439384), on the device."""
+    """The n-th prime, on the device (n beyond 216289611853439384 through the
+    128-bit entry point, answers above 2^63)."""
+    if int(n) > 216289611853439384:
+        return nth_prime128(n)
     out = ctypes.c_int64()
     _check(load().pcg_nth_prime(int(n), ctypes.byref(out)))
     return out.value
+
+
+def nth_prime128(n):
+    """The n-th prime through the 128-bit entry point (any n >= 1)."""
+    out = _I128()
+    _check(load().pcg_nth_prime_128(_i128(n), ctypes.byref(out)))
+    return _from128(out)
 
 
 def last_pi_stats():

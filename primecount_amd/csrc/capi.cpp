@@ -212,22 +212,37 @@ int pcg_dr_params(pcg_i128 x, int64_t* y, int64_t* z, int64_t* c)
   });
 }
 
+static void set_pi_stats(const PixStats& st)
+{
+  t_pi_stats.seconds_total = st.seconds_total;
+  t_pi_stats.seconds_table = st.seconds_table;
+  t_pi_stats.seconds_sigma = st.seconds_sigma;
+  t_pi_stats.seconds_phi0 = st.seconds_phi0;
+  t_pi_stats.seconds_ac = st.seconds_ac;
+  t_pi_stats.seconds_b = st.seconds_b;
+  t_pi_stats.seconds_d = st.seconds_d;
+  t_pi_stats.ac_leaves = st.ac_leaves;
+  t_pi_stats.table_limit = st.table_limit;
+  t_pi_stats.table_windows = st.table_windows;
+}
+
+int pcg_nth_prime_128(pcg_i128 n, pcg_i128* out)
+{
+  return guarded([&] {
+    if (!out) throw pcg_error("out must not be NULL");
+    PixStats st;
+    *out = to_abi(nth_prime_128(from_abi(n), &st));
+    set_pi_stats(st);
+  });
+}
+
 int pcg_pi(pcg_i128 x, pcg_i128* out)
 {
   return guarded([&] {
     if (!out) throw pcg_error("out must not be NULL");
     PixStats st;
     *out = to_abi(pi_gourdon(from_abi(x), &st));
-    t_pi_stats.seconds_total = st.seconds_total;
-    t_pi_stats.seconds_table = st.seconds_table;
-    t_pi_stats.seconds_sigma = st.seconds_sigma;
-    t_pi_stats.seconds_phi0 = st.seconds_phi0;
-    t_pi_stats.seconds_ac = st.seconds_ac;
-    t_pi_stats.seconds_b = st.seconds_b;
-    t_pi_stats.seconds_d = st.seconds_d;
-    t_pi_stats.ac_leaves = st.ac_leaves;
-    t_pi_stats.table_limit = st.table_limit;
-    t_pi_stats.table_windows = st.table_windows;
+    set_pi_stats(st);
   });
 }
 
@@ -238,16 +253,7 @@ int pcg_nth_prime(int64_t n, int64_t* out)
     if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
     PixStats st;
     *out = (int64_t) nth_prime((uint64_t) n, &st);
-    t_pi_stats.seconds_total = st.seconds_total;
-    t_pi_stats.seconds_table = st.seconds_table;
-    t_pi_stats.seconds_sigma = st.seconds_sigma;
-    t_pi_stats.seconds_phi0 = st.seconds_phi0;
-    t_pi_stats.seconds_ac = st.seconds_ac;
-    t_pi_stats.seconds_b = st.seconds_b;
-    t_pi_stats.seconds_d = st.seconds_d;
-    t_pi_stats.ac_leaves = st.ac_leaves;
-    t_pi_stats.table_limit = st.table_limit;
-    t_pi_stats.table_windows = st.table_windows;
+    set_pi_stats(st);
   });
 }
 

@@ -656,6 +656,23 @@ uint64_t riemann_r_inverse(uint64_t n)
   return (uint64_t) llroundl(x);
 }
 
+/// riemann_r_inverse for n above the 64-bit range of answers (x up to about
+/// 1e23: the long double keeps x to about 1e4, far inside the sieve
+/// interval around it).
+u128 riemann_r_inverse_128(u128 n)
+{
+  const long double nd = (long double) n;
+  long double x = nd * logl(nd);
+  for (int it = 0; it < 100; it++) {
+    const long double nx = x - (riemann_r(x) - nd) * logl(x);
+    if (nx < 2) { x = 2; break; }
+    const bool done = fabsl(nx - x) < 0.5L + x * 1e-18L;
+    x = nx;
+    if (done) break;
+  }
+  return (u128) roundl(x);
+}
+
 } // namespace
 
 i128 gourdon_term(GourdonTerm term, i128 x, int64_t y, int64_t z, int64_t k, PixStats* st)
@@ -1231,13 +1248,29 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
     if (st) *st = s;
     return result;
   }
+  // answers from 2^62 on (n above about 1.1e17): the sieve interval anchored
+  // at a 128-bit base, one query at a time
+  {
+    std::vector<size_t> keep;
+    for (size_t i : big_i) {
+      if (riemann_r_inverse(ns[i]) < ((uint64_t) 1 << 62)) {
+        keep.push_back(i);
+      } else {
+        PixStats s1;
+        result[i] = (uint64_t) nth_prime_128((i128) ns[i], &s1);
+        s.table_limit = std::max(s.table_limit, s1.table_limit);
+      }
+    }
+    big_i.swap(keep);
+  }
+  if (big_i.empty()) {
+    s.seconds_total = now_s() - t0;
+    if (st) *st = s;
+    return result;
+  }
   // pi at the starting points of all queries in one batch
   std::vector<i128> x0s;
-  for (size_t i : big_i) {
-    const uint64_t x0 = riemann_r_inverse(ns[i]);
-    if (x0 >= ((uint64_t) 1 << 62)) throw pcg_error("nth_prime(n): n is too large");
-    x0s.push_back((i128) x0);
-  }
+  for (size_t i : big_i) x0s.push_back((i128) riemann_r_inverse(ns[i]));
   const std::vector<i128> cs = pi_gourdon_batch(x0s, &s, max_slots);
   int64_t ymax = 0;
   for (i128 x0 : x0s) ymax = std::max<int64_t>(ymax, gourdon_params(x0).z);
@@ -1330,6 +1363,99 @@ std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats*
 uint64_t nth_prime(uint64_t n, PixStats* st)
 {
   return nth_prime_batch({n}, st)[0];
+}
+
+i128 nth_prime_128(i128 n, PixStats* st)
+{
+  // src/nth_prime.cpp:141-175: the 64-bit path below 2^63, then pi at a
+  // close estimate and a sieve of the interval between it and the answer
+  const i128 max_n = (i128) 216289611853439384ull;
+  if (n < 1) throw pcg_error("nth_prime(n): n must be >= 1");
+  if (n <= max_n && riemann_r_inverse((uint64_t) n) < ((uint64_t) 1 << 62))
+    return (i128) nth_prime((uint64_t) n, st);
+  const double t0 = now_s();
+  const u128 x0 = riemann_r_inverse_128((u128) n);
+  pi_validate((i128) x0);
+  PixStats s;
+  const i128 c = pi_gourdon_batch({(i128) x0}, &s)[0];
+  const int64_t ymax = gourdon_params((i128) x0).z;
+  const bool down = c >= n;
+  const u128 k128 = down ? (u128) (c - n + 1) : (u128) (n - c);   // rank from x0 down / up
+  if (k128 >= ((u128) 1 << 50)) throw pcg_error("internal: nth_prime estimate", kErrInternal);
+  const uint64_t k = (uint64_t) k128;
+  const double lx = std::log((double) x0);
+  uint64_t delta = (uint64_t) ((double) (k + 16) * lx * 2.0) + 100000;
+  auto lock = engine_lock();
+  u128 res = 0;
+  for (;;) {
+    if (delta >= ((uint64_t) 1 << 61)) throw pcg_error("internal: nth_prime interval", kErrInternal);
+    const u128 lo = down ? x0 - delta : x0 + 1;
+    const u128 hi = down ? x0 : x0 + delta;
+    const u128 L = lo - lo % 240;                   // a multiple of 240 (and of 30)
+    const uint64_t o_lo = (uint64_t) (lo - L), o_hi = (uint64_t) (hi - L);
+    const uint64_t r = (uint64_t) isqrt(hi);
+    const EngineView v = engine_view(ymax, ymax, r + 1);
+    const std::vector<uint64_t> pr = pi_points(v, {r, 15359});
+    const uint64_t n_all = pr[0] >= 3 ? pr[0] - 3 : 0;       // sieving primes >= 7
+    const uint64_t n_small = std::min<uint64_t>(pr[1] - 3, n_all);
+    uint64_t* sp = (uint64_t*) engine_buffer(5, (n_all + 1) * 8);
+    if (n_all) PIX_HIP(pix_launch_extract64(v.pix_tab, r, sp, v.stream));
+    const uint64_t nwords = o_hi / 240 + 1;
+    const uint64_t nblk = (nwords + kPixSegWords - 1) / kPixSegWords;
+    uint64_t* bits = (uint64_t*) engine_buffer(6, nwords * 8);
+    uint64_t* cnts = (uint64_t*) engine_buffer(7, nblk * 8);
+    PIX_HIP(pix_launch_interval128(bits, (uint64_t) (L >> 64), (uint64_t) L, o_lo, o_hi, sp,
+                                   (uint32_t) n_small, n_all, v.stream));
+    PIX_HIP(pix_launch_block_counts(bits, nwords, cnts, v.stream));
+    std::vector<uint64_t> bc(nblk);
+    PIX_HIP(hipMemcpyAsync(bc.data(), cnts, nblk * 8, hipMemcpyDeviceToHost, v.stream));
+    PIX_HIP(hipStreamSynchronize(v.stream));
+    uint64_t total = 0;
+    for (uint64_t b : bc) total += b;
+    if (total < k) {
+      delta *= 2;
+      continue;
+    }
+    // the block, then the word and bit, of the k-th prime from the top
+    // (down) or the bottom of the interval
+    uint64_t need = k, blk = 0;
+    if (down) {
+      for (blk = nblk; blk-- > 0;) {
+        if (bc[blk] >= need) break;
+        need -= bc[blk];
+      }
+    } else {
+      for (blk = 0; blk < nblk; blk++) {
+        if (bc[blk] >= need) break;
+        need -= bc[blk];
+      }
+    }
+    const uint64_t w0 = blk * kPixSegWords;
+    const uint64_t nw = std::min<uint64_t>(kPixSegWords, nwords - w0);
+    std::vector<uint64_t> words(nw);
+    PIX_HIP(hipMemcpyAsync(words.data(), bits + w0, nw * 8, hipMemcpyDeviceToHost, v.stream));
+    PIX_HIP(hipStreamSynchronize(v.stream));
+    static const uint8_t wheel[8] = {1, 7, 11, 13, 17, 19, 23, 29};
+    for (uint64_t i = 0; i < nw && !res; i++) {
+      const uint64_t wi = down ? nw - 1 - i : i;
+      const uint64_t wv = words[wi];
+      const uint64_t c1 = (uint64_t) __builtin_popcountll(wv);
+      if (c1 < need) { need -= c1; continue; }
+      for (int jb = 0; jb < 64; jb++) {
+        const int bit = down ? 63 - jb : jb;
+        if (!((wv >> bit) & 1)) continue;
+        if (--need == 0) {
+          res = L + (u128) (240 * (w0 + wi) + 30 * (bit >> 3) + wheel[bit & 7]);
+          break;
+        }
+      }
+    }
+    if (!res) throw pcg_error("internal: nth_prime selection", kErrInternal);
+    break;
+  }
+  s.seconds_total = now_s() - t0;
+  if (st) *st = s;
+  return (i128) res;
 }
 
 } // namespace pcg

@@ -47,6 +47,12 @@ std::vector<i128> pi_gourdon_batch(const std::vector<i128>& xs, PixStats* st,
 /// pcg_error.
 uint64_t nth_prime(uint64_t n, PixStats* st);
 
+/// The n-th prime for any n >= 1 whose answer lies in pi(x)'s domain (x up
+/// to about 1e23): the 64-bit path up to n = 216289611853439384, beyond it
+/// pi at x0 = R^-1(n) and a sieve of the interval from x0 anchored at a
+/// 128-bit base (src/nth_prime.cpp:141-175).  Throws pcg_error.
+i128 nth_prime_128(i128 n, PixStats* st);
+
 /// nth_prime for many n (pi at the starting points as one pi_gourdon_batch).
 std::vector<uint64_t> nth_prime_batch(const std::vector<uint64_t>& ns, PixStats* st,
                                       uint64_t max_slots = 0);

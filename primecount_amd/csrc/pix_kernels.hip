@@ -713,6 +713,134 @@ __global__ __launch_bounds__(kThreads) void pix_interval_large_kernel(
   }
 }
 
+/// The primes 7 <= p <= p_max as 64-bit values (sieving primes above 2^32).
+__global__ __launch_bounds__(kThreads) void pix_extract64_kernel(const PixWord* __restrict__ tab,
+                                                                 uint64_t p_max,
+                                                                 uint64_t* __restrict__ out)
+{
+  const uint64_t w_hi = p_max / 240;
+  for (uint64_t w = (uint64_t) blockIdx.x * kThreads + threadIdx.x; w <= w_hi;
+       w += (uint64_t) gridDim.x * kThreads) {
+    uint64_t v = tab[w].bits;
+    uint64_t pos = tab[w].cnt;
+    while (v) {
+      const uint32_t bit = (uint32_t) __ffsll((long long) v) - 1;
+      v &= v - 1;
+      const uint64_t p = 240 * w + 30 * (bit >> 3) + ((kWheelBytes >> (8 * (bit & 7))) & 0xff);
+      if (p <= p_max)
+        out[pos] = p;
+      pos++;
+    }
+  }
+}
+
+/// (L_hi 2^64 + L_lo) mod p for L_hi < 2^20 and p < 2^42: L_hi (2^64 mod p)
+/// stays below 2^62.
+__device__ __forceinline__ uint64_t mod128_small(uint64_t L_hi, uint64_t L_lo, uint64_t p)
+{
+  const uint64_t c = (~0ull % p + 1) % p;            // 2^64 mod p
+  return (L_hi * c + L_lo % p) % p;
+}
+
+/// q^-1 mod 30 for q coprime to 30 (q mod 30 in {1, 7, 11, 13, 17, 19, 23, 29}).
+__device__ __forceinline__ uint32_t inv_mod30(uint32_t q)
+{
+  switch (q) {
+    case 7: return 13;
+    case 11: return 11;
+    case 13: return 7;
+    case 17: return 23;
+    case 19: return 19;
+    case 23: return 17;
+    case 29: return 29;
+    default: return 1;
+  }
+}
+
+/// Sieve of the interval [L + o_lo, L + o_hi] for a base L above 2^63 (a
+/// multiple of 240, L = L_hi 2^64 + L_lo): word w of `out` stands for the
+/// numbers [L + 240 w, L + 240 w + 240).  One workgroup per kPixSegWords
+/// words in LDS crosses off every multiple of the sieving primes
+/// sp[0 .. n_small): the interval lies far above their squares, so every
+/// multiple in it is composite and p itself is not in it.  Thread class c
+/// (8 per prime) takes the multiples p q with p q = 30 k + r_c (the wheel
+/// residue of bit c): q = r_c p^-1 (mod 30), one multiple every 30 p.
+__global__ __launch_bounds__(kThreads) void pix_interval128_kernel(
+    uint64_t* __restrict__ out, uint64_t L_hi, uint64_t L_lo, uint64_t o_lo, uint64_t o_hi,
+    uint64_t nwords, const uint64_t* __restrict__ sp, uint32_t n_small)
+{
+  __shared__ uint32_t bm[2 * kPixSegWords];
+  const uint32_t t = threadIdx.x;
+  const uint64_t w0 = (uint64_t) blockIdx.x * kPixSegWords;
+  const uint32_t nw = (uint32_t) min<uint64_t>(kPixSegWords, nwords - w0);
+  const uint64_t s = 240 * w0;                             // block offset from L
+  for (uint32_t w = t; w < nw; w += kThreads) {
+    const uint64_t base = s + 240ull * w;
+    uint64_t v = ~0ull;
+    if (base + 239 > o_hi)
+      v = base > o_hi ? 0ull : le_mask((uint32_t) (o_hi - base));
+    if (base < o_lo)
+      v &= o_lo - base >= 240 ? 0ull : ~(o_lo - base == 0 ? 0ull : le_mask((uint32_t) (o_lo - base - 1)));
+    bm[2 * w] = (uint32_t) v;
+    bm[2 * w + 1] = (uint32_t) (v >> 32);
+  }
+  __syncthreads();
+  const uint64_t seg_len = 240ull * nw;
+  const uint32_t cls = t & 7;
+  const uint32_t t0 = t >> 3;
+  const uint32_t wr = (uint32_t) ((kWheelBytes >> (8 * cls)) & 0xff);
+  for (uint32_t j = 0; j < n_small; j++) {
+    const uint64_t p = sp[j];
+    // first multiple at or after L + s: offset d from the block start
+    const uint64_t r = (mod128_small(L_hi, L_lo, p) + s % p) % p;
+    const uint64_t d = r ? p - r : 0;
+    // its cofactor mod 30 (L + s is a multiple of 30), then the class's first
+    const uint32_t pinv = inv_mod30((uint32_t) (p % 30));
+    const uint32_t q0m = (uint32_t) (d % 30) * pinv % 30;
+    const uint32_t qc = wr * pinv % 30;
+    const uint64_t off = d + (uint64_t) ((qc + 30 - q0m) % 30) * p;
+    if (off >= seg_len)
+      continue;
+    const uint32_t n = (uint32_t) ((seg_len - 1 - off) / (30 * p)) + 1;
+    const uint32_t byte0 = (uint32_t) (off / 30);
+    for (uint32_t k = t0; k < n; k += kThreads / 8) {
+      const uint32_t B = byte0 + k * (uint32_t) p;
+      atomicAnd(&bm[B >> 2], ~(1u << (((B & 3) << 3) | cls)));
+    }
+  }
+  __syncthreads();
+  for (uint32_t w = t; w < nw; w += kThreads)
+    out[w0 + w] = ((uint64_t) bm[2 * w + 1] << 32) | bm[2 * w];
+}
+
+/// The sieving primes sp[j_lo .. j_hi) on the interval of
+/// pix_interval128_kernel, one thread per prime, with 64-bit atomics on the
+/// interval's words in HBM: the multiples p q with q coprime to 30.
+__global__ __launch_bounds__(kThreads) void pix_interval128_large_kernel(
+    uint64_t* __restrict__ out, uint64_t L_hi, uint64_t L_lo, uint64_t o_lo, uint64_t o_hi,
+    const uint64_t* __restrict__ sp, uint64_t j_lo, uint64_t j_hi)
+{
+  for (uint64_t j = j_lo + (uint64_t) blockIdx.x * kThreads + threadIdx.x; j < j_hi;
+       j += (uint64_t) gridDim.x * kThreads) {
+    const uint64_t p = sp[j];
+    const uint64_t r = (mod128_small(L_hi, L_lo, p) + o_lo % p) % p;
+    const uint64_t off = o_lo + (r ? p - r : 0);           // first multiple at or after L + o_lo
+    if (off > o_hi)
+      continue;
+    const uint32_t qm = (uint32_t) (off % 30) * inv_mod30((uint32_t) (p % 30)) % 30;
+    for (uint32_t idx = qm ? le_count(qm - 1) : 0u;; idx++) {
+      const uint64_t dq = 30ull * (idx >> 3) + ((kWheelBytes >> (8 * (idx & 7))) & 0xff) - qm;
+      const uint64_t o = off + dq * p;
+      if (o > o_hi)
+        break;
+      const uint64_t w = o / 240;
+      const uint32_t r240 = (uint32_t) (o - w * 240);
+      const uint32_t b = 8 * (r240 / 30) + wheel_bit(r240 % 30);
+      atomicAnd((unsigned long long*) &out[w], ~(1ull << b));
+    }
+  }
+}
+
 /// Primes per block of kPixSegWords words of the interval bitmap.
 __global__ __launch_bounds__(kThreads) void pix_block_count_kernel(const uint64_t* __restrict__ bits,
                                                                    uint64_t nwords,
@@ -891,6 +1019,31 @@ hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const ui
   if (e != hipSuccess || n_all <= n_small) return e;
   hipLaunchKernelGGL(pix_interval_large_kernel, dim3(grid_for(n_all - n_small, 65536)),
                      dim3(kThreads), 0, st, out, lo, hi, sp, (uint64_t) n_small, n_all);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_extract64(const PixWord* tab, uint64_t p_max, uint64_t* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(pix_extract64_kernel, dim3(grid_for(p_max / 240 + 1, 65536)), dim3(kThreads), 0,
+                     st, tab, p_max, out);
+  return hipGetLastError();
+}
+
+hipError_t pix_launch_interval128(uint64_t* out, uint64_t L_hi, uint64_t L_lo, uint64_t o_lo,
+                                  uint64_t o_hi, const uint64_t* sp, uint32_t n_small,
+                                  uint64_t n_all, hipStream_t st)
+{
+  // (L itself is a multiple of 240; its low word need not be when L >= 2^64)
+  if (L_hi >= (1ull << 20) || o_lo > o_hi || o_hi >= (1ull << 62))
+    return hipErrorInvalidValue;
+  const uint64_t nwords = o_hi / 240 + 1;
+  const uint64_t nseg = (nwords + kPixSegWords - 1) / kPixSegWords;
+  hipLaunchKernelGGL(pix_interval128_kernel, dim3((uint32_t) nseg), dim3(kThreads), 0, st, out,
+                     L_hi, L_lo, o_lo, o_hi, nwords, sp, n_small);
+  if (n_all > n_small)
+    hipLaunchKernelGGL(pix_interval128_large_kernel, dim3(grid_for(n_all - n_small, 65536)),
+                       dim3(kThreads), 0, st, out, L_hi, L_lo, o_lo, o_hi, sp, (uint64_t) n_small,
+                       n_all);
   return hipGetLastError();
 }
 

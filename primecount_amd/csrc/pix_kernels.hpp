@@ -101,6 +101,15 @@ hipError_t pix_launch_phi0(uint64_t x_lo, uint64_t x_hi, const void* factor, boo
 hipError_t pix_launch_extract(const PixWord* tab, uint64_t p_max, uint32_t* out, hipStream_t st);
 hipError_t pix_launch_interval(uint64_t* out, uint64_t lo, uint64_t hi, const uint32_t* sp,
                                uint32_t n_small, uint64_t n_all, hipStream_t st);
+/// The primes 7 <= p <= p_max as 64-bit values.
+hipError_t pix_launch_extract64(const PixWord* tab, uint64_t p_max, uint64_t* out, hipStream_t st);
+/// Sieve of [L + o_lo, L + o_hi] for L = L_hi 2^64 + L_lo above 2^63, a
+/// multiple of 240 (L_hi < 2^20, o_hi < 2^62): word w of out holds the
+/// numbers [L + 240 w, L + 240 w + 240); sp are the sieving primes from 7
+/// up to sqrt(L + o_hi), the first n_small sieved in LDS.
+hipError_t pix_launch_interval128(uint64_t* out, uint64_t L_hi, uint64_t L_lo, uint64_t o_lo,
+                                  uint64_t o_hi, const uint64_t* sp, uint32_t n_small,
+                                  uint64_t n_all, hipStream_t st);
 hipError_t pix_launch_block_counts(const uint64_t* bits, uint64_t nwords, uint64_t* counts,
                                    hipStream_t st);
 hipError_t pix_launch_select(const PixWord* tab, uint64_t nwords, uint64_t n, uint64_t* out,

@@ -57,7 +57,12 @@ int main()
   expect("pi_deleglise_rivat(1e15)", pm::pi_deleglise_rivat((int64_t) 1000000000000000ll),
          29844570422669ll);
   // nth_prime (values printed by the reference build)
-  expect("nth_prime(1e9)", pm::nth_prime(1000000000ll), 22801763489ll);
+  expect("nth_prime(1e9)", pm::nth_prime((int64_t) 1000000000ll), 22801763489ll);
+  // the first prime above 2^63 through the 128-bit overload (pi(p) = n and
+  // pi(p - 1) = n - 1 pin it; n = pi(2^63 - 1) + 1)
+  expect("nth_prime(int128 216289611853439385)",
+         pm::nth_prime((pm::int128_t) 216289611853439385ll),
+         ((pm::int128_t) 1 << 63) + 29);
   // an argument outside the domain throws primecount_error, and the engine
   // keeps working afterwards
   bool threw = false;

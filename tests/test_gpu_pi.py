@@ -175,3 +175,30 @@ def test_pi_table_windows(pc):
         assert pc.last_pi_stats()["table_windows"] == 1
     finally:
         pc.set_pi_table_bytes(0)
+
+
+def test_nth_prime_128(pc):
+    # answers from 2^62 on: the sieve interval anchored at a 128-bit base
+    # with 64-bit sieving primes.  Each answer p is pinned by pi(p) = n and
+    # pi(p - 1) = n - 1 (pi itself is pinned by the README values); at the
+    # 64-bit boundary n = pi(2^63 - 1) the answers are the largest prime
+    # below 2^63 and the smallest above it.
+    max_n = 216289611853439384
+    cases = [2 * 10 ** 17, max_n, max_n + 1, 10 ** 18 + 7, 2 * 10 ** 18 + 123457]
+    for n in cases:
+        log("nth_prime(%d) through the 128-bit entry point" % n)
+        p = pc.nth_prime128(n)
+        assert pc.pi(p) == n and pc.pi(p - 1) == n - 1, (n, p)
+        if n <= max_n:
+            assert p < 2 ** 63 and pc.nth_prime(n) == p
+        else:
+            assert p > 2 ** 63 and pc.nth_prime(n) == p
+    assert pc.nth_prime128(max_n) == 2 ** 63 - 25
+    assert pc.nth_prime128(max_n + 1) == 2 ** 63 + 29
+    # the 128-bit entry point gives the 64-bit answers below
+    rows = [(int(r["n"]), int(r["p"])) for r in golden("nth_prime.json")["rows"]]
+    for n, p in [r for r in rows if 10 ** 6 <= r[0] <= 10 ** 15][:4] + [(1, 2), (169, 1009)]:
+        assert pc.nth_prime128(n) == p, n
+    for bad in (0, -5):
+        with pytest.raises(pc.PrimecountError):
+            pc.nth_prime128(bad)
