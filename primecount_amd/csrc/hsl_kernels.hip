@@ -139,6 +139,19 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t lane)
   return ((uint64_t) hi << 32) | lo;
 }
 
+/// A wave-uniform double moved to scalar registers (the compiler keeps the
+/// result of a vector FP64 operation in vector registers and folds the
+/// readfirstlane builtin on a value it knows to be uniform, so the move is
+/// written as the instruction itself; no memory access).
+__device__ __forceinline__ double uniform_f64(double v)
+{
+  const uint64_t b = __builtin_bit_cast(uint64_t, v);
+  uint32_t lo, hi;
+  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(lo) : "v"((uint32_t) b));
+  asm volatile("v_readfirstlane_b32 %0, %1" : "=s"(hi) : "v"((uint32_t) (b >> 32)));
+  return __builtin_bit_cast(double, ((uint64_t) hi << 32) | lo);
+}
+
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, uint32_t lane)
 {
   return (uint32_t) __builtin_amdgcn_readlane((int) v, (int) lane);
@@ -554,9 +567,10 @@ __device__ __forceinline__ void sieve_segments(
     const uint64_t high = min(low + SEG, qd.limit);
     const uint32_t seg_len = (uint32_t) (high - low);
     const uint64_t low1 = low > 0 ? low : 1;
-    // reciprocals of the segment bounds for the leaf-range quotients
-    const double inv_low1 = 1.0 / (double) low1;
-    const double inv_high = 1.0 / (double) high;
+    // reciprocals of the segment bounds for the leaf-range quotients, in
+    // scalar registers
+    const double inv_low1 = uniform_f64(1.0 / (double) low1);
+    const double inv_high = uniform_f64(1.0 / (double) high);
     const double low_d = (double) low, last_d = (double) (high - 1);   // exact: < 2^53
 
     // numbers coprime to 30 in [low, high)
