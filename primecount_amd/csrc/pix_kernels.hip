@@ -278,8 +278,10 @@ __device__ __forceinline__ uint64_t div49(uint64_t a, double ad, uint32_t d, dou
 /// descriptor of its first term by binary search and then steps forward by
 /// 64 terms.  A lane works on kGroup terms at a time so that their prime
 /// loads, and then their table loads, are in flight together; the masks of
-/// the table words sit in LDS.  tab holds the table words [w_lo, w_hi): a
-/// term whose quotient lies outside adds nothing (another window counts it).
+/// the table words sit in LDS.  With WIN, tab holds the table words
+/// [w_lo, w_hi): a term whose quotient lies outside adds nothing (another
+/// window counts it); without, tab is the whole table.
+template <bool WIN>
 __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
     const PixLeafDesc* __restrict__ desc, uint32_t ndesc, uint64_t total, uint64_t per_wave,
     const uint32_t* __restrict__ primes, const PixWord* __restrict__ tab, uint64_t w_lo,
@@ -346,8 +348,8 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
         // integer, far more than the rounding error for v < 2^49
         const double wf = __builtin_trunc(__builtin_fma(vd, 1.0 / 240.0, 0.5 / 240.0));
         const uint64_t wv = __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
-        in[s] = wv - w_lo < w_hi - w_lo;
-        w[s] = in[s] ? wv - w_lo : 0;
+        in[s] = !WIN || wv - w_lo < w_hi - w_lo;
+        w[s] = !WIN ? wv : (in[s] ? wv - w_lo : 0);
         rr[s] = (uint32_t) v - 240u * (uint32_t) wv;
         xp[s] = v;
       }
@@ -378,8 +380,9 @@ __global__ __launch_bounds__(kThreads) void pix_leafsum_kernel(
 /// quotients, then the eight table loads, then the sums.  Each wave takes
 /// one contiguous range of items (group, run of kPixTileChunk tiles),
 /// finding its first group by binary search.  Lanes outside a descriptor's
-/// range, or whose quotient lies outside the window [w_lo, w_hi) of table
-/// words that tab holds, read its first word and add nothing.
+/// range, or (WIN) whose quotient lies outside the window [w_lo, w_hi) of
+/// table words that tab holds, read its first word and add nothing.
+template <bool WIN>
 __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
     const PixTileDesc* __restrict__ desc, const PixTileGroup* __restrict__ grp, uint32_t ngrp,
     uint64_t nitems, uint64_t per_wave, const uint32_t* __restrict__ primes, uint32_t nprimes,
@@ -465,8 +468,9 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
           const double wf = __builtin_trunc(__builtin_fma(vd[k], 1.0 / 240.0, 0.5 / 240.0));
           const uint64_t wv =
               __builtin_bit_cast(uint64_t, wf + 4503599627370496.0) & ((1ull << 52) - 1);
-          ok[k] = ok[k] & (wv - w_lo < w_hi - w_lo);
-          w[k] = ok[k] ? wv - w_lo : 0;
+          if constexpr (WIN)
+            ok[k] = ok[k] & (wv - w_lo < w_hi - w_lo);
+          w[k] = ok[k] ? (WIN ? wv - w_lo : wv) : 0;
           rr[k] = ok[k] ? v_lo[k] - 240u * (uint32_t) wv : 0u;
         }
         PixWord tw[G];
@@ -477,7 +481,8 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
 #pragma unroll
         for (uint32_t k = 0; k < G; k++) {
           const uint64_t pv = tw[k].cnt + (uint64_t) __popcll(tw[k].bits & masks[rr[k]]);
-          part += ok[k] ? pv << sh[k] : 0;
+          // without windows a lane out of range read word 0 with the empty mask: 0
+          part += (!WIN || ok[k]) ? pv << sh[k] : 0;
         }
         sum += part;
       }
@@ -933,7 +938,9 @@ hipError_t pix_launch_leafsum(const PixLeafDesc* desc, uint32_t ndesc, uint64_t 
   const uint64_t waves = std::min<uint64_t>((total + 4095) / 4096, max_waves);
   const uint64_t per_wave = ((total + waves - 1) / waves + 63) / 64 * 64;
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
-  hipLaunchKernelGGL(pix_leafsum_kernel, dim3(grid), dim3(kThreads), 0, st, desc, ndesc, total,
+  const bool win = w_lo != 0 || w_hi != ~0ull;
+  hipLaunchKernelGGL(win ? pix_leafsum_kernel<true> : pix_leafsum_kernel<false>, dim3(grid),
+                     dim3(kThreads), 0, st, desc, ndesc, total,
                      per_wave, primes, tab, w_lo, w_hi, acc);
   return hipGetLastError();
 }
@@ -953,7 +960,9 @@ hipError_t pix_launch_leaftiles(const PixTileDesc* desc, const PixTileGroup* grp
   const uint64_t per_wave = (nitems + max_waves - 1) / max_waves;
   const uint64_t waves = (nitems + per_wave - 1) / per_wave;
   const uint32_t grid = (uint32_t) ((waves + kThreads / 64 - 1) / (kThreads / 64));
-  hipLaunchKernelGGL(pix_leaftile_kernel, dim3(grid), dim3(kThreads), 0, st, desc, grp, ngrp,
+  const bool win = w_lo != 0 || w_hi != ~0ull;
+  hipLaunchKernelGGL(win ? pix_leaftile_kernel<true> : pix_leaftile_kernel<false>, dim3(grid),
+                     dim3(kThreads), 0, st, desc, grp, ngrp,
                      nitems, per_wave, primes, nprimes, tab, w_lo, w_hi, acc);
   return hipGetLastError();
 }
