@@ -8,6 +8,7 @@
 #include "wheel_dev.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace pcg {
 
@@ -418,9 +419,21 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
       const PixTileGroup gr = grp[g];
       const uint32_t ta = gr.t_lo + (uint32_t) (j - gr.item0) * kPixTileChunk;
       const uint32_t tb = gr.t_hi + 1 < ta + kPixTileChunk ? gr.t_hi + 1 : ta + kPixTileChunk;
-      for (uint32_t t = ta; t < tb; t++) {
+      // the tiles whose 64 primes lie in every descriptor's range (most of
+      // them) take a body without the per-lane range masks
+      uint32_t in_lo = 0, in_hi = 0xFFFFFFFFu;
+#pragma unroll
+      for (uint32_t k = 0; k < G; k++) {
+        const PixTileDesc D = desc[gr.d0 + k];
+        in_lo = max(in_lo, D.i_lo);
+        in_hi = min(in_hi, D.i_hi);
+      }
+      in_hi = min(in_hi, nprimes - 1);
+      // one tile: the sum of its leaves' w * (pi - 3) over the group
+      auto tile = [&](auto full_c, uint32_t t) -> uint64_t {
+        constexpr bool FULL = decltype(full_c)::value;
         const uint32_t idx = 64 * t + lane;
-        const uint32_t q0 = primes[idx < nprimes ? idx : nprimes - 1];
+        const uint32_t q0 = primes[FULL || idx < nprimes ? idx : nprimes - 1];
         const uint32_t q = q0 > 1 ? q0 : 1u;
         const double qd = (double) q;
         double r = __builtin_amdgcn_rcp(qd);
@@ -434,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
         for (uint32_t k = 0; k < G; k++) {
           // groups are padded to G descriptors with empty ranges
           const PixTileDesc D = desc[gr.d0 + k];
-          ok[k] = (idx >= D.i_lo) & (idx <= D.i_hi);
+          ok[k] = FULL || ((idx >= D.i_lo) & (idx <= D.i_hi));
           sh[k] = q > D.q1 ? 1u : 0u;
           // quotient xp / q.  xd * r is within v 2^-50 of xp / q, so when
           // its fraction lies in [v 2^-46, 1 - v 2^-46] its integer part is
@@ -484,6 +497,14 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
           // without windows a lane out of range read word 0 with the empty mask: 0
           part += (!WIN || ok[k]) ? pv << sh[k] : 0;
         }
+        return part;
+      };
+      for (uint32_t t = ta; t < tb; t++) {
+        uint64_t part;
+        if (64 * t >= in_lo && 64 * t + 63 <= in_hi)
+          part = tile(std::true_type{}, t);
+        else
+          part = tile(std::false_type{}, t);
         sum += part;
       }
     }
