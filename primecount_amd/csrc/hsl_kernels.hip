@@ -139,6 +139,16 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, uint32_t lane)
   return ((uint64_t) hi << 32) | lo;
 }
 
+/// Calls f(std::integral_constant<int, i>{}) for i = 0 .. N - 1, in order.
+template <int N, typename F>
+__device__ __forceinline__ void static_for(F&& f)
+{
+  if constexpr (N > 0) {
+    static_for<N - 1>(f);
+    f(std::integral_constant<int, N - 1>{});
+  }
+}
+
 /// A wave-uniform double moved to scalar registers (the compiler keeps the
 /// result of a vector FP64 operation in vector registers and folds the
 /// readfirstlane builtin on a value it knows to be uniform, so the move is
@@ -445,7 +455,9 @@ __device__ __forceinline__ void sieve_segments(
     uint32_t nitems, uint32_t W, uint32_t cbits, uint32_t bcap)
 {
   extern __shared__ __align__(16) uint32_t lds[];
-  constexpr bool BATCH = NQ == 1;                             // batched leaf counting
+  // batched leaf counting; the kernel for groups of 2 keeps its 88 VGPRs
+  // (5 waves per SIMD) without it: with it, 98 and 4 waves, 2% slower
+  constexpr bool BATCH = NQ != 2;
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
   uint8_t* lut = reinterpret_cast<uint8_t*>(pre2 + W / 2);    // 240 u8
@@ -916,7 +928,13 @@ __device__ __forceinline__ void sieve_segments(
       // bcap hits (records), which bounds blist and the correction work.
       bool bat = false;
       uint32_t bat_from = 64;
-      uint32_t s_ent = 0;                 // per lane: factor entries of the leaf primes <= it
+      // per lane: factor entries of the member's leaf primes <= the lane's
+      // prime (s_ent), and of all members at and up to it (e_tot, s_tot)
+      uint32_t s_ent[NQ];
+      uint32_t e_tot = 0, s_tot = 0;
+#pragma unroll
+      for (int k = 0; k < NQ; k++)
+        s_ent[k] = 0;
       if constexpr (BATCH) {
         bat_from = max(seq_end, d_win);
         const unsigned long long cand = bat_from < 64 ? lmask & (~0ull << bat_from) : 0ull;
@@ -924,22 +942,35 @@ __device__ __forceinline__ void sieve_segments(
           // no prime from bat_from on has a window prime among its cofactors
           const uint32_t pf = readlane_u32(r_p, 0), pl = readlane_u32(r_p, nwin - 1);
           bat = __ballot(lane >= bat_from && lane < nwin && cq_lo <= pl && cq_hi >= pf) == 0;
-          if (bat)
-            s_ent = wave_incl_scan(((lmask >> lane) & 1) ? r_hi[0] - r_lo[0] : 0u);
+          if (bat) {
+#pragma unroll
+            for (int k = 0; k < NQ; k++) {
+              const uint32_t ek = ((lmask_m[k] >> lane) & 1) ? r_hi[k] - r_lo[k] : 0u;
+              if (NQ > 1)
+                s_ent[k] = wave_incl_scan(ek);
+              e_tot += ek;
+            }
+            s_tot = wave_incl_scan(e_tot);
+            if (NQ == 1)
+              s_ent[0] = s_tot;
+          }
         }
       }
       // the last prime with leaves that can join a batch starting at j: the
       // batch's primes with leaves have at most kBatchEntries factor entries
-      // (or prime leaves), its recorded primes [j, jb) at most bcap hits
+      // (or prime leaves) over all members, its recorded primes [j, jb) at
+      // most bcap hits
       auto batch_end = [&](uint32_t j) -> uint32_t {
-        const uint32_t be = readlane_u32(s_ent, j) - (readlane_u32(r_hi[0], j) - readlane_u32(r_lo[0], j));
+        const uint32_t ej = NQ == 1 ? readlane_u32(r_hi[0], j) - readlane_u32(r_lo[0], j)
+                                    : readlane_u32(e_tot, j);
+        const uint32_t be = readlane_u32(s_tot, j) - ej;
         const uint32_t bh = readlane_u32(h_scan, j) - readlane_u32(h_win, j);
-        const bool ok = ((lmask >> lane) & 1) && lane >= j && s_ent - be <= kBatchEntries &&
+        const bool ok = ((lmask >> lane) & 1) && lane >= j && s_tot - be <= kBatchEntries &&
                         h_scan - h_win - bh <= bcap;
         const unsigned long long bal = __ballot(ok);
         return bal ? 63u - (uint32_t) __clzll((long long) bal) : j;
       };
-      // Counts the leaves of the primes with leaves in [j, jb] (NQ == 1).
+      // Counts the leaves of the primes with leaves in [j, jb].
       auto run_batch = [&](uint32_t j, uint32_t jb, uint32_t r0) {
         if (r0 < bat_from) {
           flush(r0, bat_from);
@@ -965,122 +996,137 @@ __device__ __forceinline__ void sieve_segments(
           u_pbuilds++;
         }
         phase(kHslPhaseBatchEntry);
-        const uint32_t xlo_lo = (uint32_t) r_xlo[0], xlo_hi = (uint32_t) (r_xlo[0] >> 32);
-        uint32_t qh = 0, qt = 0;                   // queue head and tail (mod 128)
-        auto drain_b = [&](uint32_t n) {
-          phase(kHslPhaseBatchDrain);
-          wave_lds_sync();
-          uint32_t e = 0, jl = 0;
-          if (lane < n) {
-            e = lq[(qt + lane) & 127];
-            jl = lqb[(qt + lane) & 127];
-          }
-          // the leaf's prime: x / p and its first record, from its lane
-          const int src = (int) (jl << 2);
-          const uint32_t xl = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_lo);
-          const uint32_t xh = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_hi);
-          const uint32_t st = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) foff);
-          uint32_t bound = 0;
-          int cnt = 0;
-          if (lane < n) {
-            const uint32_t m = (e << 1) | 1u;
-            cnt = (int) seg_count_b(bm, pre2, lut, leaf_stop(((uint64_t) xh << 32) | xl, low, m),
-                                    bound);
-          }
-          // lane 0 holds the earliest prime of the step (the queue is in
-          // prime order), so records before its first one count for no lane
-          const uint32_t e0 = (uint32_t) __builtin_amdgcn_readfirstlane((int) st);
-          for (uint32_t cb = e0; cb < R; cb += kBlock) {
-            const uint32_t ix = cb + lane;
-            const uint32_t rv = ix < R ? (uint32_t) blist[ix] : 0u;
-            // wave-uniform trip count, kept in a scalar register
-            const uint32_t ne = (uint32_t) __builtin_amdgcn_readfirstlane(
-                (int) min((uint32_t) kBlock, R - cb));
-            for (uint32_t i = 0; i < ne; i++) {
-              const uint32_t v = readlane_u32(rv, i);
-              cnt += (cb + i >= st && v < bound) ? 1 : 0;
+        // member by member (their leaves are counted against the same
+        // prefix build and records): each member's leaves go through the
+        // queue in prime order and are drained before the next member's
+        const unsigned long long in_bat = (~0ull << j) & (jb >= 63 ? ~0ull : ((2ull << jb) - 1));
+        static_for<NQ>([&](auto kc) {
+          constexpr int K = decltype(kc)::value;
+          const unsigned long long lm = lmask_m[K] & in_bat;
+          if (!lm)
+            return;
+          auto next_k = [&](uint32_t c) -> uint32_t {
+            const unsigned long long m = c >= 64 ? 0ull : (lm & (~0ull << c));
+            return m ? (uint32_t) __ffsll((long long) m) - 1 : 64u;
+          };
+          const uint32_t xlo_lo = (uint32_t) r_xlo[K], xlo_hi = (uint32_t) (r_xlo[K] >> 32);
+          uint32_t qh = 0, qt = 0;                   // queue head and tail (mod 128)
+          auto drain_b = [&](uint32_t n) {
+            phase(kHslPhaseBatchDrain);
+            wave_lds_sync();
+            uint32_t e = 0, jl = 0;
+            if (lane < n) {
+              e = lq[(qt + lane) & 127];
+              jl = lqb[(qt + lane) & 127];
             }
-          }
-          if (lane < n)
-            S[0] += (long long) ((e >> 31) ? cnt : -cnt);
-          wave_lds_sync();
-          qt += n;
-          u_qsteps++;
-          phase(kHslPhaseBatchEntry);
-        };
-        // The leaves of the batch's primes with entries in [j, jb]: their
-        // factor entries (or primes) are gathered 64 at a time across the
-        // primes.  The ranges follow each other in prime order (s_ent is the
-        // inclusive scan of the window's entry counts), so entry e of the
-        // batch belongs to the last prime pe whose first entry pst is <= e,
-        // at factor-table (or prime) index lo(pe) + 1 + e - pst.  One load
-        // per lane and step replaces a step per prime; the leaves go to the
-        // queue with their prime's lane.
-        const uint32_t E1 = readlane_u32(s_ent, jb);
-        uint32_t jc = j;                                  // first prime of the step
-        uint32_t jc_st = readlane_u32(s_ent, j) - (readlane_u32(r_hi[0], j) - readlane_u32(r_lo[0], j));
-        {
-          // M of a prime whose leaves are primes: all of them, positive
-          const bool in_b = lane >= j && lane <= jb && ((lmask >> lane) & 1) && !(r_T[0] >> 31);
-          w_m[0] = in_b ? (int) (r_hi[0] - r_lo[0]) : w_m[0];
-          u_lprimes += (uint32_t) __popcll(lmask & (~0ull << j) & (jb >= 63 ? ~0ull : ((2ull << jb) - 1)));
-        }
-        for (uint32_t e0 = jc_st; e0 < E1; e0 += kBlock) {
-          const uint32_t e = e0 + lane;
-          uint32_t pe = jc, pst = jc_st;
-          uint32_t nst = readlane_u32(s_ent, jc);        // first entry of the next prime
-          for (uint32_t jj = next_leaf(jc + 1); jj <= jb && nst < e0 + kBlock;
-               jj = next_leaf(jj + 1)) {
-            if (e >= nst) {
-              pe = jj;
-              pst = nst;
+            // the leaf's prime: x / p and its first record, from its lane
+            const int src = (int) (jl << 2);
+            const uint32_t xl = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_lo);
+            const uint32_t xh = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) xlo_hi);
+            const uint32_t st = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) foff);
+            uint32_t bound = 0;
+            int cnt = 0;
+            if (lane < n) {
+              const uint32_t m = (e << 1) | 1u;
+              cnt = (int) seg_count_b(bm, pre2, lut, leaf_stop(((uint64_t) xh << 32) | xl, low, m),
+                                      bound);
             }
-            nst = readlane_u32(s_ent, jj);
-          }
-          const bool valid = e < E1;
-          const int src = (int) (pe << 2);
-          const uint32_t lo = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_lo[0]);
-          const uint32_t Tt = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_T[0]);
-          const uint32_t idx = lo + 1 + (e - pst);
-          const bool t1 = (Tt >> 31) != 0;
-          uint32_t f = 1, m = 0;                         // a prime leaf: positive
-          if (valid) {
-            if (t1) {
-              f = (uint32_t) factor[idx];
-              const uint32_t qd = idx / 480;
-              m = 2310 * qd + c_cop[idx - 480 * qd];     // to_number(idx)
-            } else {
-              m = primes[idx];
+            // lane 0 holds the earliest prime of the step (the member's queue
+            // is in prime order), so records before its first one count for
+            // no lane
+            const uint32_t e0 = (uint32_t) __builtin_amdgcn_readfirstlane((int) st);
+            for (uint32_t cb = e0; cb < R; cb += kBlock) {
+              const uint32_t ix = cb + lane;
+              const uint32_t rv = ix < R ? (uint32_t) blist[ix] : 0u;
+              // wave-uniform trip count, kept in a scalar register
+              const uint32_t ne = (uint32_t) __builtin_amdgcn_readfirstlane(
+                  (int) min((uint32_t) kBlock, R - cb));
+              for (uint32_t i = 0; i < ne; i++) {
+                const uint32_t v = readlane_u32(rv, i);
+                cnt += (cb + i >= st && v < bound) ? 1 : 0;
+              }
             }
+            if (lane < n)
+              S[K] += (long long) ((e >> 31) ? cnt : -cnt);
+            wave_lds_sync();
+            qt += n;
+            u_qsteps++;
+            phase(kHslPhaseBatchEntry);
+          };
+          // The member's leaves of the batch's primes with entries in [j, jb]:
+          // their factor entries (or primes) are gathered 64 at a time across
+          // the primes.  The ranges follow each other in prime order (s_ent is
+          // the inclusive scan of the window's entry counts), so entry e of the
+          // batch belongs to the last prime pe whose first entry pst is <= e,
+          // at factor-table (or prime) index lo(pe) + 1 + e - pst.  One load
+          // per lane and step replaces a step per prime; the leaves go to the
+          // queue with their prime's lane.
+          const uint32_t j0 = next_k(j);
+          const uint32_t E1 = readlane_u32(s_ent[K], jb);
+          uint32_t jc = j0;                                 // first prime of the step
+          uint32_t jc_st = readlane_u32(s_ent[K], j0) - (readlane_u32(r_hi[K], j0) - readlane_u32(r_lo[K], j0));
+          {
+            // M of a prime whose leaves are primes: all of them, positive
+            const bool in_b = ((lm >> lane) & 1) && !(r_T[K] >> 31);
+            w_m[K] = in_b ? (int) (r_hi[K] - r_lo[K]) : w_m[K];
+            u_lprimes += (uint32_t) __popcll(lm);
           }
-          const bool leaf = valid && (!t1 || f > (Tt & 0x7fffffffu));
-          const unsigned long long lb = __ballot(leaf);
-          if (leaf) {
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
-                (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
-            lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
-            lqb[(qh + rank) & 127] = (uint8_t) pe;
+          for (uint32_t e0 = jc_st; e0 < E1; e0 += kBlock) {
+            const uint32_t e = e0 + lane;
+            uint32_t pe = jc, pst = jc_st;
+            uint32_t nst = readlane_u32(s_ent[K], jc);     // first entry of the next prime
+            for (uint32_t jj = next_k(jc + 1); jj <= jb && nst < e0 + kBlock; jj = next_k(jj + 1)) {
+              if (e >= nst) {
+                pe = jj;
+                pst = nst;
+              }
+              nst = readlane_u32(s_ent[K], jj);
+            }
+            const bool valid = e < E1;
+            const int src = (int) (pe << 2);
+            const uint32_t lo = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_lo[K]);
+            const uint32_t Tt = (uint32_t) __builtin_amdgcn_ds_bpermute(src, (int) r_T[K]);
+            const uint32_t idx = lo + 1 + (e - pst);
+            const bool t1 = (Tt >> 31) != 0;
+            uint32_t f = 1, m = 0;                         // a prime leaf: positive
+            if (valid) {
+              if (t1) {
+                f = (uint32_t) factor[idx];
+                const uint32_t qd = idx / 480;
+                m = 2310 * qd + c_cop[idx - 480 * qd];     // to_number(idx)
+              } else {
+                m = primes[idx];
+              }
+            }
+            const bool leaf = valid && (!t1 || f > (Tt & 0x7fffffffu));
+            const unsigned long long lb = __ballot(leaf);
+            if (leaf) {
+              const uint32_t rank = __builtin_amdgcn_mbcnt_hi(
+                  (uint32_t) (lb >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t) lb, 0u));
+              lq[(qh + rank) & 127] = (m >> 1) | (f << 31);
+              lqb[(qh + rank) & 127] = (uint8_t) pe;
+            }
+            const uint32_t nl = (uint32_t) __popcll(lb);
+            u_entries += (uint32_t) __popcll(__ballot(valid && t1));
+            u_leaves += nl;
+            u_lsteps++;
+            // M of the step's primes with factor entries: 2 (positive leaves) - leaves
+            const uint32_t pe_last = readlane_u32(pe, 63);
+            for (uint32_t jj = jc; jj <= pe_last; jj = next_k(jj + 1)) {
+              const bool mine = leaf && t1 && pe == jj;
+              const int a = (int) __popcll(__ballot(mine));
+              const int pp = (int) __popcll(__ballot(mine && (f & 1)));
+              w_m[K] += lane == jj ? 2 * pp - a : 0;
+            }
+            qh += nl;
+            if (qh - qt >= 64)
+              drain_b(64);
+            jc = pe_last;
+            jc_st = readlane_u32(pst, 63);
           }
-          const uint32_t nl = (uint32_t) __popcll(lb);
-          u_entries += (uint32_t) __popcll(__ballot(valid && t1));
-          u_leaves += nl;
-          u_lsteps++;
-          // M of the step's primes with factor entries: 2 (positive leaves) - leaves
-          const uint32_t pe_last = readlane_u32(pe, 63);
-          for (uint32_t jj = jc; jj <= pe_last; jj = next_leaf(jj + 1)) {
-            const bool mine = leaf && t1 && pe == jj;
-            const int a = (int) __popcll(__ballot(mine));
-            const int pp = (int) __popcll(__ballot(mine && (f & 1)));
-            w_m[0] += lane == jj ? 2 * pp - a : 0;
-          }
-          qh += nl;
-          if (qh - qt >= 64)
-            drain_b(64);
-          jc = pe_last;
-          jc_st = readlane_u32(pst, 63);
-        }
-        if (qh != qt)
-          drain_b(qh - qt);
+          if (qh != qt)
+            drain_b(qh - qt);
+        });
       };
 
 
