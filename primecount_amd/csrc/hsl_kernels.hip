@@ -455,9 +455,7 @@ __device__ __forceinline__ void sieve_segments(
     uint32_t nitems, uint32_t W, uint32_t cbits, uint32_t bcap)
 {
   extern __shared__ __align__(16) uint32_t lds[];
-  // batched leaf counting; the kernel for groups of 2 keeps its 88 VGPRs
-  // (5 waves per SIMD) without it: with it, 98 and 4 waves, 2% slower
-  constexpr bool BATCH = NQ != 2;
+  constexpr bool BATCH = true;                                // batched leaf counting
   uint32_t* bm = lds;                                         // 2W u32
   uint32_t* pre2 = bm + 2 * W;                                // W/2 u32: word-pair prefixes
   uint8_t* lut = reinterpret_cast<uint8_t*>(pre2 + W / 2);    // 240 u8
@@ -1375,6 +1373,15 @@ void hsl_sieve_kernel_w6(PCG_SIEVE_PARAMS)
   sieve_segments<FT, 1>(PCG_SIEVE_ARGS);
 }
 
+/// The kernel for groups of 2 queries compiled for at least 5 waves per SIMD
+/// (at most 96 VGPRs; batched leaf counting takes it to 98 otherwise).
+template <typename FT>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void hsl_sieve_kernel_g2(PCG_SIEVE_PARAMS)
+{
+  sieve_segments<FT, 2>(PCG_SIEVE_ARGS);
+}
+
 /// The single-query kernel with phase timers (HslArgs::timers): the same
 /// sums, plus each wave's clock cycles per phase in counters[16 ..].
 template <typename FT>
@@ -1398,6 +1405,8 @@ static hipError_t launch_sieve_t(const HslArgs& a, uint32_t grid, hipStream_t st
     PCG_SIEVE_LAUNCH((hsl_sieve_kernel_timed<FT>));
   else if (NQ == 1 && a.waves == 6)
     PCG_SIEVE_LAUNCH((hsl_sieve_kernel_w6<FT>));
+  else if (NQ == 2)
+    PCG_SIEVE_LAUNCH((hsl_sieve_kernel_g2<FT>));
   else
     PCG_SIEVE_LAUNCH((hsl_sieve_kernel<FT, NQ>));
 #undef PCG_SIEVE_LAUNCH
