@@ -106,6 +106,14 @@ def test_argument_errors(pc):
     with pytest.raises(pc.PrimecountError):
         pc.set_host_threads(-1)
     pc.set_segment_words(0)
+    # a prime-count table budget below 1 MiB, and n < 1 for the 128-bit
+    # nth_prime, are refused before any device work
+    with pytest.raises(pc.PrimecountError):
+        pc.set_pi_table_bytes(1000)
+    pc.set_pi_table_bytes(0)
+    for bad in (0, -7):
+        with pytest.raises(pc.PrimecountError):
+            pc.nth_prime128(bad)
 
 
 # li(10^n) - pi(10^n) for n = 10 .. 22: the published table of the error of
