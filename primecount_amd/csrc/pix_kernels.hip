@@ -441,31 +441,35 @@ __global__ __launch_bounds__(kThreads) void pix_leaftile_kernel(
         r = __builtin_fma(r, __builtin_fma(-qd, r, 1.0), r);
         double vd[G];
         uint32_t v_lo[G], sh[G];
-        bool ok[G];
-        uint32_t bad = 0;
+        bool ok[G], bad[G];
+        bool bad_any = false;
 #pragma unroll
         for (uint32_t k = 0; k < G; k++) {
           // groups are padded to G descriptors with empty ranges
           const PixTileDesc D = desc[gr.d0 + k];
           ok[k] = FULL || ((idx >= D.i_lo) & (idx <= D.i_hi));
           sh[k] = q > D.q1 ? 1u : 0u;
-          // quotient xp / q.  xd * r is within v 2^-50 of xp / q, so when
-          // its fraction lies in [v 2^-46, 1 - v 2^-46] its integer part is
-          // the quotient; otherwise (rare) the residual fixes it below.  The
-          // low word of vd + 2^52 is the low word of the integer vd.
-          const double pf = D.xd * r;
-          vd[k] = __builtin_trunc(pf);
-          const double h = (pf - vd[k]) - 0.5;
+          // quotient xp / q.  xd * r is within v 2^-50 of xp / q, and
+          // t = xd * r - 1/2 takes one more rounding (below v 2^-53), so
+          // vd = rint(t) is floor(xp / q) when |t - vd| < 1/2 - v 2^-46
+          // (the fraction of xd * r lies in [v 2^-46, 1 - v 2^-46]; t - vd
+          // is exact); otherwise (rare) vd is within one of the quotient
+          // and the residual fixes it below.  The low word of vd + 2^52 is
+          // the low word of the integer vd.
+          const double t = __builtin_fma(D.xd, r, -0.5);
+          vd[k] = __builtin_rint(t);
+          const double h = t - vd[k];
           const double tol = 0.5 - vd[k] * 0x1p-46;
           v_lo[k] = (uint32_t) __builtin_bit_cast(uint64_t, vd[k] + 4503599627370496.0);
-          bad |= (uint32_t) (ok[k] & (__builtin_fabs(h) > tol)) << k;
+          bad[k] = ok[k] & (__builtin_fabs(h) >= tol);
+          bad_any |= bad[k];
         }
-        if (__builtin_amdgcn_ballot_w64(bad != 0) != 0) {
+        if (__builtin_amdgcn_ballot_w64(bad_any) != 0) {
           // the residual xp - s q lies in (-q, 2 q) for the lanes in range;
           // its low 32 bits give it exactly for q < 2^30
 #pragma unroll
           for (uint32_t k = 0; k < G; k++) {
-            if (bad & (1u << k)) {
+            if (bad[k]) {
               const PixTileDesc D = desc[gr.d0 + k];
               const int32_t rem = (int32_t) ((uint32_t) D.xp - v_lo[k] * q);
               const int32_t fix = rem < 0 ? -1 : ((uint32_t) rem >= q ? 1 : 0);
