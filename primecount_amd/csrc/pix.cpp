@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <future>
 #include <map>
 #include <thread>
 
@@ -715,10 +716,11 @@ void add_elapsed(hipEvent_t a, hipEvent_t b, double* sec)
 }
 
 /// pi(x) for the queries xs[idx[j]] > 10^8 whose parameters gps[idx[j]] one
-/// table set serves (view v).  Sigma, Phi0, B and the A/C leaf sums of every
-/// query are launched back to back on the engine stream (the host builds
-/// the A/C descriptor lists meanwhile), then the D leaves of all of them in
-/// one persistent launch; the term sums are read back once.  With win > 0
+/// table set serves (view v).  Sigma, Phi0 and B of every query are launched
+/// back to back on the engine stream, then the D leaves of all of them in
+/// one persistent launch, then the A/C leaf sums, whose descriptor lists a
+/// host thread builds while the device runs the terms before them; the
+/// term sums are read back once.  With win > 0
 /// the prime-count table up to `need` is built in windows of win words
 /// (v's resident table covers the low part only) and B and the A/C sums run
 /// once per window.  Caller holds the engine lock.
@@ -741,6 +743,21 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
   PIX_HIP(hipMemcpyAsync(dn, pts.data(), n * 8, hipMemcpyHostToDevice, v.stream));
   PIX_HIP(pix_launch_pi_points(dn, dpts, (uint32_t) n, v.pix_tab, v.stream));
 
+  // A and C: descriptor lists on host threads while the device runs Sigma,
+  // Phi0, B and D (the lists read only the host prime tables, which stay
+  // as they are while the engine lock is held)
+  const uint64_t max_q = win ? need : v.pix_n;
+  std::future<std::vector<AcWork>> lists =
+      std::async(std::launch::async, [&xs, &idx, &gps, &v, max_q, n] {
+        std::vector<AcWork> W(n);
+        for (size_t j = 0; j < n; j++) {
+          const GourdonParams& gp = gps[idx[j]];
+          W[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host,
+                         max_q);
+        }
+        return W;
+      });
+
   PIX_HIP(hipEventRecord(ev[0], v.stream));
   for (size_t j = 0; j < n; j++) {
     const GourdonParams& gp = gps[idx[j]];
@@ -758,13 +775,22 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
       launch_B(xs[idx[j]], gps[idx[j]].y, v, acc + kTermWords * j + 8);
   PIX_HIP(hipEventRecord(ev[3], v.stream));
 
-  // A and C: descriptor lists on the host while the kernels above run
-  const uint64_t max_q = win ? need : v.pix_n;
-  std::vector<AcWork> L(n);
-  size_t bytes = 0;
+  // D: the hard special leaves of all queries in one persistent launch
+  std::vector<HslQuery> dq(n);
   for (size_t j = 0; j < n; j++) {
     const GourdonParams& gp = gps[idx[j]];
-    L[j] = ac_work(ac_lists(xs[idx[j]], gp.y, gp.z, gp.k, gp.x_star, *v.host), *v.host, max_q);
+    dq[j].kind = LeafSum::D;
+    dq[j].x = xs[idx[j]];
+    dq[j].y = gp.y;
+    dq[j].z = gp.z;
+    dq[j].k = gp.k;
+  }
+  const std::vector<i128> d = hsl_compute_batch_locked(dq, nullptr, max_slots);
+  PIX_HIP(hipEventRecord(ev[4], v.stream));
+
+  std::vector<AcWork> L = lists.get();
+  size_t bytes = 0;
+  for (size_t j = 0; j < n; j++) {
     bytes += L[j].bytes();
     s.ac_leaves += L[j].leaves;
   }
@@ -790,19 +816,6 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
         dd += L[j].bytes();
       }
     });
-  PIX_HIP(hipEventRecord(ev[4], v.stream));
-
-  // D: the hard special leaves of all queries in one persistent launch
-  std::vector<HslQuery> dq(n);
-  for (size_t j = 0; j < n; j++) {
-    const GourdonParams& gp = gps[idx[j]];
-    dq[j].kind = LeafSum::D;
-    dq[j].x = xs[idx[j]];
-    dq[j].y = gp.y;
-    dq[j].z = gp.z;
-    dq[j].k = gp.k;
-  }
-  const std::vector<i128> d = hsl_compute_batch_locked(dq, nullptr, max_slots);
   PIX_HIP(hipEventRecord(ev[5], v.stream));
 
   std::vector<uint64_t> h(n * kTermWords), pis(n);
@@ -812,8 +825,8 @@ void pi_group(uint64_t max_slots, const std::vector<i128>& xs, const std::vector
   add_elapsed(ev[0], ev[1], &s.seconds_sigma);
   add_elapsed(ev[1], ev[2], &s.seconds_phi0);
   add_elapsed(ev[2], ev[3], &s.seconds_b);
-  add_elapsed(ev[3], ev[4], &s.seconds_ac);
-  add_elapsed(ev[4], ev[5], &s.seconds_d);
+  add_elapsed(ev[3], ev[4], &s.seconds_d);
+  add_elapsed(ev[4], ev[5], &s.seconds_ac);
   const PrimeTables& P = *v.host;
   for (size_t j = 0; j < n; j++) {
     const i128 x = xs[idx[j]];
