@@ -1111,9 +1111,9 @@ i128 run_s1(i128 x, int64_t y, int64_t c, const EngineView& v)
   return phi_tiny_host(x, t) + acc_value(A.h);
 }
 
-i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
+/// S2_easy from its descriptor lists W (built by ac_work over s2_easy_lists).
+i128 run_s2_easy_lists(const AcWork& W, const EngineView& v)
 {
-  const AcWork W = ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host, v.pix_n);
   Acc A;
   A.d = (unsigned long long*) engine_buffer(0, 128);
   acc_zero(A, v.stream);
@@ -1121,6 +1121,11 @@ i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
   launch_ac(W, v, dd, A.d);
   acc_read(A, v.stream);
   return ac_value(W, A.h);
+}
+
+i128 run_s2_easy(i128 x, int64_t y, int64_t z, int64_t c, const EngineView& v)
+{
+  return run_s2_easy_lists(ac_work(s2_easy_lists(x, y, z, c, *v.host), *v.host, v.pix_n), v);
 }
 
 /// S2_easy - P2 with the prime-count table up to `need` built in windows of
@@ -1208,13 +1213,14 @@ i128 pi_deleglise_rivat(i128 x)
   if ((uint64_t) p.y > v.host->max) throw pcg_error("internal: prime table below y", kErrInternal);
   i128 r = run_s1(x, p.y, p.c, v);
   r += s2_trivial_host(x, p.y, p.z, p.c, *v.host);
-  if (win) {
-    r += s2_easy_minus_p2_windowed(x, p.y, p.z, p.c, v, need, win);
-  } else {
-    r += run_s2_easy(x, p.y, p.z, p.c, v);
-    r -= run_p2(x, p.y, v);
-  }
-  r += (i128) v.host->pi((uint64_t) p.y) - 1;
+  // without windows the S2_easy descriptor lists are built on host threads
+  // while the device runs S2_hard and P2 (they read only the host prime
+  // tables, which stay as they are while the engine lock is held)
+  std::future<AcWork> easy;
+  if (!win)
+    easy = std::async(std::launch::async, [&x, &p, &v] {
+      return ac_work(s2_easy_lists(x, p.y, p.z, p.c, *v.host), *v.host, v.pix_n);
+    });
   HslQuery q;
   q.kind = LeafSum::S2_hard;
   q.x = x;
@@ -1222,6 +1228,14 @@ i128 pi_deleglise_rivat(i128 x)
   q.z = p.z;
   q.k = p.c;
   r += hsl_compute_batch_locked({q}, nullptr)[0];
+  if (win) {
+    r += s2_easy_minus_p2_windowed(x, p.y, p.z, p.c, v, need, win);
+  } else {
+    r -= run_p2(x, p.y, v);
+    const AcWork W = easy.get();
+    r += run_s2_easy_lists(W, v);
+  }
+  r += (i128) v.host->pi((uint64_t) p.y) - 1;
   // src/deleglise-rivat/pi_deleglise_rivat.cpp:92,138
   verify_pix("pi_deleglise_rivat", x, r);
   return r;
